@@ -1,0 +1,289 @@
+"""finess_amd -- MI355X (gfx950) implementation of FINESS's finite-difference
+WENO operator and its time steppers, driven through the C-ABI of
+``libfiness_hip.so`` (declared in ``include/finess_hip.h``).
+
+This module is the Python mirror of the reference's operator interface:
+``Solver.construct_l`` is ``ConstructL`` (lib/2d/ConstructL.cpp:13,
+lib/1d/ConstructL.cpp:16), ``Solver.fin_solve_rk`` is ``FinSolveRK``
+(lib/2d/FinSolveRK.cpp:13) and ``Solver.fin_solve_lxw`` is ``FinSolveLxW``
+(lib/2d/FinSolveLxW.cpp:16), with the reference's argument meaning: state
+arrays in the reference tensor layout with ghost cells, ``dtv`` a 3-vector
+whose entry 1 is the in/out time step and entry 2 the maximum time step.
+
+Errors: the reference exits (``exit(1)``) or throws; here every failing call
+raises :class:`FinessError` carrying the C status code.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfiness_hip.so")
+
+# status codes (include/finess_hip.h)
+OK, ERR_CONFIG, ERR_NONFINITE, ERR_NV, ERR_HIP, ERR_DT, ERR_CFL, ERR_ARG, ERR_NODEVICE = 0, -1, -2, -3, -4, -5, -6, -7, -8
+WENO_JS, WENO_FD, WENO_Z = 0, 1, 2
+TS_RK, TS_LXW = 0, 1
+PHYS_ADVECTION, PHYS_EULER, PHYS_SHALLOW_WATER = 0, 1, 2
+BC_EXTRAP, BC_PERIODIC, BC_REFLECT = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+ROUTE_NONE, ROUTE_LEARNED, ROUTE_ORACLE = 0, 1, 2
+
+_ERRNAMES = {
+    ERR_CONFIG: "unsupported configuration",
+    ERR_NONFINITE: "non-finite state",
+    ERR_NV: "exceeded allowed # of time steps (nv)",
+    ERR_HIP: "HIP runtime error",
+    ERR_DT: "dt is out of bounds (dt > max_dt)",
+    ERR_CFL: "CFL number is out of bounds",
+    ERR_ARG: "bad argument",
+    ERR_NODEVICE: "no GPU device",
+}
+
+
+class FinessError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__("finess error %d (%s)%s" % (code, _ERRNAMES.get(code, "?"), (": " + msg) if msg else ""))
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``finess_params`` (include/finess_hip.h)."""
+
+    _fields_ = [
+        ("ndims", ctypes.c_int),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("mbc", ctypes.c_int),
+        ("xlow", ctypes.c_double), ("xhigh", ctypes.c_double),
+        ("ylow", ctypes.c_double), ("yhigh", ctypes.c_double),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("meqn", ctypes.c_int), ("maux", ctypes.c_int),
+        ("space_order", ctypes.c_int), ("time_order", ctypes.c_int),
+        ("time_stepping_method", ctypes.c_int),
+        ("initial_dt", ctypes.c_double), ("max_dt", ctypes.c_double),
+        ("desired_cfl", ctypes.c_double), ("max_cfl", ctypes.c_double),
+        ("nv", ctypes.c_int),
+        ("global_alpha", ctypes.c_int), ("source_term", ctypes.c_int), ("verbosity", ctypes.c_int),
+        ("weno_version", ctypes.c_int),
+        ("epsilon", ctypes.c_double), ("power_param", ctypes.c_double), ("alpha_scaling", ctypes.c_double),
+        ("physics", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("sw_compat_hl", ctypes.c_int),
+        ("bc", ctypes.c_int * 4),
+        ("precision", ctypes.c_int),
+        ("routing", ctypes.c_int),
+        ("route_tau", ctypes.c_double),
+        ("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int),
+    ]
+
+    def copy(self):
+        p = Params()
+        ctypes.pointer(p)[0] = self
+        return p
+
+    def shape_q(self):
+        """Shape of a reference-layout state array (ghosts included)."""
+        if self.ndims == 1:
+            return (self.mx + 2 * self.mbc, self.meqn)
+        return (self.mx + 2 * self.mbc, self.my + 2 * self.mbc, self.meqn)
+
+    def shape_aux(self):
+        na = max(1, self.maux)
+        if self.ndims == 1:
+            return (self.mx + 2 * self.mbc, na)
+        return (self.mx + 2 * self.mbc, self.my + 2 * self.mbc, na)
+
+    def shape_smax(self):
+        if self.ndims == 1:
+            return (self.mx + 2 * self.mbc,)
+        return (self.mx + 1 + 2 * self.mbc, self.my + 1 + 2 * self.mbc, 2)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("steps_accepted", ctypes.c_longlong),
+        ("steps_rejected", ctypes.c_longlong),
+        ("dt_min", ctypes.c_double),
+        ("dt_max", ctypes.c_double),
+        ("last_cfl", ctypes.c_double),
+        ("t", ctypes.c_double),
+    ]
+
+
+_lib = None
+_PD = ctypes.POINTER(ctypes.c_double)
+
+# every symbol include/finess_hip.h declares, with its ctypes signature
+SIGNATURES = {
+    "finess_params_default": (None, [ctypes.POINTER(Params)]),
+    "finess_params_finalize": (ctypes.c_int, [ctypes.POINTER(Params)]),
+    "finess_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    "finess_destroy": (None, [ctypes.c_void_p]),
+    "finess_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "finess_set_state": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, ctypes.c_double]),
+    "finess_get_state": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD]),
+    "finess_construct_l": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, _PD]),
+    "finess_fin_solve_rk": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
+    "finess_fin_solve_lxw": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
+    "finess_construct_integrated_r": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD]),
+    "finess_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, ctypes.POINTER(Stats)]),
+    "finess_advance_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _PD, ctypes.POINTER(Stats)]),
+    "finess_run_fixed_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _PD]),
+    "finess_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "finess_last_kernel_time": (ctypes.c_int, [ctypes.c_void_p, _PD, ctypes.POINTER(ctypes.c_longlong)]),
+    "finess_version": (ctypes.c_char_p, []),
+    "finess_device_count": (ctypes.c_int, []),
+}
+
+
+def load_library(path=None):
+    """Loads ``libfiness_hip.so``.  Raises if it has not been built: there is
+    no CPU fallback for any operator of this package."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FinessError(ERR_ARG, "%s is missing; build it with `python -m finess_amd.build`" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def default_params(**kw):
+    """``finess_params_default`` then keyword overrides, then finalize."""
+    lib = load_library()
+    p = Params()
+    lib.finess_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "bc":
+            for i, b in enumerate(v):
+                p.bc[i] = b
+        else:
+            setattr(p, k, v)
+    rc = lib.finess_params_finalize(ctypes.byref(p))
+    if rc:
+        raise FinessError(rc, "parameters rejected")
+    return p
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PD)
+
+
+class Solver:
+    """One device context (``finess_ctx``) bound to a parameter set."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.p = params.copy()
+        h = ctypes.c_void_p()
+        rc = self.lib.finess_create(ctypes.byref(self.p), ctypes.byref(h))
+        if rc:
+            raise FinessError(rc, "finess_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.finess_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.finess_last_error(self.h)
+            raise FinessError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    # ---- state -------------------------------------------------------
+    def set_state(self, q, aux=None, t=0.0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if aux is not None:
+            aux = np.ascontiguousarray(aux, dtype=np.float64)
+        self._check(self.lib.finess_set_state(self.h, _ptr(q), _ptr(aux), float(t)), "set_state")
+
+    def get_state(self):
+        q = np.zeros(self.p.shape_q())
+        t = ctypes.c_double()
+        self._check(self.lib.finess_get_state(self.h, _ptr(q), ctypes.byref(t)), "get_state")
+        return q, t.value
+
+    # ---- drop-in operators ---------------------------------------------
+    def construct_l(self, q, aux=None, smax=None):
+        """ConstructL: returns (Lstar, smax) in the reference layout."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if aux is None:
+            aux = np.ones(self.p.shape_aux())
+        aux = np.ascontiguousarray(aux, dtype=np.float64)
+        L = np.zeros(self.p.shape_q())
+        sm = np.zeros(self.p.shape_smax()) if smax is None else np.ascontiguousarray(smax, dtype=np.float64)
+        self._check(self.lib.finess_construct_l(self.h, _ptr(q), _ptr(aux), _ptr(L), _ptr(sm)), "construct_l")
+        return L, sm
+
+    def construct_integrated_r(self, dt, q, aux=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        F = np.zeros(self.p.shape_q())
+        G = np.zeros(self.p.shape_q())
+        self._check(self.lib.finess_construct_integrated_r(self.h, float(dt), _ptr(q), None, _ptr(F), _ptr(G)),
+                    "construct_integrated_r")
+        return F, G
+
+    def fin_solve_rk(self, q, t, tend, dtv, aux=None):
+        """FinSolveRK: advances q in place from t to tend; returns t."""
+        return self._fin_solve(self.lib.finess_fin_solve_rk, q, t, tend, dtv, aux)
+
+    def fin_solve_lxw(self, q, t, tend, dtv, aux=None):
+        return self._fin_solve(self.lib.finess_fin_solve_lxw, q, t, tend, dtv, aux)
+
+    def _fin_solve(self, fn, q, t, tend, dtv, aux):
+        assert q.dtype == np.float64 and q.flags["C_CONTIGUOUS"]
+        assert dtv.dtype == np.float64 and dtv.shape == (3,)
+        tt = ctypes.c_double(t)
+        if aux is not None:
+            aux = np.ascontiguousarray(aux, dtype=np.float64)
+        self._check(fn(self.h, _ptr(q), _ptr(aux), ctypes.byref(tt), float(tend), _ptr(dtv)), "fin_solve")
+        return tt.value
+
+    # ---- device-resident stepping ---------------------------------------
+    def advance(self, tend, dtv):
+        st = Stats()
+        self._check(self.lib.finess_advance(self.h, float(tend), _ptr(dtv), ctypes.byref(st)), "advance")
+        return st
+
+    def advance_steps(self, nsteps, dtv):
+        st = Stats()
+        self._check(self.lib.finess_advance_steps(self.h, int(nsteps), _ptr(dtv), ctypes.byref(st)), "advance_steps")
+        return st
+
+    def run_fixed_steps(self, dt, nsteps):
+        cfl = ctypes.c_double()
+        self._check(self.lib.finess_run_fixed_steps(self.h, float(dt), int(nsteps), ctypes.byref(cfl)),
+                    "run_fixed_steps")
+        return cfl.value
+
+    def sync(self):
+        self._check(self.lib.finess_sync(self.h), "sync")
+
+    def last_kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self.lib.finess_last_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)), "last_kernel_time")
+        return ms.value, n.value

@@ -1,0 +1,766 @@
+// capi.cpp -- host controller and C-ABI (include/finess_hip.h).
+//
+// The state lives on the device as field planes q[m][i][j] (interior cells
+// only; the halo is produced by each stage kernel through the boundary map).
+// SSP-RK3 uses three plane sets A, B, C with A = q^n never written during a
+// step:
+//     stage 1: B = A + dt L(A)
+//     stage 2: C = 3/4 A + 1/4 B + 1/4 dt L(B)
+//     stage 3: B = 2/3 C + 1/3 A + 2/3 dt L(C)
+// (coefficients of lib/2d/SetRKinfo.cpp:50-64, operand order of
+// lib/2d/FinSolveRK.cpp:158-211), so a rejected step needs no copy back:
+// the controller simply retries from A.  Accepting a step swaps the roles.
+//
+// The time-step controller is the reference's a-posteriori one
+// (lib/2d/FinSolveRK.cpp:77-367, lib/1d/FinSolveRK.cpp:88-363,
+// lib/2d/FinSolveLxW.cpp:100-229): frame clipping, CFL from the per-face wave
+// speeds of the step just taken, dt = min(max_dt, dt*desired_cfl/cfl),
+// reject when cfl > max_cfl.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/finess_hip.h"
+#include "kernels.h"
+#include "lxw_body.h"
+#include "stage1d_body.h"
+#include "stage2d_body.h"
+
+using namespace finess;
+
+struct finess_ctx {
+    finess_params p;
+    int device = 0;
+    hipStream_t st = nullptr;
+    int M = 0;
+    long long plane = 0;        // cells of one field plane
+    double* buf[3] = {nullptr, nullptr, nullptr};
+    double* aux = nullptr;      // 1D advection speed (interior cells)
+    double* lxwF = nullptr;
+    double* lxwG = nullptr;
+    double* staging = nullptr;  // reference layout with ghosts
+    double* ghosted = nullptr;  // field planes with ghosts (drop-in operators)
+    double* faces_x = nullptr;  // per-face wave speeds (drop-in ConstructL)
+    double* faces_y = nullptr;
+    double* fhat1d = nullptr;
+    unsigned long long* smax = nullptr;    // device CFL slots [8][2]
+    unsigned long long* smax_h = nullptr;  // pinned copy
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double t = 0.0;
+    int cur = 0;
+    bool have_state = false;
+    double last_ms = 0.0;
+    long long last_launches = 0;
+    std::string err;
+};
+
+namespace {
+
+constexpr int NSLOT = 8;
+
+int fail(finess_ctx* c, int code, const char* msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) {                                                             \
+            if (ctx) (ctx)->err = std::string(#x ": ") + hipGetErrorString(e_);            \
+            return FINESS_ERR_HIP;                                                          \
+        }                                                                                   \
+    } while (0)
+
+int weno_kind(const finess_params& p)
+{
+    if (p.weno_version == FINESS_WENO_FD) return WENO_FD5;
+    if (p.weno_version == FINESS_WENO_Z) return WENO_Z5;
+    return p.power_param == 2.0 ? WENO_JS5_P2 : WENO_JS5_POW;
+}
+
+size_t ref_numel(const finess_params& p, int fields)
+{
+    if (p.ndims == 1) return (size_t)(p.mx + 2 * p.mbc) * fields;
+    return (size_t)(p.mx + 2 * p.mbc) * (p.my + 2 * p.mbc) * fields;
+}
+
+int normal_x(const finess_params& p) { return p.physics == FINESS_PHYS_ADVECTION ? -1 : 1; }
+int normal_y(const finess_params& p) { return p.physics == FINESS_PHYS_ADVECTION ? -1 : 2; }
+
+double bits_to_double(unsigned long long b)
+{
+    double d;
+    memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+StageArgs base_stage_args(finess_ctx* c)
+{
+    StageArgs a;
+    memset(&a, 0, sizeof a);
+    const finess_params& p = c->p;
+    a.mx = p.mx;
+    a.my = p.my;
+    a.plane = c->plane;
+    for (int k = 0; k < 4; k++) a.bc[k] = p.bc[k];
+    a.dx = p.dx;
+    a.dy = p.dy;
+    a.alpha_scaling = p.alpha_scaling;
+    a.eps = p.epsilon;
+    a.power = p.power_param;
+    a.pp.gamma = p.gamma;
+    a.pp.sw_compat_hl = p.sw_compat_hl;
+    return a;
+}
+
+Stage1DArgs base_stage1d_args(finess_ctx* c)
+{
+    Stage1DArgs a;
+    memset(&a, 0, sizeof a);
+    const finess_params& p = c->p;
+    a.mx = p.mx;
+    a.bc_lo = p.bc[0];
+    a.bc_hi = p.bc[1];
+    a.dx = p.dx;
+    a.alpha_scaling = p.alpha_scaling;
+    a.eps = p.epsilon;
+    a.power = p.power_param;
+    a.aux = c->aux;
+    a.fhat = c->fhat1d;
+    return a;
+}
+
+// One stage: out = a1*qa + a2*qb + (beta*dt) L(src).  slot: CFL slot.
+int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2, double beta, double dt, int slot)
+{
+    const finess_params& p = c->p;
+    if (p.ndims == 1) {
+        Stage1DArgs a = base_stage1d_args(c);
+        a.src = c->buf[src];
+        a.qa = c->buf[qa];
+        a.qb = qb >= 0 ? c->buf[qb] : nullptr;
+        a.use_b = (qb >= 0 && a2 != 0.0);
+        a.out = c->buf[out];
+        a.a1 = a1;
+        a.a2 = a2;
+        a.bdt = beta * dt;
+        a.smax_slot = c->smax + 2 * slot;
+        const int rc = launch_stage1d(weno_kind(p), a, c->st);
+        return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D stage launch failed") : 0;
+    }
+    StageArgs a = base_stage_args(c);
+    a.src = c->buf[src];
+    a.qa = c->buf[qa];
+    a.a_is_src = (qa == src);
+    a.qb = qb >= 0 ? c->buf[qb] : nullptr;
+    a.b_is_src = (qb == src);
+    a.use_b = (qb >= 0 && a2 != 0.0);
+    a.out = c->buf[out];
+    a.a1 = a1;
+    a.a2 = a2;
+    a.bdt = beta * dt;
+    a.smax = c->smax + 2 * slot;
+    const bool lxw = p.time_stepping_method == FINESS_TS_LXW;
+    if (lxw) {
+        a.lxw_F = c->lxwF;
+        a.lxw_G = c->lxwG;
+    }
+    const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), lxw, a, c->st);
+    return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
+}
+
+int lxw_fluxes(finess_ctx* c, int src, double dt)
+{
+    const finess_params& p = c->p;
+    LxwArgs a;
+    memset(&a, 0, sizeof a);
+    a.q = c->buf[src];
+    a.F = c->lxwF;
+    a.G = c->lxwG;
+    a.mx = p.mx;
+    a.my = p.my;
+    a.plane = c->plane;
+    for (int k = 0; k < 4; k++) a.bc[k] = p.bc[k];
+    a.time_order = p.time_order;
+    a.dt = dt;
+    a.dx = p.dx;
+    a.dy = p.dy;
+    a.pp.gamma = p.gamma;
+    a.pp.sw_compat_hl = p.sw_compat_hl;
+    const int rc = launch_lxw_flux(p.physics, a, c->st);
+    return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "LxW flux launch failed") : 0;
+}
+
+// Enqueues one full time step from buf[cur]; returns the index holding the
+// result in *res and the slot holding the CFL speeds in *slot.
+int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
+{
+    const finess_params& p = c->p;
+    const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
+    int rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st);
+    if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
+    if (p.time_stepping_method == FINESS_TS_LXW) {
+        if ((rc = lxw_fluxes(c, A, dt))) return rc;
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        *res = B;
+        *slot = 0;
+        return 0;
+    }
+    const bool one_d = p.ndims == 1;  // 1D: smax overwritten per stage
+    switch (p.time_order) {
+    case 1:
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        *res = B;
+        *slot = 0;
+        return 0;
+    case 2:
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        if ((rc = stage(c, B, B, A, C, 0.5, 0.5, 0.5, dt, one_d ? 1 : 0))) return rc;
+        *res = C;
+        *slot = one_d ? 1 : 0;
+        return 0;
+    case 3:
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        if ((rc = stage(c, B, A, B, C, 0.75, 0.25, 0.25, dt, one_d ? 1 : 0))) return rc;
+        if ((rc = stage(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, one_d ? 2 : 0))) return rc;
+        *res = B;
+        *slot = one_d ? 2 : 0;
+        return 0;
+    default:
+        return fail(c, FINESS_ERR_CONFIG, "time_order not supported");
+    }
+}
+
+// GetCFL from the slot's maxima (lib/2d/GetCFL.cpp:31-74, lib/1d/GetCFL.cpp:9-45).
+int read_cfl(finess_ctx* c, int slot, double dt, double dtmax, double* cfl_out)
+{
+    HIPCHK(c, hipMemcpyAsync(c->smax_h, c->smax + 2 * slot, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const finess_params& p = c->p;
+    const double sx = bits_to_double(c->smax_h[0]), sy = bits_to_double(c->smax_h[1]);
+    if (dt > dtmax) return fail(c, FINESS_ERR_DT, "dt is out of bounds (dt > max_dt)");
+    if (!isfinite(sx) || !isfinite(sy)) return fail(c, FINESS_ERR_NONFINITE, "non-finite wave speed");
+    double cfl = -100.0;
+    if (p.ndims == 1) {
+        cfl = rmax(dt * sx / p.dx, cfl);
+        *cfl_out = cfl;
+        if (cfl > 1.0e10) return fail(c, FINESS_ERR_CFL, "CFL number is out of bounds");
+        return 0;
+    }
+    cfl = rmax(dt * sx / p.dx, cfl);
+    cfl = rmax(dt * sy / p.dy, cfl);
+    *cfl_out = cfl;
+    if (cfl > 1.0e8) return fail(c, FINESS_ERR_CFL, "CFL number is out of bounds");
+    return 0;
+}
+
+int alloc(finess_ctx* c, void** ptr, size_t bytes)
+{
+    HIPCHK(c, hipMalloc(ptr, bytes));
+    return 0;
+}
+
+int ensure_ghosted(finess_ctx* c)
+{
+    const finess_params& p = c->p;
+    if (!c->ghosted) {
+        int rc;
+        if ((rc = alloc(c, (void**)&c->ghosted, ref_numel(p, c->M) * sizeof(double)))) return rc;
+    }
+    return 0;
+}
+
+// Uploads a reference-layout host array with ghosts into c->ghosted (planes
+// with ghosts, dg = mbc).
+int upload_ghosted(finess_ctx* c, const double* q)
+{
+    const finess_params& p = c->p;
+    int rc;
+    if ((rc = ensure_ghosted(c))) return rc;
+    const size_t n = ref_numel(p, c->M);
+    if (p.ndims == 1) {
+        std::vector<double> tmp(n);
+        const int L = p.mx + 2 * p.mbc;
+        for (int i = 0; i < L; i++)
+            for (int m = 0; m < c->M; m++) tmp[(size_t)m * L + i] = q[(size_t)i * c->M + m];
+        HIPCHK(c, hipMemcpyAsync(c->ghosted, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        return 0;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->staging, q, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (launch_ref_to_planes(c->staging, c->ghosted, p.mx, p.my, c->M, p.mbc, p.mbc, c->st))
+        return fail(c, FINESS_ERR_HIP, "layout conversion failed");
+    return 0;
+}
+
+} // namespace
+
+// ======================================================================
+// C-ABI
+// ======================================================================
+extern "C" {
+
+void finess_params_default(finess_params* p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->ndims = 2;
+    p->mx = p->my = 64;
+    p->mbc = 3;
+    p->xlow = p->ylow = 0.0;
+    p->xhigh = p->yhigh = 1.0;
+    p->meqn = 5;
+    p->maux = 0;
+    p->space_order = 5;
+    p->time_order = 3;
+    p->time_stepping_method = FINESS_TS_RK;
+    p->initial_dt = 1.0e-3;
+    p->max_dt = 1.0e10;
+    p->desired_cfl = 0.4;
+    p->max_cfl = 0.45;
+    p->nv = 1000000;
+    p->weno_version = FINESS_WENO_JS;   // python/finess/params/weno.py:15-51
+    p->epsilon = 1.0e-6;
+    p->power_param = 2.0;
+    p->alpha_scaling = 1.1;
+    p->physics = FINESS_PHYS_EULER;
+    p->gamma = 1.4;                      // python/finess/params/euler.py:13-19
+    p->sw_compat_hl = 1;
+    for (int k = 0; k < 4; k++) p->bc[k] = FINESS_BC_EXTRAP;
+    p->precision = FINESS_PREC_FP64;
+    p->routing = FINESS_ROUTE_NONE;
+    p->route_tau = 0.05;
+    finess_params_finalize(p);
+}
+
+int finess_params_finalize(finess_params* p)
+{
+    if (!p) return FINESS_ERR_ARG;
+    if (p->mx < 1 || (p->ndims == 2 && p->my < 1)) return FINESS_ERR_CONFIG;
+    p->dx = (p->xhigh - p->xlow) / p->mx;
+    p->dy = p->ndims == 2 ? (p->yhigh - p->ylow) / p->my : 1.0;
+    // checks of the reference's parameter schema (python/finess/params/*.py)
+    if (!(p->nv > 0) || !(p->initial_dt > 0.0) || !(p->max_dt >= p->initial_dt)) return FINESS_ERR_CONFIG;
+    if (!(p->desired_cfl > 0.0) || !(p->max_cfl >= p->desired_cfl)) return FINESS_ERR_CONFIG;
+    if (!(p->alpha_scaling >= 1.0) || !(p->epsilon > 0.0)) return FINESS_ERR_CONFIG;
+    // what the device path implements (GetWenoReconstruct throws for the rest)
+    if (p->space_order != 5) return FINESS_ERR_CONFIG;
+    if (p->weno_version != FINESS_WENO_JS && p->weno_version != FINESS_WENO_FD && p->weno_version != FINESS_WENO_Z)
+        return FINESS_ERR_CONFIG;
+    if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
+    if (p->ndims == 1) {
+        if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;
+        if (p->time_stepping_method != FINESS_TS_RK) return FINESS_ERR_CONFIG;
+        if (p->mx < 3) return FINESS_ERR_CONFIG;
+    } else if (p->ndims == 2) {
+        if (p->physics == FINESS_PHYS_EULER && p->meqn != 5) return FINESS_ERR_CONFIG;
+        if (p->physics == FINESS_PHYS_SHALLOW_WATER && p->meqn != 3) return FINESS_ERR_CONFIG;
+        if (p->physics == FINESS_PHYS_ADVECTION) return FINESS_ERR_CONFIG;
+        if (p->mx < 5 || p->my < 5) return FINESS_ERR_CONFIG;
+    } else {
+        return FINESS_ERR_CONFIG;
+    }
+    if (p->mbc < 3) return FINESS_ERR_CONFIG;
+    // the boundary map folds ghost indices g in [-n, 2n-1] only
+    if (p->mbc > p->mx || (p->ndims == 2 && p->mbc > p->my)) return FINESS_ERR_CONFIG;
+    if (p->time_stepping_method == FINESS_TS_LXW) {
+        if (p->ndims != 2 || p->mbc < 5 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
+        if (p->time_order < 2 || p->time_order > 3) return FINESS_ERR_CONFIG;
+    } else if (p->time_order < 1 || p->time_order > 3) {
+        return FINESS_ERR_CONFIG;
+    }
+    if (p->precision == FINESS_PREC_FP32 && weno_kind(*p) != WENO_JS5_P2) return FINESS_ERR_CONFIG;
+    for (int k = 0; k < 4; k++)
+        if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_REFLECT) return FINESS_ERR_CONFIG;
+    return FINESS_OK;
+}
+
+const char* finess_version(void) { return "finess-hip 0.1 (gfx950)"; }
+
+int finess_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int finess_create(const finess_params* pin, finess_ctx** out)
+{
+    if (!pin || !out) return FINESS_ERR_ARG;
+    *out = nullptr;
+    finess_params p = *pin;
+    int rc = finess_params_finalize(&p);
+    if (rc) return rc;
+    if (finess_device_count() < 1) return FINESS_ERR_NODEVICE;
+    finess_ctx* c = new finess_ctx();
+    c->p = p;
+    c->M = p.meqn;
+    c->plane = p.ndims == 1 ? (long long)p.mx : (long long)p.mx * p.my;
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        finess_destroy(c);
+        return FINESS_ERR_HIP;
+    }
+    const size_t sbytes = (size_t)c->M * c->plane * sizeof(double);
+    for (int k = 0; k < 3 && !rc; k++) rc = alloc(c, (void**)&c->buf[k], sbytes);
+    if (!rc) rc = alloc(c, (void**)&c->smax, 2 * NSLOT * sizeof(unsigned long long));
+    if (!rc && hipHostMalloc((void**)&c->smax_h, 2 * NSLOT * sizeof(unsigned long long), hipHostMallocDefault) !=
+                   hipSuccess)
+        rc = FINESS_ERR_HIP;
+    if (!rc) rc = alloc(c, (void**)&c->staging, ref_numel(p, c->M) * sizeof(double));
+    if (!rc && p.ndims == 1) {
+        rc = alloc(c, (void**)&c->aux, (size_t)p.mx * sizeof(double));
+        if (!rc) rc = alloc(c, (void**)&c->fhat1d, (size_t)(p.mx + 1) * sizeof(double));
+        if (!rc) {
+            std::vector<double> one((size_t)p.mx, 1.0);  // AuxFunc default: speed 1
+            if (hipMemcpy(c->aux, one.data(), p.mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                rc = FINESS_ERR_HIP;
+        }
+    }
+    if (!rc && p.time_stepping_method == FINESS_TS_LXW) {
+        const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6) * sizeof(double);
+        rc = alloc(c, (void**)&c->lxwF, lb);
+        if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
+    }
+    if (!rc) rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
+    if (rc) {
+        finess_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FINESS_OK;
+}
+
+void finess_destroy(finess_ctx* c)
+{
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (int k = 0; k < 3; k++) hipFree(c->buf[k]);
+    hipFree(c->aux);
+    hipFree(c->lxwF);
+    hipFree(c->lxwG);
+    hipFree(c->staging);
+    hipFree(c->ghosted);
+    hipFree(c->faces_x);
+    hipFree(c->faces_y);
+    hipFree(c->fhat1d);
+    hipFree(c->smax);
+    if (c->smax_h) hipHostFree(c->smax_h);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* finess_last_error(const finess_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int finess_set_state(finess_ctx* c, const double* q, const double* aux, double t)
+{
+    if (!c || !q) return FINESS_ERR_ARG;
+    const finess_params& p = c->p;
+    const size_t n = ref_numel(p, c->M);
+    c->cur = 0;
+    if (p.ndims == 1) {
+        std::vector<double> tmp((size_t)p.mx);
+        for (int i = 0; i < p.mx; i++) tmp[i] = q[(size_t)(i + p.mbc) * c->M];
+        HIPCHK(c, hipMemcpy(c->buf[0], tmp.data(), p.mx * sizeof(double), hipMemcpyHostToDevice));
+        if (aux) {
+            const int na = p.maux > 0 ? p.maux : 1;
+            for (int i = 0; i < p.mx; i++) tmp[i] = aux[(size_t)(i + p.mbc) * na];
+            HIPCHK(c, hipMemcpy(c->aux, tmp.data(), p.mx * sizeof(double), hipMemcpyHostToDevice));
+        }
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->staging, q, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        if (launch_ref_to_planes(c->staging, c->buf[0], p.mx, p.my, c->M, p.mbc, 0, c->st))
+            return fail(c, FINESS_ERR_HIP, "layout conversion failed");
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    c->t = t;
+    c->have_state = true;
+    return FINESS_OK;
+}
+
+int finess_get_state(finess_ctx* c, double* q, double* t)
+{
+    if (!c || !q) return FINESS_ERR_ARG;
+    if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
+    const finess_params& p = c->p;
+    const size_t n = ref_numel(p, c->M);
+    if (p.ndims == 1) {
+        std::vector<double> tmp((size_t)p.mx);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), c->buf[c->cur], p.mx * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int i = -p.mbc; i < p.mx + p.mbc; i++) {
+            int fl;
+            q[(size_t)(i + p.mbc) * c->M] = tmp[bc_map(p.bc[0], p.bc[1], i, p.mx, fl)];
+        }
+    } else {
+        if (launch_planes_to_ref(c->buf[c->cur], c->staging, p.mx, p.my, c->M, p.mbc, p.bc, normal_x(p), normal_y(p),
+                                 c->st))
+            return fail(c, FINESS_ERR_HIP, "layout conversion failed");
+        HIPCHK(c, hipMemcpyAsync(q, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    if (t) *t = c->t;
+    return FINESS_OK;
+}
+
+int finess_construct_l(finess_ctx* c, const double* q, const double* aux, double* Lstar, double* smax)
+{
+    if (!c || !q || !Lstar || !smax) return FINESS_ERR_ARG;
+    const finess_params& p = c->p;
+    int rc;
+    if ((rc = upload_ghosted(c, q))) return rc;
+    double* Ld = c->buf[(c->cur + 2) % 3];  // scratch: never the current state
+    if (p.ndims == 1) {
+        if (!c->faces_x && (rc = alloc(c, (void**)&c->faces_x, (size_t)(p.mx + 1) * sizeof(double)))) return rc;
+        double* auxg = nullptr;
+        if ((rc = alloc(c, (void**)&auxg, (size_t)(p.mx + 2 * p.mbc) * sizeof(double)))) return rc;
+        std::vector<double> at((size_t)(p.mx + 2 * p.mbc), 1.0);
+        const int na = p.maux > 0 ? p.maux : 1;
+        if (aux)
+            for (int i = 0; i < p.mx + 2 * p.mbc; i++) at[i] = aux[(size_t)i * na];
+        hipError_t e = hipMemcpy(auxg, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice);
+        Stage1DArgs a = base_stage1d_args(c);
+        a.src = c->ghosted;
+        a.aux = auxg;
+        a.src_ghosts = p.mbc;
+        a.smax_faces = c->faces_x;
+        a.Lout = Ld;
+        if (e == hipSuccess && launch_stage1d(weno_kind(p), a, c->st)) e = hipErrorLaunchFailure;
+        std::vector<double> L((size_t)p.mx), al((size_t)p.mx + 1);
+        if (e == hipSuccess) e = hipMemcpyAsync(L.data(), Ld, p.mx * sizeof(double), hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(al.data(), c->faces_x, (p.mx + 1) * sizeof(double), hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        hipFree(auxg);
+        if (e != hipSuccess) return fail(c, FINESS_ERR_HIP, hipGetErrorString(e));
+        for (int i = 0; i < p.mx; i++) Lstar[(size_t)(i + p.mbc) * c->M] = L[i];
+        for (int f = 0; f <= p.mx; f++) smax[f + p.mbc] = al[f];  // smax.set(i, alpha): overwritten
+        return FINESS_OK;
+    }
+    if (!c->faces_x && (rc = alloc(c, (void**)&c->faces_x, (size_t)(p.mx + 1) * p.my * sizeof(double)))) return rc;
+    if (!c->faces_y && (rc = alloc(c, (void**)&c->faces_y, (size_t)p.mx * (p.my + 1) * sizeof(double)))) return rc;
+    StageArgs a = base_stage_args(c);
+    a.src = c->ghosted;
+    a.src_ghosts = p.mbc;
+    a.Lout = Ld;
+    a.smax_faces_x = c->faces_x;
+    a.smax_faces_y = c->faces_y;
+    rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 0, a, c->st);
+    if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "stage launch failed");
+    const size_t nL = (size_t)c->M * c->plane;
+    std::vector<double> L(nL), ax((size_t)(p.mx + 1) * p.my), ay((size_t)p.mx * (p.my + 1));
+    HIPCHK(c, hipMemcpyAsync(L.data(), Ld, nL * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ax.data(), c->faces_x, ax.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ay.data(), c->faces_y, ay.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int g = p.mbc, M = c->M;
+    const long long ldq = p.my + 2 * g, lds = p.my + 1 + 2 * g;
+    for (int i = 0; i < p.mx; i++)
+        for (int j = 0; j < p.my; j++)
+            for (int m = 0; m < M; m++)
+                Lstar[((i + g) * ldq + (j + g)) * M + m] = L[(size_t)m * c->plane + (size_t)i * p.my + j];
+    // smax(i,j,1) for x-face i-1/2 (i = 1..mx+1, j = 1..my), smax(i,j,2) for y
+    for (int i = 0; i <= p.mx; i++)
+        for (int j = 0; j < p.my; j++) {
+            double& s = smax[((i + g) * lds + (j + g)) * 2 + 0];
+            s = rmax(s, ax[(size_t)i * p.my + j]);
+        }
+    for (int i = 0; i < p.mx; i++)
+        for (int j = 0; j <= p.my; j++) {
+            double& s = smax[((i + g) * lds + (j + g)) * 2 + 1];
+            s = rmax(s, ay[(size_t)i * (p.my + 1) + j]);
+        }
+    return FINESS_OK;
+}
+
+int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, const double* aux, double* F, double* G)
+{
+    (void)aux;
+    if (!c || !q || !F || !G) return FINESS_ERR_ARG;
+    const finess_params& p = c->p;
+    if (p.ndims != 2 || p.mbc < 5 || p.physics != FINESS_PHYS_EULER) return fail(c, FINESS_ERR_CONFIG, "LxW needs 2D Euler, mbc>=5");
+    int rc;
+    if ((rc = upload_ghosted(c, q))) return rc;
+    double *Fd = c->lxwF, *Gd = c->lxwG;
+    bool own = false;
+    const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6);
+    if (!Fd) {
+        own = true;
+        if ((rc = alloc(c, (void**)&Fd, lb * sizeof(double)))) return rc;
+        if ((rc = alloc(c, (void**)&Gd, lb * sizeof(double)))) { hipFree(Fd); return rc; }
+    }
+    LxwArgs a;
+    memset(&a, 0, sizeof a);
+    a.q = c->ghosted;
+    a.src_ghosts = p.mbc;
+    a.F = Fd;
+    a.G = Gd;
+    a.mx = p.mx;
+    a.my = p.my;
+    a.plane = c->plane;
+    for (int k = 0; k < 4; k++) a.bc[k] = p.bc[k];
+    a.time_order = p.time_order;
+    a.dt = dt;
+    a.dx = p.dx;
+    a.dy = p.dy;
+    a.pp.gamma = p.gamma;
+    std::vector<double> hF(lb), hG(lb);
+    hipError_t e = launch_lxw_flux(p.physics, a, c->st) ? hipErrorLaunchFailure : hipSuccess;
+    if (e == hipSuccess) e = hipMemcpyAsync(hF.data(), Fd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hG.data(), Gd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (own) { hipFree(Fd); hipFree(Gd); }
+    if (e != hipSuccess) return fail(c, FINESS_ERR_HIP, hipGetErrorString(e));
+    const int g = p.mbc, M = c->M;
+    const long long ldq = p.my + 2 * g, le = p.my + 6, lpl = (long long)(p.mx + 6) * le;
+    for (int i = -3; i < p.mx + 3; i++)
+        for (int j = -3; j < p.my + 3; j++)
+            for (int m = 0; m < M; m++) {
+                const long long o = (long long)m * lpl + (long long)(i + 3) * le + (j + 3);
+                F[((i + g) * ldq + (j + g)) * M + m] = hF[o];
+                G[((i + g) * ldq + (j + g)) * M + m] = hG[o];
+            }
+    return FINESS_OK;
+}
+
+static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
+{
+    if (!c || !dtv) return FINESS_ERR_ARG;
+    if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
+    const finess_params& p = c->p;
+    const double CFL_max = p.max_cfl, CFL_target = p.desired_cfl;
+    double t = c->t, dt = dtv[1], cfl = 0.0;
+    double dtmin = dt, dtmax = dt;
+    long long n_step = 0, acc = 0, rej = 0;
+    int rc = 0;
+    const bool by_steps = fixed_steps > 0;
+    while (by_steps ? (acc < fixed_steps) : (t < tend)) {
+        int m_accept = 0;
+        n_step = n_step + 1;
+        if (!by_steps && n_step > p.nv) { rc = fail(c, FINESS_ERR_NV, "exceeded allowed # of time steps"); break; }
+        while (m_accept == 0) {
+            const double told = t;
+            if (!by_steps && told + dt > tend) dt = tend - told;
+            t = told + dt;
+            int res = 0, slot = 0;
+            if ((rc = enqueue_step(c, dt, &res, &slot))) break;
+            if ((rc = read_cfl(c, slot, dt, dtv[2], &cfl))) break;
+            if (cfl > 0.0) {
+                dt = rmin(dtv[2], dt * CFL_target / cfl);
+                dtmin = rmin(dt, dtmin);
+                dtmax = rmax(dt, dtmax);
+            } else {
+                dt = dtv[2];
+            }
+            if (cfl <= CFL_max) {
+                m_accept = 1;
+                c->cur = res;
+                acc++;
+            } else {
+                t = told;
+                rej++;
+            }
+            if (p.verbosity)
+                printf("finess-hip ... Step %5lld   CFL =%6.3f   dt =%11.3e   t =%11.3e%s\n", n_step, cfl, dt, t,
+                       m_accept ? "" : "  (rejected)");
+        }
+        if (rc) break;
+    }
+    if (!rc) dtv[1] = dt;
+    c->t = t;
+    if (stats) {
+        stats->steps_accepted = acc;
+        stats->steps_rejected = rej;
+        stats->dt_min = dtmin;
+        stats->dt_max = dtmax;
+        stats->last_cfl = cfl;
+        stats->t = t;
+    }
+    return rc;
+}
+
+int finess_advance(finess_ctx* c, double tend, double* dtv, finess_stats* stats)
+{
+    return advance_impl(c, tend, dtv, 0, stats);
+}
+
+int finess_advance_steps(finess_ctx* c, long long nsteps, double* dtv, finess_stats* stats)
+{
+    if (nsteps < 1) return FINESS_ERR_ARG;
+    return advance_impl(c, 0.0, dtv, nsteps, stats);
+}
+
+int finess_fin_solve_rk(finess_ctx* c, double* q, const double* aux, double* t, double tend, double* dtv)
+{
+    if (!c || !q || !t || !dtv) return FINESS_ERR_ARG;
+    if (c->p.time_stepping_method != FINESS_TS_RK) return fail(c, FINESS_ERR_CONFIG, "context is not RK");
+    int rc = finess_set_state(c, q, aux, *t);
+    if (rc) return rc;
+    rc = finess_advance(c, tend, dtv, nullptr);
+    const int rc2 = finess_get_state(c, q, t);
+    return rc ? rc : rc2;
+}
+
+int finess_fin_solve_lxw(finess_ctx* c, double* q, const double* aux, double* t, double tend, double* dtv)
+{
+    if (!c || !q || !t || !dtv) return FINESS_ERR_ARG;
+    if (c->p.time_stepping_method != FINESS_TS_LXW) return fail(c, FINESS_ERR_CONFIG, "context is not LxW");
+    int rc = finess_set_state(c, q, aux, *t);
+    if (rc) return rc;
+    rc = finess_advance(c, tend, dtv, nullptr);
+    const int rc2 = finess_get_state(c, q, t);
+    return rc ? rc : rc2;
+}
+
+int finess_run_fixed_steps(finess_ctx* c, double dt, int nsteps, double* last_cfl)
+{
+    if (!c || nsteps < 0) return FINESS_ERR_ARG;
+    if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
+    int res = c->cur, slot = 0, rc;
+    const int nst = c->p.time_stepping_method == FINESS_TS_LXW ? 1 : c->p.time_order;
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    for (int n = 0; n < nsteps; n++) {
+        if ((rc = enqueue_step(c, dt, &res, &slot))) return rc;
+        c->cur = res;
+        c->t += dt;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = (long long)nsteps * nst;
+    if (last_cfl && nsteps > 0) {
+        double cfl = 0.0;
+        if ((rc = read_cfl(c, slot, dt, 1e300, &cfl))) return rc;
+        *last_cfl = cfl;
+    }
+    return FINESS_OK;
+}
+
+int finess_sync(finess_ctx* c)
+{
+    if (!c) return FINESS_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FINESS_OK;
+}
+
+int finess_last_kernel_time(finess_ctx* c, double* ms, long long* launches)
+{
+    if (!c) return FINESS_ERR_ARG;
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return FINESS_OK;
+}
+
+} // extern "C"

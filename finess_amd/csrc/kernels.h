@@ -1,0 +1,26 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (internal to the
+// library; the public interface is include/finess_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace finess {
+
+enum : int { PHYS_ADVECTION = 0, PHYS_EULER = 1, PHYS_SHALLOW_WATER = 2 };
+
+struct StageArgs;
+struct Stage1DArgs;
+struct LxwArgs;
+
+// 2D fused stage.  wk: WenoKind; fp32: stencil arithmetic in FP32.
+int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st);
+// 1D stage (two launches: faces, update).
+int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
+// Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].
+int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st);
+
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
+int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st);
+int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],
+                         int nx_mom, int ny_mom, hipStream_t st);
+
+} // namespace finess

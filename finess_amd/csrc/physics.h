@@ -1,0 +1,247 @@
+// physics.h -- device restatement of the app callback libraries the kernels
+// are compiled with.  Every 2D function works in the frame rotated to the
+// face normal: for an x-face the conserved vector is used as stored, for a
+// y-face the two in-plane momentum components are swapped.  The reference's
+// callbacks do the same with their (mu, mv) index pairs (ixy = 1: mu=2, mv=3;
+// ixy = 2: mu=3, mv=2), so the rotated x-formulas below are theirs.
+//
+//   Euler:         FluxFunc        apps/2d/euler/lib/FluxFunc.cpp:9-42
+//                  ProjectLeftEig  apps/2d/euler/lib/ProjectLeftEig.cpp:9-67
+//                  ProjectRightEig apps/2d/euler/lib/ProjectRightEig.cpp:9-63
+//                  SetWaveSpd      apps/2d/euler/lib/SetWaveSpd.cpp:11-69
+//   Shallow water: apps/2d/shallow_water/lib/{FluxFunc,ProjectLeftEig,
+//                  ProjectRightEig,SetWaveSpd}.cpp  (g = 1)
+//
+// Per-cell values kept in LDS ("cell record"): the conserved state followed
+// by the primitive values the flux needs, so each cell's divisions are done
+// once per stage and not once per stencil point.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace finess {
+
+template <typename R>
+__host__ __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }  // lib/dog_math.h:18-22
+template <typename R>
+__host__ __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+
+struct PhysParams {
+    double gamma;
+    int sw_compat_hl;
+};
+
+// ------------------------------------------------------------------------
+// 2D Euler, meqn = 5: (rho, rho u, rho v, rho w, E)
+// cell record: q0..q4, u, v, w, p
+// ------------------------------------------------------------------------
+struct Euler2D {
+    static constexpr int MEQN = 5;
+    static constexpr int NV = 9;
+    static constexpr int NORMAL_X = 1;  // momentum component normal to x-walls
+    static constexpr int NORMAL_Y = 2;
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void cell_record(const R q[5], const PhysParams& pp, R v[9])
+    {
+        const R gamma = R(pp.gamma);
+        const R rho = q[0];
+        const R u1 = q[1] / rho, u2 = q[2] / rho, u3 = q[3] / rho;
+        const R press = (gamma - R(1)) * (q[4] - R(0.5) * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
+        v[5] = u1; v[6] = u2; v[7] = u3; v[8] = press;
+    }
+
+    // LDS variable index of rotated component k (dir 0: x, 1: y)
+    __host__ __device__ __forceinline__ static int rot_q(int k, int dir)
+    {
+        return (dir && (k == 1 || k == 2)) ? 3 - k : k;
+    }
+
+    // Rotated state and flux of one cell record.  c[] is the record.
+    template <typename R>
+    __host__ __device__ __forceinline__ static void state_flux(const R c[9], int dir, R q[5], R f[5])
+    {
+        const R rho = c[0], E = c[4], p = c[8];
+        const R mn = dir ? c[2] : c[1], mt = dir ? c[1] : c[2];
+        const R un = dir ? c[6] : c[5], ut = dir ? c[5] : c[6], uw = c[7];
+        q[0] = rho; q[1] = mn; q[2] = mt; q[3] = c[3]; q[4] = E;
+        const R run = rho * un;
+        f[0] = run;
+        f[1] = run * un + p;
+        f[2] = run * ut;
+        f[3] = run * uw;
+        f[4] = un * (E + p);
+    }
+
+    template <typename R>
+    struct Eig {
+        R u1, u2, u3, umag2, H, c, cR;
+        R k1a, k1b, k5a, k5b, k2, inv15, inv2;
+    };
+
+    // Eigen-system at the arithmetic-mean state (lib/2d/ConstructL.cpp:72-77)
+    template <typename R>
+    __host__ __device__ __forceinline__ static void eig(const R qa[5], const PhysParams& pp, Eig<R>& e)
+    {
+        const R gamma = R(pp.gamma);
+        const R rho = qa[0];
+        const R irho = R(1) / rho;
+        e.u1 = qa[1] * irho; e.u2 = qa[2] * irho; e.u3 = qa[3] * irho;
+        const R energy = qa[4];
+        e.umag2 = e.u1 * e.u1 + e.u2 * e.u2 + e.u3 * e.u3;
+        const R press = (gamma - R(1)) * (energy - R(0.5) * rho * e.umag2);
+        const R c2 = gamma * press * irho;
+        e.c = sqrt(c2);          // ProjectLeftEig.cpp:43
+        e.cR = sqrt(fabs(c2));   // ProjectRightEig.cpp:42
+        e.H = (energy + press) * irho;
+        const R hu = e.umag2 * R(0.5);
+        e.k1a = hu - e.H - e.u1 * e.c;
+        e.k1b = hu * (e.c - e.u1) + e.H * e.u1;
+        e.k5a = e.H - hu - e.u1 * e.c;
+        e.k5b = hu * (e.c + e.u1) - e.H * e.u1;
+        e.k2 = e.H - e.umag2;
+        const R den = R(2) * e.H - e.umag2;
+        e.inv2 = R(2) / den;
+        e.inv15 = R(1) / (e.c * den);
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void left(const Eig<R>& e, const R q[5], R w[5])
+    {
+        const R t = e.c * (q[4] - e.u2 * q[2] - e.u3 * q[3]);
+        w[0] = (e.k1a * q[1] + e.k1b * q[0] + t) * e.inv15;
+        w[1] = ((e.k2 * q[0] + e.u1 * q[1] + e.u2 * q[2] + e.u3 * q[3]) - q[4]) * e.inv2;
+        w[2] = q[2] - e.u2 * q[0];
+        w[3] = q[3] - e.u3 * q[0];
+        w[4] = (e.k5a * q[1] + e.k5b * q[0] + t) * e.inv15;
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void right(const Eig<R>& e, const R w[5], R q[5])
+    {
+        const R s = w[0] + w[1] + w[4];
+        const R uc = e.u1 * e.cR;
+        q[0] = s;
+        q[1] = (e.u1 - e.cR) * w[0] + e.u1 * w[1] + (e.u1 + e.cR) * w[4];
+        q[2] = e.u2 * s + w[2];
+        q[3] = e.u3 * s + w[3];
+        q[4] = (e.H - uc) * w[0] + (e.umag2 * R(0.5)) * w[1] + e.u2 * w[2] + e.u3 * w[3] + (e.H + uc) * w[4];
+    }
+
+    // SetWaveSpd with nvec = the face normal; cl, cr are the cell records of
+    // the left and right cells.  Returns max(|s1|,|s2|).
+    template <typename R>
+    __host__ __device__ __forceinline__ static R wave_speed(const R cl[9], const R cr[9], int dir, const PhysParams& pp)
+    {
+        const R gamma = R(pp.gamma);
+        const R rhol = cl[0], rhor = cr[0], pl = cl[8], pr = cr[8];
+        const R unl = dir ? cl[6] : cl[5], unr = dir ? cr[6] : cr[5];
+        const R rho = R(0.5) * (rhol + rhor);
+        const R un = R(0.5) * (unl + unr);
+        const R press = R(0.5) * (pl + pr);
+        const R c_l = sqrt(fabs(gamma * pl / rhol));
+        const R c_r = sqrt(fabs(gamma * pr / rhor));
+        const R c = sqrt(fabs(gamma * press / rho));
+        const R s1 = rmin(unl - c_l, un - c);
+        const R s2 = rmax(unr + c_r, un + c);
+        return rmax(fabs(s1), fabs(s2));
+    }
+};
+
+// ------------------------------------------------------------------------
+// 2D shallow water (g = 1), meqn = 3: (h, hu, hv)
+// cell record: q0..q2, u, v
+// ------------------------------------------------------------------------
+struct ShallowWater2D {
+    static constexpr int MEQN = 3;
+    static constexpr int NV = 5;
+    static constexpr int NORMAL_X = 1;
+    static constexpr int NORMAL_Y = 2;
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void cell_record(const R q[3], const PhysParams&, R v[5])
+    {
+        const R h = q[0];
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
+        v[3] = q[1] / h; v[4] = q[2] / h;
+    }
+
+    __host__ __device__ __forceinline__ static int rot_q(int k, int dir)
+    {
+        return (dir && (k == 1 || k == 2)) ? 3 - k : k;
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void state_flux(const R c[5], int dir, R q[3], R f[3])
+    {
+        const R h = c[0];
+        const R mn = dir ? c[2] : c[1], mt = dir ? c[1] : c[2];
+        const R un = dir ? c[4] : c[3], ut = dir ? c[3] : c[4];
+        q[0] = h; q[1] = mn; q[2] = mt;
+        const R hun = h * un;
+        f[0] = hun;
+        f[1] = hun * un + R(0.5) * h * h;
+        f[2] = hun * ut;
+    }
+
+    template <typename R>
+    struct Eig {
+        R u1, u2, sh, a1, a3, b;
+    };
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void eig(const R qa[3], const PhysParams&, Eig<R>& e)
+    {
+        const R h = qa[0];
+        e.u1 = qa[1] / h;
+        e.u2 = qa[2] / h;
+        e.sh = sqrt(h);
+        const R i2s = R(1) / (R(2) * e.sh);
+        e.a1 = (e.sh + e.u1) * i2s;
+        e.a3 = (e.sh - e.u1) * i2s;
+        e.b = R(0.5) / e.sh;
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void left(const Eig<R>& e, const R q[3], R w[3])
+    {
+        w[0] = e.a1 * q[0] - e.b * q[1];
+        w[1] = -e.u2 * q[0] + q[2];
+        w[2] = e.a3 * q[0] + e.b * q[1];
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void right(const Eig<R>& e, const R w[3], R q[3])
+    {
+        q[0] = w[0] + w[2];
+        q[1] = (e.u1 - e.sh) * w[0] + (e.u1 + e.sh) * w[2];
+        q[2] = e.u2 * w[0] + w[1] + e.u2 * w[2];
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static R wave_speed(const R cl[5], const R cr[5], int dir, const PhysParams& pp)
+    {
+        const R hl = cl[0], hr = cr[0];
+        const R unl = dir ? cl[4] : cl[3];
+        // apps/2d/shallow_water/lib/SetWaveSpd.cpp:27-28 divides the right
+        // momentum by h_l; kept when sw_compat_hl is set.
+        const R mr1 = cr[1], mr2 = cr[2];
+        const R hden = pp.sw_compat_hl ? hl : hr;
+        const R u1r = mr1 / hden, u2r = mr2 / hden;
+        const R unr = dir ? u2r : u1r;
+        const R u1l = cl[3], u2l = cl[4];
+        const R h = R(0.5) * (hl + hr);
+        const R shl = sqrt(hl), shr = sqrt(hr);
+        const R idn = R(1) / (shl + shr);
+        const R u1 = (shl * u1l + shr * u1r) * idn;
+        const R u2 = (shl * u2l + shr * u2r) * idn;
+        const R un = dir ? u2 : u1;
+        const R sh = sqrt(h);
+        const R s1 = rmin(un - sh, unl - shl);
+        const R s2 = rmax(un + sh, unr + shr);
+        return rmax(fabs(s1), fabs(s2));
+    }
+};
+
+} // namespace finess

@@ -1,0 +1,53 @@
+// stage1d.hip -- the 1D path: lib/1d/ConstructL.cpp:16-224 with the
+// advection callbacks of apps/1d/advection/lib (f = aux(1) q, identity
+// eigenvectors, s1 = min(a_l, (a_l+a_r)/2), s2 = max(a_r, (a_l+a_r)/2)),
+// UpdateSoln (lib/1d/UpdateSoln.cpp:17-54) and the periodic halo
+// (apps/1d/advection/smooth_example/SetBndValues.cpp:6-63).
+//
+// 1D grids are small (the advection case has 200 cells), so the stage is two
+// plain launches: one thread per face, then one thread per cell.
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+#include "stage1d_body.h"
+
+namespace finess {
+
+template <int WK>
+__global__ __launch_bounds__(256) void k_faces1d(Stage1DArgs a)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    double alpha = 0.0;
+    if (f <= a.mx) alpha = face1d<WK>(f, a);
+    if (a.smax_slot) {
+        if (alpha != alpha) alpha = __builtin_inf();
+        // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
+        if (f < a.mx) atomicMax(a.smax_slot, (unsigned long long)__double_as_longlong(alpha));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_update1d(Stage1DArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.mx) update1d(i, a);
+}
+
+template <int WK>
+static int launch_wk(const Stage1DArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_faces1d<WK>, dim3((a.mx + 1 + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_update1d, dim3((a.mx + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
+{
+    switch (wk) {
+    case WENO_JS5_P2: return launch_wk<WENO_JS5_P2>(a, st);
+    case WENO_JS5_POW: return launch_wk<WENO_JS5_POW>(a, st);
+    case WENO_Z5: return launch_wk<WENO_Z5>(a, st);
+    case WENO_FD5: return launch_wk<WENO_FD5>(a, st);
+    default: return -1;
+    }
+}
+
+} // namespace finess

@@ -1,0 +1,70 @@
+// stage1d_body.h -- per-face and per-cell work of the 1D stage (see
+// stage1d.hip); plain functions so that a CPU harness can run them too.
+#pragma once
+#include "physics.h"
+#include "stage2d_body.h"
+#include "weno.h"
+
+namespace finess {
+
+struct Stage1DArgs {
+    const double* src;   // q planes (meqn=1): q[i], i in [0, mx)
+    const double* qa;
+    const double* qb;
+    double* out;
+    const double* aux;   // aux(1)[i] (advection speed), i in [0, mx)
+    double* fhat;        // mx+1 face fluxes
+    int mx;
+    int bc_lo, bc_hi;
+    int src_ghosts;      // >0: src/aux include that many ghosts per side
+    int a_is_src, b_is_src, use_b;
+    double a1, a2, bdt, dx, alpha_scaling, eps, power;
+    unsigned long long* smax_slot;
+    double* smax_faces;  // optional: alpha of each face f in [0, mx]
+    double* Lout;        // optional: L of each cell
+};
+
+__host__ __device__ __forceinline__ long long idx1d(const Stage1DArgs& a, int g)
+{
+    if (a.src_ghosts) return g + a.src_ghosts;
+    int fl;
+    return bc_map(a.bc_lo, a.bc_hi, g, a.mx, fl);
+}
+
+// Face f sits between cells f-1 and f (0-based); lib/1d/ConstructL.cpp:49-165.
+template <int WK>
+__host__ __device__ __forceinline__ double face1d(int f, const Stage1DArgs& a)
+{
+    const WenoParams<double> wp{a.eps, a.power};
+    double q[6], fl[6];
+    for (int s = 0; s < 6; s++) {
+        const long long k = idx1d(a, f - 3 + s);
+        q[s] = a.src[k];
+        fl[s] = q[s] * a.aux[k];  // FluxFunc: Q(i,1)*Aux(i,1)
+    }
+    const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
+    const double s1 = rmin(al, 0.5 * (al + ar));
+    const double s2 = rmax(ar, 0.5 * (al + ar));
+    const double alpha = rmax(fabs(s1), fabs(s2));
+    const double la = a.alpha_scaling * alpha;
+    double gp[5], gm[5];
+    for (int s = 0; s < 5; s++) {
+        gp[s] = 0.5 * (fl[s] + la * q[s]);
+        gm[s] = 0.5 * (fl[5 - s] - la * q[5 - s]);
+    }
+    a.fhat[f] = weno5<WK>(gp, wp) + weno5<WK>(gm, wp);
+    if (a.smax_faces) a.smax_faces[f] = alpha;
+    return alpha;
+}
+
+__host__ __device__ __forceinline__ void update1d(int i, const Stage1DArgs& a)
+{
+    const double L = -(a.fhat[i + 1] - a.fhat[i]) / a.dx;
+    if (a.Lout) a.Lout[i] = L;
+    if (!a.out) return;
+    double v = a.a1 * a.qa[i];
+    if (a.use_b) v = v + a.a2 * a.qb[i];
+    a.out[i] = v + a.bdt * L;
+}
+
+} // namespace finess

@@ -1,0 +1,181 @@
+// stage2d.hip -- gfx950 kernels of the 2D path: the fused stage kernel and
+// the small helpers around it (layout conversion, CFL-slot reset).
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+#include "stage2d_body.h"
+
+namespace finess {
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remapping the
+// launch index gives each XCD a contiguous band of tiles, so the halo rows
+// that neighbouring tiles share are found in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    constexpr int NXCD = 8;
+    if (n % NXCD) return b;
+    return (b % NXCD) * (n / NXCD) + b / NXCD;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Reduces two per-thread maxima over the workgroup and folds them into
+// slot[0..1] (non-negative doubles compare like their bit patterns).
+template <int NT>
+__device__ __forceinline__ void block_max2(double ax, double ay, unsigned long long* slot)
+{
+    __shared__ double red[2][NT / 64];
+    ax = wave_max(ax);
+    ay = wave_max(ay);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wid] = ax; red[1][wid] = ay; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx_ = red[0][0], my_ = red[1][0];
+        for (int w = 1; w < NT / 64; w++) {
+            mx_ = red[0][w] > mx_ ? red[0][w] : mx_;
+            my_ = red[1][w] > my_ ? red[1][w] : my_;
+        }
+        atomicMax(&slot[0], (unsigned long long)__double_as_longlong(mx_));
+        atomicMax(&slot[1], (unsigned long long)__double_as_longlong(my_));
+    }
+}
+
+template <class P, typename R, int WK, bool LXW, int TI, int TJ>
+__global__ __launch_bounds__(TI* TJ) void k_stage2d(StageArgs a)
+{
+    constexpr int NT = TI * TJ;
+    __shared__ StageSmem<P, R, TI, TJ> sm;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
+    const int i0 = ti * TI, j0 = tj * TJ;
+    phase_load<P, R, TI, TJ>(threadIdx.x, NT, i0, j0, a, sm);
+    __syncthreads();
+    double ax = 0.0, ay = 0.0;
+    phase_faces<P, R, WK, LXW, TI, TJ>(threadIdx.x, NT, i0, j0, a, sm, ax, ay);
+    __syncthreads();
+    phase_update<P, R, TI, TJ>(threadIdx.x, i0, j0, a, sm);
+    if (a.smax) {
+        // a NaN speed must not vanish in the max: report it as +inf
+        if (ax != ax) ax = __builtin_inf();
+        if (ay != ay) ay = __builtin_inf();
+        block_max2<NT>(ax, ay, a.smax);
+    }
+}
+
+template <class P, typename R, int WK, bool LXW, int TI, int TJ>
+static int launch_stage(const StageArgs& a0, hipStream_t st)
+{
+    StageArgs a = a0;
+    const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
+    a.ntj = ntj;
+    hipLaunchKernelGGL((k_stage2d<P, R, WK, LXW, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+template <class P, typename R, bool LXW>
+static int dispatch_weno(int wk, const StageArgs& a, hipStream_t st)
+{
+    switch (wk) {
+    case WENO_JS5_P2: return launch_stage<P, R, WENO_JS5_P2, LXW, 8, 32>(a, st);
+    case WENO_FD5: return launch_stage<P, R, WENO_FD5, LXW, 8, 32>(a, st);
+    case WENO_JS5_POW: return launch_stage<P, R, WENO_JS5_POW, LXW, 8, 32>(a, st);
+    case WENO_Z5: return launch_stage<P, R, WENO_Z5, LXW, 8, 32>(a, st);
+    default: return -1;
+    }
+}
+
+int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st)
+{
+    if (physics == PHYS_EULER) {
+        if (!fp32) return lxw ? dispatch_weno<Euler2D, double, true>(wk, a, st)
+                              : dispatch_weno<Euler2D, double, false>(wk, a, st);
+        if (wk != WENO_JS5_P2) return -1;
+        return lxw ? launch_stage<Euler2D, float, WENO_JS5_P2, true, 8, 32>(a, st)
+                   : launch_stage<Euler2D, float, WENO_JS5_P2, false, 8, 32>(a, st);
+    }
+    if (physics == PHYS_SHALLOW_WATER) {
+        if (lxw) return -1;
+        if (!fp32) return dispatch_weno<ShallowWater2D, double, false>(wk, a, st);
+        if (wk != WENO_JS5_P2) return -1;
+        return launch_stage<ShallowWater2D, float, WENO_JS5_P2, false, 8, 32>(a, st);
+    }
+    return -1;
+}
+
+// ---- helpers ------------------------------------------------------------
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// Reference layout (i slowest, m fastest, mbc ghosts) -> field planes.
+// ndims 2: src index ((i+g)*(my+2g) + (j+g))*meqn + m for 0-based interior
+// (i, j); dst planes m*(mx+2*dg)*(my+2*dg) + (i+dg)*(my+2dg) + (j+dg) where dg
+// = 0 copies the interior only and dg = g copies ghosts too.
+__global__ void k_ref_to_planes(const double* __restrict__ src, double* __restrict__ dst, int mx, int my,
+                                int meqn, int g, int dg)
+{
+    const long long n = (long long)(mx + 2 * dg) * (my + 2 * dg);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int ld = my + 2 * dg;
+    const int ii = (int)(k / ld), jj = (int)(k - (long long)ii * ld);
+    const int i = ii - dg, j = jj - dg;
+    const long long s = ((long long)(i + g) * (my + 2 * g) + (j + g)) * meqn;
+    for (int m = 0; m < meqn; m++) dst[m * n + k] = src[s + m];
+}
+
+int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st)
+{
+    const long long n = (long long)(mx + 2 * dg) * (my + 2 * dg);
+    hipLaunchKernelGGL(k_ref_to_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, mx, my, meqn,
+                       g, dg);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// Field planes (interior) -> reference layout with every ghost (corners too)
+// filled through the boundary map.
+__global__ void k_planes_to_ref(const double* __restrict__ src, double* __restrict__ dst, int mx, int my, int meqn,
+                                int g, int bc0, int bc1, int bc2, int bc3, int nx_mom, int ny_mom)
+{
+    const long long n = (long long)(mx + 2 * g) * (my + 2 * g);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int ld = my + 2 * g;
+    const int i = (int)(k / ld) - g, j = (int)(k - (long long)(k / ld) * ld) - g;
+    int fi, fj;
+    const int si = bc_map(bc0, bc1, i, mx, fi);
+    const int sj = bc_map(bc2, bc3, j, my, fj);
+    const long long plane = (long long)mx * my;
+    for (int m = 0; m < meqn; m++) {
+        double v = src[m * plane + (long long)si * my + sj];
+        if ((fi && m == nx_mom) || (fj && m == ny_mom)) v = -v;
+        dst[k * meqn + m] = v;
+    }
+}
+
+int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],
+                         int nx_mom, int ny_mom, hipStream_t st)
+{
+    const long long n = (long long)(mx + 2 * g) * (my + 2 * g);
+    hipLaunchKernelGGL(k_planes_to_ref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, mx, my, meqn,
+                       g, bc[0], bc[1], bc[2], bc[3], nx_mom, ny_mom);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+} // namespace finess

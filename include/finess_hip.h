@@ -1,0 +1,188 @@
+/*
+ * finess_hip.h -- C-ABI of the MI355X (gfx950) implementation of FINESS's
+ * finite-difference WENO spatial operator and its time steppers.
+ *
+ * Every entry point takes plain pointers and sizes.  State arrays crossing
+ * this boundary use the reference's own tensor layout (lib/tensors.h:267-273,
+ * lib/tensors.cpp:650-667): a 2D field q(i,j,m), 1-based, with mbc ghost
+ * layers on the two spatial indices, lives at
+ *     ((i+mbc-1)*(my+2*mbc) + (j+mbc-1))*meqn + (m-1)
+ * and a 1D field q(i,m) at (i+mbc-1)*meqn + (m-1).  The library keeps its own
+ * device copy in a field-major layout; the conversion happens on the device.
+ *
+ * Status codes: every function returns FINESS_OK (0) or a negative code.  The
+ * reference reports the same conditions with exit(1) or an exception; the C++
+ * drop-in layer (finess_amd/compat) maps the codes back to that behaviour.
+ */
+#ifndef FINESS_HIP_H
+#define FINESS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define FINESS_OK               0
+#define FINESS_ERR_CONFIG      -1  /* unsupported parameter combination (lib/WenoReconstruct.cpp:70) */
+#define FINESS_ERR_NONFINITE   -2  /* non-finite state detected on the device                        */
+#define FINESS_ERR_NV          -3  /* n_step > nv            (lib/2d/FinSolveRK.cpp:84-92)           */
+#define FINESS_ERR_HIP         -4  /* HIP runtime error                                               */
+#define FINESS_ERR_DT          -5  /* dt > max_dt            (lib/2d/GetCFL.cpp:43-50)               */
+#define FINESS_ERR_CFL         -6  /* cfl > 1e8 (2D) / 1e10 (1D) (lib/2d/GetCFL.cpp:60-67)           */
+#define FINESS_ERR_ARG         -7  /* null pointer / bad size                                        */
+#define FINESS_ERR_NODEVICE    -8  /* no GPU visible                                                  */
+
+/* ---- enumerations (IniParams::WenoVersion, TimeSteppingMethod) ------- */
+#define FINESS_WENO_JS 0
+#define FINESS_WENO_FD 1
+#define FINESS_WENO_Z  2
+
+#define FINESS_TS_RK   0
+#define FINESS_TS_LXW  1
+
+/* physics compiled into the kernels (the app callback libraries):
+ *   ADVECTION      apps/1d/advection/lib  f = aux(1)*q           (1D, meqn=1)
+ *   EULER          apps/2d/euler/lib      (rho,rho u,rho v,rho w,E) (2D, meqn=5)
+ *   SHALLOW_WATER  apps/2d/shallow_water/lib (h,hu,hv), g=1        (2D, meqn=3) */
+#define FINESS_PHYS_ADVECTION     0
+#define FINESS_PHYS_EULER         1
+#define FINESS_PHYS_SHALLOW_WATER 2
+
+/* boundary condition per side (the SetBndValues templates):
+ *   EXTRAP   zeroth-order extrapolation, apps/2d/euler/riemann_test/SetBndValues.cpp:9-114
+ *   PERIODIC apps/2d/advection/smooth_example/SetBndValues.cpp:28-194 (with corners),
+ *            apps/1d/advection/smooth_example/SetBndValues.cpp:6-63
+ *   REFLECT  solid wall: mirror image, normal momentum negated                    */
+#define FINESS_BC_EXTRAP   0
+#define FINESS_BC_PERIODIC 1
+#define FINESS_BC_REFLECT  2
+
+/* stencil arithmetic of the device path */
+#define FINESS_PREC_FP64 0   /* plain form, FP64 end to end                       */
+#define FINESS_PREC_FP32 1   /* FP32 stencil, FP64 stage accumulation              */
+
+/* interface routing */
+#define FINESS_ROUTE_NONE    0 /* full characteristic WENO everywhere              */
+#define FINESS_ROUTE_LEARNED 1 /* MLP indicator: smooth -> component-wise FD5     */
+#define FINESS_ROUTE_ORACLE  2 /* route by the weight-departure label itself       */
+
+/*
+ * Parameters.  Field names follow the accessors of the reference's generated
+ * IniParams (schema: python/finess/params/{finess,weno,dim1,dim2,euler}.py).
+ */
+typedef struct finess_params {
+    /* [grid] (python/finess/params/dim2.py:8-65) */
+    int    ndims;
+    int    mx, my, mbc;
+    double xlow, xhigh, ylow, yhigh;
+    double dx, dy;                      /* derived: (xhigh-xlow)/mx, ... */
+    /* [finess] (python/finess/params/finess.py:13-170) */
+    int    meqn, maux;
+    int    space_order;                 /* only 5 */
+    int    time_order;                  /* RK: 1..3; LxW: 2..3 */
+    int    time_stepping_method;        /* FINESS_TS_* */
+    double initial_dt, max_dt, desired_cfl, max_cfl;
+    int    nv;
+    int    global_alpha;                /* only 0 (default) */
+    int    source_term;                 /* only 0 */
+    int    verbosity;
+    /* [weno] (python/finess/params/weno.py:15-51) */
+    int    weno_version;                /* FINESS_WENO_* */
+    double epsilon, power_param, alpha_scaling;
+    /* physics */
+    int    physics;                     /* FINESS_PHYS_* */
+    double gamma;                       /* [euler] gamma (euler.py:13-19) */
+    int    sw_compat_hl;                /* 1: keep the reference's u_r = (hu)_r / h_l */
+    int    bc[4];                       /* left, right, bottom, top: FINESS_BC_* */
+    /* device-path options (no counterpart in the reference) */
+    int    precision;                   /* FINESS_PREC_* */
+    int    routing;                     /* FINESS_ROUTE_* */
+    double route_tau;                   /* label threshold on max_k |w_k - d_k| */
+    int    tile_i, tile_j;              /* 0 = library default */
+} finess_params;
+
+/* Fills defaults exactly as the reference's IniParams defaults do. */
+void finess_params_default(finess_params* p);
+/* Recomputes dx, dy and validates the combination; returns a status code. */
+int  finess_params_finalize(finess_params* p);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct finess_ctx finess_ctx;
+
+/* Creates a context on the current HIP device.  *out receives the handle. */
+int  finess_create(const finess_params* p, finess_ctx** out);
+void finess_destroy(finess_ctx* ctx);
+/* Last error message of this context (static storage inside the context). */
+const char* finess_last_error(const finess_ctx* ctx);
+
+/* ---- state transfer (reference layout, host memory) -------------------- */
+/* Uploads q (and aux, may be NULL when maux==0 or unused) with ghosts; t. */
+int finess_set_state(finess_ctx* ctx, const double* q, const double* aux, double t);
+/* Downloads the state into q (reference layout; ghost cells are filled by the
+ * configured boundary condition, corners included). */
+int finess_get_state(finess_ctx* ctx, double* q, double* t);
+
+/* ---- drop-in operators -------------------------------------------------- */
+
+/* ConstructL (2D: lib/2d/ConstructL.cpp:13-408, 1D: lib/1d/ConstructL.cpp:16-224).
+ * q, aux: host arrays in the reference layout with ghosts already filled by
+ * the caller (as FinSolveRK does before each call).  Lstar receives the
+ * interior; smax is max-accumulated (2D, dTensorBC3(mx+1,my+1,2,mbc)) or
+ * overwritten (1D, dTensorBC1(mx,mbc)) exactly as the reference does. */
+int finess_construct_l(finess_ctx* ctx, const double* q, const double* aux,
+                       double* Lstar, double* smax);
+
+/* FinSolveRK (lib/2d/FinSolveRK.cpp:13-371, lib/1d/FinSolveRK.cpp:27-363).
+ * Advances q (host, reference layout) from *t to tend.  dtv[1] is the in/out
+ * time step, dtv[2] the max_dt (reference convention, dtv has 3 entries). */
+int finess_fin_solve_rk(finess_ctx* ctx, double* q, const double* aux,
+                        double* t, double tend, double* dtv);
+
+/* FinSolveLxW (lib/2d/FinSolveLxW.cpp:16-245), same conventions. */
+int finess_fin_solve_lxw(finess_ctx* ctx, double* q, const double* aux,
+                         double* t, double tend, double* dtv);
+
+/* ConstructIntegratedR (lib/2d/ConstructIntegratedR.cpp:59-296): time-
+ * integrated fluxes F, G (dTensorBC3(mx,my,meqn,mbc)) on cells [1-3, m+3]. */
+int finess_construct_integrated_r(finess_ctx* ctx, double dt, const double* q,
+                                  const double* aux, double* F, double* G);
+
+/* ---- device-resident stepping ------------------------------------------ */
+typedef struct finess_stats {
+    long long steps_accepted;
+    long long steps_rejected;
+    double    dt_min, dt_max;
+    double    last_cfl;
+    double    t;
+} finess_stats;
+
+/* Advances the state held by the context (set by finess_set_state) from its
+ * current time to tend with the configured stepper, leaving it on the device.
+ * dtv as above.  max_steps <= 0 means "nv".  stats may be NULL. */
+int finess_advance(finess_ctx* ctx, double tend, double* dtv, finess_stats* stats);
+
+/* Advances exactly nsteps accepted steps with the CFL controller (no frame
+ * clipping, no nv limit).  dtv as above. */
+int finess_advance_steps(finess_ctx* ctx, long long nsteps, double* dtv, finess_stats* stats);
+
+/* Runs exactly nsteps accepted steps of size dt (no frame clipping, no CFL
+ * rejection; the CFL of each step is still computed).  Used for timing. */
+int finess_run_fixed_steps(finess_ctx* ctx, double dt, int nsteps, double* last_cfl);
+
+/* Device synchronisation of the context's stream. */
+int finess_sync(finess_ctx* ctx);
+
+/* Time of the last kernel-timed region (ms), measured with HIP events on the
+ * context's stream, and the number of stage launches it covered. */
+int finess_last_kernel_time(finess_ctx* ctx, double* ms, long long* launches);
+
+/* ---- library info --------------------------------------------------------- */
+const char* finess_version(void);
+int finess_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FINESS_HIP_H */

@@ -1,0 +1,987 @@
+/*
+ * finess_oracle.c -- CPU restatement of FINESS's finite-difference WENO
+ * operator and time steppers (the checker for the HIP path; TEST
+ * INFRASTRUCTURE ONLY -- nothing in the product links or calls this file).
+ *
+ * It follows the reference's loop order, per-interface callback granularity
+ * and floating-point operation order (pow() where the reference calls pow),
+ * with OpenMP over the slow index as the reference does.  Each function cites
+ * the reference lines it restates.  Differences from the reference:
+ *   - no per-interface heap temporaries (the reference allocates ~20 dTensor
+ *     objects per interface, lib/tensors.cpp:188-196): stack arrays instead;
+ *   - error paths return status codes instead of exit(1);
+ *   - ghost entries of Lstar are zero instead of uninitialized, and corner
+ *     ghosts are always defined (the reference's transmissive BC leaves them
+ *     unset, apps/2d/euler/riemann_test/SetBndValues.cpp:114);
+ *   - Euler Jacobian/Hessian are evaluated exactly with hyper-dual numbers
+ *     instead of the closed forms of apps/2d/euler/lib/{DFluxFunc,D2FluxFunc}.cpp;
+ *     both are the derivatives of FluxFunc.  They coincide wherever rho*w == 0;
+ *     for rho*w != 0 the reference's DFluxFunc has terms pow(q4/q1,3) that
+ *     differ from the derivative (DFluxFunc.cpp:43,62,68).
+ */
+#include "finess_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXEQ 5
+
+/* ---------------------------------------------------------------------- */
+/* layout: dTensorBC3(s1,s2,s3,mbc) with ghosts on the first two indices   */
+/* (lib/tensors.cpp:650-667), 1-based.                                      */
+/* ---------------------------------------------------------------------- */
+static inline size_t ix3(int i, int j, int m, int s2, int s3, int mbc)
+{
+    return ((size_t)(i + mbc - 1) * (size_t)(s2 + 2 * mbc) + (size_t)(j + mbc - 1)) * (size_t)s3
+           + (size_t)(m - 1);
+}
+static inline size_t ix2(int i, int m, int s2, int mbc)
+{
+    return (size_t)(i + mbc - 1) * (size_t)s2 + (size_t)(m - 1);
+}
+static inline double Max(double a, double b) { return a > b ? a : b; } /* lib/dog_math.h:18-22 */
+static inline double Min(double a, double b) { return a < b ? a : b; }
+
+static size_t state_numel(const finess_params* p, int fields)
+{
+    if (p->ndims == 1) return (size_t)(p->mx + 2 * p->mbc) * (size_t)fields;
+    return (size_t)(p->mx + 2 * p->mbc) * (size_t)(p->my + 2 * p->mbc) * (size_t)fields;
+}
+static int naux(const finess_params* p) { return p->maux > 0 ? p->maux : 1; } /* StateVars.h:9-53 */
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ====================================================================== */
+/* WENO reconstructions                                                     */
+/* ====================================================================== */
+
+/* WenoReconstruct_JS5, lib/WenoReconstruct.cpp:90-141 */
+static double weno_js5(double eps, double pw, const double* g)
+{
+    const double uim2 = g[0], uim1 = g[1], ui = g[2], uip1 = g[3], uip2 = g[4];
+    const double beta0 = (13. / 12.) * pow(uim2 - 2 * uim1 + ui, 2) + 0.25 * pow(uim2 - 4 * uim1 + 3 * ui, 2);
+    const double beta1 = (13. / 12.) * pow(uim1 - 2 * ui + uip1, 2) + 0.25 * pow(uim1 - uip1, 2);
+    const double beta2 = (13. / 12.) * pow(ui - 2 * uip1 + uip2, 2) + 0.25 * pow(3 * ui - 4 * uip1 + uip2, 2);
+    const double u1 = (1. / 3.) * uim2 - (7. / 6.) * uim1 + (11. / 6.) * ui;
+    const double u2 = (-1. / 6.) * uim1 + (5. / 6.) * ui + (1. / 3.) * uip1;
+    const double u3 = (1. / 3.) * ui + (5. / 6.) * uip1 - (1. / 6.) * uip2;
+    const double omt0 = 0.1 * pow(eps + beta0, -pw);
+    const double omt1 = 0.6 * pow(eps + beta1, -pw);
+    const double omt2 = 0.3 * pow(eps + beta2, -pw);
+    const double omts = omt0 + omt1 + omt2;
+    return (omt0 * u1 + omt1 * u2 + omt2 * u3) / omts;
+}
+
+void orc_weno5_js_weights(double eps, double pw, const double* g, double* w)
+{
+    const double uim2 = g[0], uim1 = g[1], ui = g[2], uip1 = g[3], uip2 = g[4];
+    const double beta0 = (13. / 12.) * pow(uim2 - 2 * uim1 + ui, 2) + 0.25 * pow(uim2 - 4 * uim1 + 3 * ui, 2);
+    const double beta1 = (13. / 12.) * pow(uim1 - 2 * ui + uip1, 2) + 0.25 * pow(uim1 - uip1, 2);
+    const double beta2 = (13. / 12.) * pow(ui - 2 * uip1 + uip2, 2) + 0.25 * pow(3 * ui - 4 * uip1 + uip2, 2);
+    const double omt0 = 0.1 * pow(eps + beta0, -pw);
+    const double omt1 = 0.6 * pow(eps + beta1, -pw);
+    const double omt2 = 0.3 * pow(eps + beta2, -pw);
+    const double omts = omt0 + omt1 + omt2;
+    w[0] = omt0 / omts;
+    w[1] = omt1 / omts;
+    w[2] = omt2 / omts;
+}
+
+/* WenoReconstruct_Z5, lib/WenoReconstruct.cpp:385-438 */
+static double weno_z5(double eps, double pw, const double* g)
+{
+    const double uim2 = g[0], uim1 = g[1], ui = g[2], uip1 = g[3], uip2 = g[4];
+    const double beta0 = (13. / 12.) * pow(uim2 - 2 * uim1 + ui, 2) + 0.25 * pow(uim2 - 4 * uim1 + 3 * ui, 2);
+    const double beta1 = (13. / 12.) * pow(uim1 - 2 * ui + uip1, 2) + 0.25 * pow(uim1 - uip1, 2);
+    const double beta2 = (13. / 12.) * pow(ui - 2 * uip1 + uip2, 2) + 0.25 * pow(3 * ui - 4 * uip1 + uip2, 2);
+    const double tau5 = fabs(beta0 - beta2);
+    const double u1 = (1. / 3.) * uim2 - (7. / 6.) * uim1 + (11. / 6.) * ui;
+    const double u2 = (-1. / 6.) * uim1 + (5. / 6.) * ui + (1. / 3.) * uip1;
+    const double u3 = (1. / 3.) * ui + (5. / 6.) * uip1 - (1. / 6.) * uip2;
+    const double omt0 = 0.1 * (1. + pow(tau5 / (eps + beta0), pw));
+    const double omt1 = 0.6 * (1. + pow(tau5 / (eps + beta1), pw));
+    const double omt2 = 0.3 * (1. + pow(tau5 / (eps + beta2), pw));
+    const double omts = omt0 + omt1 + omt2;
+    return (omt0 * u1 + omt1 * u2 + omt2 * u3) / omts;
+}
+
+/* WenoReconstruct_FD5, lib/WenoReconstruct.cpp:580-616 */
+static double weno_fd5(const double* g)
+{
+    const double uim2 = g[0], uim1 = g[1], ui = g[2], uip1 = g[3], uip2 = g[4];
+    const double u1 = (1. / 3.) * uim2 - (7. / 6.) * uim1 + (11. / 6.) * ui;
+    const double u2 = (-1. / 6.) * uim1 + (5. / 6.) * ui + (1. / 3.) * uip1;
+    const double u3 = (1. / 3.) * ui + (5. / 6.) * uip1 - (1. / 6.) * uip2;
+    return 0.1 * u1 + 0.6 * u2 + 0.3 * u3;
+}
+
+double orc_weno5(int version, double eps, double pw, const double* g)
+{
+    switch (version) {
+    case FINESS_WENO_FD: return weno_fd5(g);
+    case FINESS_WENO_Z: return weno_z5(eps, pw, g);
+    default: return weno_js5(eps, pw, g);
+    }
+}
+
+/* ====================================================================== */
+/* physics callbacks                                                        */
+/* ====================================================================== */
+
+/* FluxFunc: Euler apps/2d/euler/lib/FluxFunc.cpp:9-42, shallow water
+ * apps/2d/shallow_water/lib/FluxFunc.cpp:8-35, advection
+ * apps/1d/advection/lib/FluxFunc.cpp (f = q*aux(1)). */
+void orc_flux(const finess_params* p, const double* Q, const double* aux, double* F, double* G)
+{
+    if (p->physics == FINESS_PHYS_EULER) {
+        const double gamma = p->gamma;
+        const double rho = Q[0];
+        const double u1 = Q[1] / rho, u2 = Q[2] / rho, u3 = Q[3] / rho;
+        const double energy = Q[4];
+        const double press = (gamma - 1.0e0) * (energy - 0.5e0 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+        F[0] = rho * u1;
+        F[1] = rho * u1 * u1 + press;
+        F[2] = rho * u1 * u2;
+        F[3] = rho * u1 * u3;
+        F[4] = u1 * (energy + press);
+        if (G) {
+            G[0] = rho * u2;
+            G[1] = rho * u2 * u1;
+            G[2] = rho * u2 * u2 + press;
+            G[3] = rho * u2 * u3;
+            G[4] = u2 * (energy + press);
+        }
+    } else if (p->physics == FINESS_PHYS_SHALLOW_WATER) {
+        const double h = Q[0], u1 = Q[1] / h, u2 = Q[2] / h;
+        F[0] = h * u1;
+        F[1] = h * u1 * u1 + 0.5 * h * h;
+        F[2] = h * u1 * u2;
+        if (G) {
+            G[0] = h * u2;
+            G[1] = h * u2 * u1;
+            G[2] = h * u2 * u2 + 0.5 * h * h;
+        }
+    } else { /* advection */
+        F[0] = Q[0] * aux[0];
+        if (G) G[0] = 0.0;
+    }
+}
+
+/* ProjectLeftEig: Euler apps/2d/euler/lib/ProjectLeftEig.cpp:9-67,
+ * shallow water apps/2d/shallow_water/lib/ProjectLeftEig.cpp,
+ * advection apps/1d/advection/lib/ProjectLeftEig.cpp (identity). */
+void orc_project_left(const finess_params* p, int ixy, const double* Qa, int npts,
+                      const double* Qv, double* Wv)
+{
+#define QV(m, k) Qv[(size_t)((m)-1) * npts + (k)-1]
+#define WV(m, k) Wv[(size_t)((m)-1) * npts + (k)-1]
+    if (p->physics == FINESS_PHYS_EULER) {
+        const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2, mw = 4;
+        const double gamma = p->gamma;
+        const double rho = Qa[0];
+        const double u1 = Qa[mu - 1] / rho, u2 = Qa[mv - 1] / rho, u3 = Qa[mw - 1] / rho;
+        const double energy = Qa[4];
+        const double umag2 = (u1 * u1 + u2 * u2 + u3 * u3);
+        const double press = (gamma - 1.0e0) * (energy - 0.5e0 * rho * umag2);
+        const double c = sqrt(gamma * press / rho);
+        const double H = (energy + press) / rho;
+        for (int k = 1; k <= npts; k++) {
+            WV(1, k) = ((umag2 / 2.0 - H - u1 * c) * QV(mu, k) + (umag2 / 2.0 * (c - u1) + H * u1) * QV(1, k)
+                        + c * (QV(5, k) - u2 * QV(mv, k) - u3 * QV(mw, k))) / (c * (2.0 * H - umag2));
+            WV(mu, k) = 2.0 * ((H - umag2) * QV(1, k) + u1 * QV(mu, k) + u2 * QV(mv, k) + u3 * QV(mw, k) - QV(5, k))
+                        / (2.0 * H - umag2);
+            WV(mv, k) = QV(mv, k) - u2 * QV(1, k);
+            WV(mw, k) = QV(mw, k) - u3 * QV(1, k);
+            WV(5, k) = ((H - umag2 / 2.0 - u1 * c) * QV(mu, k) + (umag2 / 2.0 * (c + u1) - H * u1) * QV(1, k)
+                        + c * (QV(5, k) - u2 * QV(mv, k) - u3 * QV(mw, k))) / (c * (2.0 * H - umag2));
+        }
+    } else if (p->physics == FINESS_PHYS_SHALLOW_WATER) {
+        const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2;
+        const double h = Qa[0], u1 = Qa[mu - 1] / h, u2 = Qa[mv - 1] / h;
+        for (int k = 1; k <= npts; k++) {
+            WV(1, k) = (sqrt(h) + u1) / (2.0 * sqrt(h)) * QV(1, k) - (0.5 / sqrt(h)) * QV(mu, k);
+            WV(2, k) = -u2 * QV(1, k) + QV(mv, k);
+            WV(3, k) = (sqrt(h) - u1) / (2.0 * sqrt(h)) * QV(1, k) + (0.5 / sqrt(h)) * QV(mu, k);
+        }
+    } else {
+        for (int k = 1; k <= npts; k++) WV(1, k) = QV(1, k);
+    }
+#undef QV
+#undef WV
+}
+
+/* ProjectRightEig: Euler apps/2d/euler/lib/ProjectRightEig.cpp:9-63,
+ * shallow water apps/2d/shallow_water/lib/ProjectRightEig.cpp. */
+void orc_project_right(const finess_params* p, int ixy, const double* Qa, int npts,
+                       const double* Wv, double* Qv)
+{
+#define QV(m, k) Qv[(size_t)((m)-1) * npts + (k)-1]
+#define WV(m, k) Wv[(size_t)((m)-1) * npts + (k)-1]
+    if (p->physics == FINESS_PHYS_EULER) {
+        const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2, mw = 4;
+        const double gamma = p->gamma;
+        const double rho = Qa[0];
+        const double u1 = Qa[mu - 1] / rho, u2 = Qa[mv - 1] / rho, u3 = Qa[mw - 1] / rho;
+        const double energy = Qa[4];
+        const double umag2 = (u1 * u1 + u2 * u2 + u3 * u3);
+        const double press = (gamma - 1.0e0) * (energy - 0.5e0 * rho * umag2);
+        const double c = sqrt(fabs(gamma * press / rho));
+        const double H = (energy + press) / rho;
+        for (int k = 1; k <= npts; k++) {
+            QV(1, k) = WV(1, k) + WV(mu, k) + WV(5, k);
+            QV(mu, k) = (u1 - c) * WV(1, k) + u1 * WV(mu, k) + (u1 + c) * WV(5, k);
+            QV(mv, k) = u2 * (WV(1, k) + WV(mu, k) + WV(5, k)) + WV(mv, k);
+            QV(mw, k) = u3 * (WV(1, k) + WV(mu, k) + WV(5, k)) + WV(mw, k);
+            QV(5, k) = (H - u1 * c) * WV(1, k) + umag2 / 2.0 * WV(mu, k) + u2 * WV(mv, k) + u3 * WV(mw, k)
+                       + (H + u1 * c) * WV(5, k);
+        }
+    } else if (p->physics == FINESS_PHYS_SHALLOW_WATER) {
+        const int mu = ixy == 1 ? 2 : 3, mv = ixy == 1 ? 3 : 2;
+        const double h = Qa[0], u1 = Qa[mu - 1] / h, u2 = Qa[mv - 1] / h;
+        for (int k = 1; k <= npts; k++) {
+            QV(1, k) = WV(1, k) + WV(3, k);
+            QV(mu, k) = (u1 - sqrt(h)) * WV(1, k) + (u1 + sqrt(h)) * WV(3, k);
+            QV(mv, k) = u2 * WV(1, k) + WV(2, k) + u2 * WV(3, k);
+        }
+    } else {
+        for (int k = 1; k <= npts; k++) QV(1, k) = WV(1, k);
+    }
+#undef QV
+#undef WV
+}
+
+/* SetWaveSpd: Euler apps/2d/euler/lib/SetWaveSpd.cpp:11-69, shallow water
+ * apps/2d/shallow_water/lib/SetWaveSpd.cpp:9-48 (u_r from h_l when
+ * sw_compat_hl, as at :27-28), advection apps/1d/advection/lib/SetWaveSpd.cpp. */
+void orc_wave_speed(const finess_params* p, int ixy, const double* Ql, const double* Qr,
+                    const double* auxl, const double* auxr, double* s1, double* s2)
+{
+    const double n1 = ixy == 1 ? 1.0 : 0.0, n2 = ixy == 1 ? 0.0 : 1.0;
+    if (p->physics == FINESS_PHYS_EULER) {
+        const double gamma = p->gamma;
+        const double rhol = Ql[0];
+        const double u1l = Ql[1] / rhol, u2l = Ql[2] / rhol, u3l = Ql[3] / rhol;
+        const double energyl = Ql[4];
+        const double pressl = (gamma - 1.0e0) * (energyl - 0.5e0 * rhol * (u1l * u1l + u2l * u2l + u3l * u3l));
+        const double rhor = Qr[0];
+        const double u1r = Qr[1] / rhor, u2r = Qr[2] / rhor, u3r = Qr[3] / rhor;
+        const double energyr = Qr[4];
+        const double pressr = (gamma - 1.0e0) * (energyr - 0.5e0 * rhor * (u1r * u1r + u2r * u2r + u3r * u3r));
+        const double rho = 0.5e0 * (rhol + rhor);
+        const double u1 = 0.5e0 * (u1l + u1r);
+        const double u2 = 0.5e0 * (u2l + u2r);
+        const double press = 0.5e0 * (pressl + pressr);
+        const double nmag = sqrt(pow(n1, 2) + pow(n2, 2));
+        const double cl = nmag * sqrt(fabs(gamma * pressl / rhol));
+        const double cr = nmag * sqrt(fabs(gamma * pressr / rhor));
+        const double c = nmag * sqrt(fabs(gamma * press / rho));
+        const double un = n1 * u1 + n2 * u2;
+        const double unl = n1 * u1l + n2 * u2l;
+        const double unr = n1 * u1r + n2 * u2r;
+        *s1 = Min(unl - cl, un - c);
+        *s2 = Max(unr + cr, un + c);
+    } else if (p->physics == FINESS_PHYS_SHALLOW_WATER) {
+        const double hl = Ql[0], u1l = Ql[1] / hl, u2l = Ql[2] / hl;
+        const double unl = u1l * n1 + u2l * n2;
+        const double hr = Qr[0];
+        const double hden = p->sw_compat_hl ? hl : hr;
+        const double u1r = Qr[1] / hden, u2r = Qr[2] / hden;
+        const double unr = u1r * n1 + u2r * n2;
+        const double h = 0.5 * (hl + hr);
+        const double u1 = (sqrt(hl) * u1l + sqrt(hr) * u1r) / (sqrt(hl) + sqrt(hr));
+        const double u2 = (sqrt(hl) * u2l + sqrt(hr) * u2r) / (sqrt(hl) + sqrt(hr));
+        const double un = u1 * n1 + u2 * n2;
+        *s1 = Min(un - sqrt(h), unl - sqrt(hl));
+        *s2 = Max(un + sqrt(h), unr + sqrt(hr));
+    } else {
+        (void)Ql;
+        (void)Qr;
+        *s1 = Min(auxl[0], 0.5e0 * (auxl[0] + auxr[0]));
+        *s2 = Max(auxr[0], 0.5e0 * (auxl[0] + auxr[0]));
+    }
+}
+
+/* ---- hyper-dual numbers: exact first and second directional derivatives */
+typedef struct { double v, a, b, ab; } hd;
+static inline hd hd_c(double x) { hd r = {x, 0, 0, 0}; return r; }
+static inline hd hd_add(hd x, hd y) { hd r = {x.v + y.v, x.a + y.a, x.b + y.b, x.ab + y.ab}; return r; }
+static inline hd hd_sub(hd x, hd y) { hd r = {x.v - y.v, x.a - y.a, x.b - y.b, x.ab - y.ab}; return r; }
+static inline hd hd_scale(double s, hd x) { hd r = {s * x.v, s * x.a, s * x.b, s * x.ab}; return r; }
+static inline hd hd_mul(hd x, hd y)
+{
+    hd r = {x.v * y.v, x.v * y.a + x.a * y.v, x.v * y.b + x.b * y.v,
+            x.v * y.ab + x.a * y.b + x.b * y.a + x.ab * y.v};
+    return r;
+}
+static inline hd hd_inv(hd x)
+{
+    const double i = 1.0 / x.v, i2 = i * i, i3 = i2 * i;
+    hd r = {i, -x.a * i2, -x.b * i2, -x.ab * i2 + 2.0 * x.a * x.b * i3};
+    return r;
+}
+static inline hd hd_div(hd x, hd y) { return hd_mul(x, hd_inv(y)); }
+
+/* Flux of the configured physics in hyper-dual arithmetic (same expressions
+ * as orc_flux). */
+static void flux_hd(const finess_params* p, const hd* Q, hd* F, hd* G)
+{
+    if (p->physics == FINESS_PHYS_EULER) {
+        const double gm1 = p->gamma - 1.0;
+        const hd rho = Q[0];
+        const hd u1 = hd_div(Q[1], rho), u2 = hd_div(Q[2], rho), u3 = hd_div(Q[3], rho);
+        const hd e = Q[4];
+        const hd ke = hd_add(hd_add(hd_mul(u1, u1), hd_mul(u2, u2)), hd_mul(u3, u3));
+        const hd press = hd_scale(gm1, hd_sub(e, hd_scale(0.5, hd_mul(rho, ke))));
+        const hd ru1 = hd_mul(rho, u1), ru2 = hd_mul(rho, u2), ep = hd_add(e, press);
+        F[0] = ru1;
+        F[1] = hd_add(hd_mul(ru1, u1), press);
+        F[2] = hd_mul(ru1, u2);
+        F[3] = hd_mul(ru1, u3);
+        F[4] = hd_mul(u1, ep);
+        G[0] = ru2;
+        G[1] = hd_mul(ru2, u1);
+        G[2] = hd_add(hd_mul(ru2, u2), press);
+        G[3] = hd_mul(ru2, u3);
+        G[4] = hd_mul(u2, ep);
+    } else if (p->physics == FINESS_PHYS_SHALLOW_WATER) {
+        const hd h = Q[0], u1 = hd_div(Q[1], h), u2 = hd_div(Q[2], h);
+        const hd hh = hd_scale(0.5, hd_mul(h, h));
+        const hd hu1 = hd_mul(h, u1), hu2 = hd_mul(h, u2);
+        F[0] = hu1;
+        F[1] = hd_add(hd_mul(hu1, u1), hh);
+        F[2] = hd_mul(hu1, u2);
+        G[0] = hu2;
+        G[1] = hd_mul(hu2, u1);
+        G[2] = hd_add(hd_mul(hu2, u2), hh);
+    } else {
+        F[0] = Q[0];
+        G[0] = hd_c(0.0);
+    }
+}
+
+/* DFluxFunc: A[d][m1][m2] = d flux_d(m1) / d q(m2). */
+void orc_dflux(const finess_params* p, const double* Q, double* A)
+{
+    const int meqn = p->meqn;
+    for (int m2 = 0; m2 < meqn; m2++) {
+        hd q[MAXEQ], F[MAXEQ], G[MAXEQ];
+        for (int m = 0; m < meqn; m++) q[m] = hd_c(Q[m]);
+        q[m2].a = 1.0;
+        flux_hd(p, q, F, G);
+        for (int m1 = 0; m1 < meqn; m1++) {
+            A[(0 * meqn + m1) * meqn + m2] = F[m1].a;
+            A[(1 * meqn + m1) * meqn + m2] = G[m1].a;
+        }
+    }
+}
+
+/* D2FluxFunc: H[d][m][m1][m2] = d^2 flux_d(m) / d q(m1) d q(m2). */
+void orc_d2flux(const finess_params* p, const double* Q, double* H)
+{
+    const int meqn = p->meqn;
+    for (int m1 = 0; m1 < meqn; m1++)
+        for (int m2 = 0; m2 < meqn; m2++) {
+            hd q[MAXEQ], F[MAXEQ], G[MAXEQ];
+            for (int m = 0; m < meqn; m++) q[m] = hd_c(Q[m]);
+            q[m1].a += 1.0;
+            q[m2].b += 1.0;
+            flux_hd(p, q, F, G);
+            for (int m = 0; m < meqn; m++) {
+                H[((0 * meqn + m) * meqn + m1) * meqn + m2] = F[m].ab;
+                H[((1 * meqn + m) * meqn + m1) * meqn + m2] = G[m].ab;
+            }
+        }
+}
+
+/* ====================================================================== */
+/* boundary conditions                                                      */
+/* ====================================================================== */
+
+/* Source index of a ghost index g along an axis of n interior cells. */
+static int bc_src(int bc, int g, int n, int* flip)
+{
+    *flip = 0;
+    if (g >= 1 && g <= n) return g;
+    if (bc == FINESS_BC_PERIODIC) return g < 1 ? g + n : g - n;
+    if (bc == FINESS_BC_REFLECT) {
+        *flip = 1;
+        return g < 1 ? 1 - g : 2 * n + 1 - g;
+    }
+    return g < 1 ? 1 : n; /* zeroth-order extrapolation */
+}
+
+/* SetBndValues.  2D: sides as apps/2d/euler/riemann_test/SetBndValues.cpp:9-114
+ * (extrapolation; also copies q(.,m) into aux(.,m), :37-44) and
+ * apps/2d/advection/smooth_example/SetBndValues.cpp:28-194 (periodic with
+ * corners).  Corners are filled by applying the x rule then the y rule.
+ * 1D: apps/1d/advection/smooth_example/SetBndValues.cpp:6-63. */
+void orc_set_bnd(const finess_params* p, double* q, double* aux)
+{
+    const int mx = p->mx, mbc = p->mbc, meqn = p->meqn, na = naux(p);
+    if (p->ndims == 1) {
+        for (int i = 1 - mbc; i <= mx + mbc; i++) {
+            if (i >= 1 && i <= mx) continue;
+            int fl;
+            const int bcs = i < 1 ? p->bc[0] : p->bc[1];
+            const int si = bc_src(bcs, i, mx, &fl);
+            for (int m = 1; m <= meqn; m++) q[ix2(i, m, meqn, mbc)] = q[ix2(si, m, meqn, mbc)];
+            if (aux && bcs == FINESS_BC_PERIODIC)
+                for (int m = 1; m <= na; m++) aux[ix2(i, m, na, mbc)] = aux[ix2(si, m, na, mbc)];
+        }
+        return;
+    }
+    const int my = p->my;
+    const int mu_x = 2, mu_y = 3; /* momentum component normal to x / y walls */
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            if (i >= 1 && i <= mx && j >= 1 && j <= my) continue;
+            int fx, fy;
+            const int bx = i < 1 ? p->bc[0] : p->bc[1];
+            const int by = j < 1 ? p->bc[2] : p->bc[3];
+            const int si = bc_src(bx, i, mx, &fx);
+            const int sj = bc_src(by, j, my, &fy);
+            for (int m = 1; m <= meqn; m++) {
+                double v = q[ix3(si, sj, m, my, meqn, mbc)];
+                if (p->physics != FINESS_PHYS_ADVECTION) {
+                    if (fx && m == mu_x) v = -v;
+                    if (fy && m == mu_y) v = -v;
+                }
+                q[ix3(i, j, m, my, meqn, mbc)] = v;
+            }
+            if (aux) {
+                const int inside_x = (i >= 1 && i <= mx), inside_y = (j >= 1 && j <= my);
+                for (int m = 1; m <= na; m++) {
+                    double v;
+                    if (bx == FINESS_BC_EXTRAP && !inside_x && inside_y && m <= p->maux)
+                        v = q[ix3(si, sj, m, my, meqn, mbc)]; /* riemann_test :37-44 quirk */
+                    else
+                        v = aux[ix3(si, sj, m, my, na, mbc)];
+                    aux[ix3(i, j, m, my, na, mbc)] = v;
+                }
+            }
+        }
+}
+
+/* ====================================================================== */
+/* ConstructL                                                               */
+/* ====================================================================== */
+
+static int check_config(const finess_params* p)
+{
+    if (p->space_order != 5) return FINESS_ERR_CONFIG;
+    if (p->weno_version != FINESS_WENO_JS && p->weno_version != FINESS_WENO_FD && p->weno_version != FINESS_WENO_Z)
+        return FINESS_ERR_CONFIG;
+    if (p->mbc < 3) return FINESS_ERR_CONFIG;
+    if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
+    if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
+    return FINESS_OK;
+}
+
+/* One interface: characteristic-wise WENO with local Lax-Friedrichs splitting
+ * (lib/2d/ConstructL.cpp:63-208 for ixy=1, :215-358 for ixy=2).
+ * qs[s][m], fs[s][m]: the 6 stencil states / fluxes (cells i-3..i+2),
+ * Ql = qs[2], Qr = qs[3]. Returns alpha; writes fhat[meqn]. */
+static double interface_flux(const finess_params* p, int ixy, const double qs[6][MAXEQ],
+                             const double fs[6][MAXEQ], const double* auxl, const double* auxr,
+                             const double* auxavg, double* fhat)
+{
+    const int meqn = p->meqn, ws = 5;
+    double Qavg[MAXEQ];
+    for (int m = 0; m < meqn; m++) Qavg[m] = 0.5 * (qs[3][m] + qs[2][m]);
+    (void)auxavg;
+    double qvals[MAXEQ * 6], fvals[MAXEQ * 6], wvals[MAXEQ * 6], gvals[MAXEQ * 6];
+    for (int m = 0; m < meqn; m++)
+        for (int s = 0; s < 6; s++) {
+            qvals[m * 6 + s] = qs[s][m];
+            fvals[m * 6 + s] = fs[s][m];
+        }
+    orc_project_left(p, ixy, Qavg, 6, qvals, wvals);
+    orc_project_left(p, ixy, Qavg, 6, fvals, gvals);
+    double s1, s2;
+    orc_wave_speed(p, ixy, qs[2], qs[3], auxl, auxr, &s1, &s2);
+    const double alpha = Max(0.0, Max(fabs(s1), fabs(s2)));
+    const double l_alpha = p->alpha_scaling * alpha;
+    double ghat[MAXEQ];
+    for (int m = 0; m < meqn; m++) {
+        double gp[5], gm[5];
+        for (int s = 1; s <= ws; s++) {
+            gp[s - 1] = 0.5 * (gvals[m * 6 + s - 1] + l_alpha * wvals[m * 6 + s - 1]);
+            gm[s - 1] = 0.5 * (gvals[m * 6 + ws - s + 1] - l_alpha * wvals[m * 6 + ws - s + 1]);
+        }
+        const double dGp = orc_weno5(p->weno_version, p->epsilon, p->power_param, gp);
+        const double dGm = orc_weno5(p->weno_version, p->epsilon, p->power_param, gm);
+        ghat[m] = dGp + dGm;
+    }
+    orc_project_right(p, ixy, Qavg, 1, ghat, fhat);
+    return alpha;
+}
+
+/* 2D ConstructL, lib/2d/ConstructL.cpp:13-408.  If Fsrc/Gsrc are non-NULL the
+ * stencil fluxes are read from them (ConstructLxWL, lib/2d/ConstructLxWL.cpp:
+ * 106-236) instead of FluxFunc. */
+static int construct_l_2d(const finess_params* p, const double* q, const double* aux,
+                          const double* Fsrc, const double* Gsrc, double* Lstar, double* smax)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    const double dx = p->dx, dy = p->dy;
+    double* Fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn, sizeof(double));
+    double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
+    if (!Fhat || !Ghat) { free(Fhat); free(Ghat); return FINESS_ERR_ARG; }
+
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx + 1; i++)
+        for (int j = 1; j <= my; j++) {
+            double qs[6][MAXEQ], fs[6][MAXEQ], gdum[MAXEQ];
+            for (int s = 1; s <= 6; s++) {
+                const int is = i - 1 + s - 3;
+                for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix3(is, j, m, my, meqn, mbc)];
+                if (Fsrc)
+                    for (int m = 1; m <= meqn; m++) fs[s - 1][m - 1] = Fsrc[ix3(is, j, m, my, meqn, mbc)];
+                else
+                    orc_flux(p, qs[s - 1], &aux[ix3(is, j, 1, my, na, mbc)], fs[s - 1], gdum);
+            }
+            double fh[MAXEQ];
+            const double alpha = interface_flux(p, 1, qs, fs, &aux[ix3(i - 1, j, 1, my, na, mbc)],
+                                                &aux[ix3(i, j, 1, my, na, mbc)], NULL, fh);
+            double* sm = &smax[ix3(i, j, 1, my + 1, 2, mbc)];
+            *sm = Max(*sm, alpha);
+            for (int m = 1; m <= meqn; m++) Fhat[ix3(i, j, m, my, meqn, mbc)] = fh[m - 1];
+        }
+
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my + 1; j++) {
+            double qs[6][MAXEQ], fs[6][MAXEQ], fdum[MAXEQ];
+            for (int s = 1; s <= 6; s++) {
+                const int js = j - 1 + s - 3;
+                for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix3(i, js, m, my, meqn, mbc)];
+                if (Gsrc)
+                    for (int m = 1; m <= meqn; m++) fs[s - 1][m - 1] = Gsrc[ix3(i, js, m, my, meqn, mbc)];
+                else
+                    orc_flux(p, qs[s - 1], &aux[ix3(i, js, 1, my, na, mbc)], fdum, fs[s - 1]);
+            }
+            double gh[MAXEQ];
+            const double alpha = interface_flux(p, 2, qs, fs, &aux[ix3(i, j - 1, 1, my, na, mbc)],
+                                                &aux[ix3(i, j, 1, my, na, mbc)], NULL, gh);
+            double* sm = &smax[ix3(i, j, 2, my + 1, 2, mbc)];
+            *sm = Max(*sm, alpha);
+            for (int m = 1; m <= meqn; m++) Ghat[ix3(i, j, m, my + 1, meqn, mbc)] = gh[m - 1];
+        }
+
+    /* Lstar = -(Fhat(i+1)-Fhat(i))/dx - (Ghat(j+1)-Ghat(j))/dy, :388-399 */
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my; j++)
+            for (int m = 1; m <= meqn; m++) {
+                double tmp = -(Fhat[ix3(i + 1, j, m, my, meqn, mbc)] - Fhat[ix3(i, j, m, my, meqn, mbc)]) / dx;
+                tmp = tmp - (Ghat[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Ghat[ix3(i, j, m, my + 1, meqn, mbc)]) / dy;
+                Lstar[ix3(i, j, m, my, meqn, mbc)] = tmp;
+            }
+    free(Fhat);
+    free(Ghat);
+    return FINESS_OK;
+}
+
+/* 1D ConstructL, lib/1d/ConstructL.cpp:16-224; smax(i) is overwritten (:149). */
+static int construct_l_1d(const finess_params* p, const double* q, const double* aux,
+                          double* Lstar, double* smax)
+{
+    const int mx = p->mx, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    const double dx = p->dx;
+    double* fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * meqn, sizeof(double));
+    if (!fhat) return FINESS_ERR_ARG;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx + 1; i++) {
+        double qs[6][MAXEQ], fs[6][MAXEQ];
+        for (int s = 1; s <= 6; s++) {
+            const int is = i - 1 + s - 3;
+            for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix2(is, m, meqn, mbc)];
+            orc_flux(p, qs[s - 1], &aux[ix2(is, 1, na, mbc)], fs[s - 1], NULL);
+        }
+        double fh[MAXEQ];
+        const double alpha = interface_flux(p, 1, qs, fs, &aux[ix2(i - 1, 1, na, mbc)],
+                                            &aux[ix2(i, 1, na, mbc)], NULL, fh);
+        smax[i + mbc - 1] = alpha;
+        for (int m = 1; m <= meqn; m++) fhat[ix2(i, m, meqn, mbc)] = fh[m - 1];
+    }
+    for (int i = 1; i <= mx; i++)
+        for (int m = 1; m <= meqn; m++)
+            Lstar[ix2(i, m, meqn, mbc)] = -(fhat[ix2(i + 1, m, meqn, mbc)] - fhat[ix2(i, m, meqn, mbc)]) / dx;
+    free(fhat);
+    return FINESS_OK;
+}
+
+int orc_construct_l(const finess_params* p, const double* q, const double* aux, double* Lstar, double* smax)
+{
+    const int rc = check_config(p);
+    if (rc) return rc;
+    if (p->ndims == 1) return construct_l_1d(p, q, aux, Lstar, smax);
+    return construct_l_2d(p, q, aux, NULL, NULL, Lstar, smax);
+}
+
+/* ====================================================================== */
+/* CFL, update, steppers                                                    */
+/* ====================================================================== */
+
+int orc_get_cfl(const finess_params* p, double dt, double dtmax, const double* smax, double* cfl_out)
+{
+    double cfl = -100.0;
+    if (dt > dtmax) return FINESS_ERR_DT;
+    if (p->ndims == 1) { /* lib/1d/GetCFL.cpp:9-45 */
+        for (int j = 1; j <= p->mx; j++) cfl = Max(dt * smax[j + p->mbc - 1] / p->dx, cfl);
+        *cfl_out = cfl;
+        if (cfl > 1.0e10) return FINESS_ERR_CFL;
+        return FINESS_OK;
+    }
+    /* lib/2d/GetCFL.cpp:31-74: i=1..mx, j=1..my only */
+    for (int i = 1; i <= p->mx; i++)
+        for (int j = 1; j <= p->my; j++) {
+            cfl = Max(dt * smax[ix3(i, j, 1, p->my + 1, 2, p->mbc)] / p->dx, cfl);
+            cfl = Max(dt * smax[ix3(i, j, 2, p->my + 1, 2, p->mbc)] / p->dy, cfl);
+        }
+    *cfl_out = cfl;
+    if (cfl > 1.0e8) return FINESS_ERR_CFL;
+    return FINESS_OK;
+}
+
+/* UpdateSoln, lib/2d/UpdateSoln.cpp:14-38: qnew = a1*qstar + a2*qnew + b*dt*L
+ * over all numel entries. */
+static void update_soln(size_t numel, double a1, double a2, double b, double dt,
+                        const double* qstar, const double* L, double* qnew)
+{
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)numel; k++) {
+        const double tmp = a1 * qstar[k] + a2 * qnew[k] + b * dt * L[k];
+        qnew[k] = tmp;
+    }
+}
+
+static size_t smax_numel(const finess_params* p)
+{
+    if (p->ndims == 1) return (size_t)(p->mx + 2 * p->mbc);
+    return (size_t)(p->mx + 1 + 2 * p->mbc) * (size_t)(p->my + 1 + 2 * p->mbc) * 2;
+}
+
+/* SSP-RK coefficients, lib/2d/SetRKinfo.cpp:34-64 (orders 1-3). */
+static const double RK_A1[3][3] = {{1.0, 0, 0}, {1.0, 0.5, 0}, {1.0, 0.75, 2.0e0 / 3.0e0}};
+static const double RK_A2[3][3] = {{0.0, 0, 0}, {0.0, 0.5, 0}, {0.0, 0.25, 1.0e0 / 3.0e0}};
+static const double RK_B[3][3] = {{1.0, 0, 0}, {1.0, 0.5, 0}, {1.0, 0.25, 2.0e0 / 3.0e0}};
+
+/* One full RK step (all stages) from qnew into qnew, lib/2d/FinSolveRK.cpp:
+ * 115-211 (orders 1-3). qstar, L, smax are scratch. */
+static int rk_step(const finess_params* p, double* qnew, double* qstar, double* aux, double* L,
+                   double* smax, double dt)
+{
+    const size_t numel = state_numel(p, p->meqn);
+    const int to = p->time_order, k = to - 1;
+    int rc;
+    if (to == 1) {
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][0], RK_A2[k][0], RK_B[k][0], dt, qnew, L, qnew);
+    } else if (to == 2) {
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][0], RK_A2[k][0], RK_B[k][0], dt, qnew, L, qstar);
+        orc_set_bnd(p, qstar, aux);
+        if ((rc = orc_construct_l(p, qstar, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][1], RK_A2[k][1], RK_B[k][1], dt, qstar, L, qnew);
+    } else if (to == 3) {
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][0], RK_A2[k][0], RK_B[k][0], dt, qnew, L, qstar);
+        orc_set_bnd(p, qstar, aux);
+        if ((rc = orc_construct_l(p, qstar, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][1], RK_A2[k][1], RK_B[k][1], dt, qnew, L, qstar);
+        orc_set_bnd(p, qstar, aux);
+        if ((rc = orc_construct_l(p, qstar, aux, L, smax))) return rc;
+        update_soln(numel, RK_A1[k][2], RK_A2[k][2], RK_B[k][2], dt, qstar, L, qnew);
+    } else {
+        return FINESS_ERR_CONFIG;
+    }
+    return FINESS_OK;
+}
+
+/* One LxW step, lib/2d/FinSolveLxW.cpp:122-171 (no MPP limiter). */
+static int lxw_step(const finess_params* p, double* qnew, double* aux, double* F, double* G,
+                    double* L, double* smax, double dt)
+{
+    const size_t numel = state_numel(p, p->meqn);
+    int rc;
+    orc_set_bnd(p, qnew, aux);
+    if ((rc = orc_construct_integrated_r(p, dt, qnew, aux, F, G))) return rc;
+    if ((rc = orc_construct_lxw_l(p, qnew, aux, F, G, L, smax))) return rc;
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)numel; k++) qnew[k] = qnew[k] + dt * L[k];
+    return FINESS_OK;
+}
+
+/* Shared controller of FinSolveRK / FinSolveLxW: lib/2d/FinSolveRK.cpp:77-367,
+ * lib/2d/FinSolveLxW.cpp:100-229, lib/1d/FinSolveRK.cpp:88-363. */
+static int fin_solve(const finess_params* p, int lxw, double* qnew, double* aux, double* tio,
+                     double tend, double* dtv, orc_stats* st)
+{
+    int rc = check_config(p);
+    if (rc) return rc;
+    if (lxw && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
+    const size_t numel = state_numel(p, p->meqn);
+    double* qold = (double*)malloc(numel * sizeof(double));
+    double* qstar = (double*)malloc(numel * sizeof(double));
+    double* L = (double*)calloc(numel, sizeof(double));
+    double* smax = (double*)calloc(smax_numel(p), sizeof(double));
+    double* F = lxw ? (double*)calloc(numel, sizeof(double)) : NULL;
+    double* G = lxw ? (double*)calloc(numel, sizeof(double)) : NULL;
+    const double CFL_max = p->max_cfl, CFL_target = p->desired_cfl;
+    double t = *tio, dt = dtv[1], cfl = 0.0;
+    double dtmin = dt, dtmax = dt;
+    int n_step = 0;
+    if (st) { memset(st, 0, sizeof(*st)); st->dt_min = dt; st->dt_max = dt; }
+    memcpy(qold, qnew, numel * sizeof(double));
+    memcpy(qstar, qnew, numel * sizeof(double));
+    while (t < tend) {
+        int m_accept = 0;
+        n_step = n_step + 1;
+        if (n_step > p->nv) { rc = FINESS_ERR_NV; goto done; }
+        memcpy(qold, qnew, numel * sizeof(double));
+        while (m_accept == 0) {
+            const double told = t;
+            if (told + dt > tend) dt = tend - told;
+            t = told + dt;
+            memset(smax, 0, smax_numel(p) * sizeof(double));
+            rc = lxw ? lxw_step(p, qnew, aux, F, G, L, smax, dt) : rk_step(p, qnew, qstar, aux, L, smax, dt);
+            if (rc) goto done;
+            rc = orc_get_cfl(p, dt, dtv[2], smax, &cfl);
+            if (rc) goto done;
+            if (cfl > 0.0) {
+                dt = Min(dtv[2], dt * CFL_target / cfl);
+                dtmin = Min(dt, dtmin);
+                dtmax = Max(dt, dtmax);
+            } else {
+                dt = dtv[2];
+            }
+            if (cfl <= CFL_max) {
+                m_accept = 1;
+                if (st) st->steps_accepted++;
+            } else {
+                t = told;
+                if (st) st->steps_rejected++;
+                memcpy(qnew, qold, numel * sizeof(double));
+            }
+        }
+        if (p->ndims == 1) orc_set_bnd(p, qnew, aux); /* lib/1d/FinSolveRK.cpp:353 */
+    }
+    dtv[1] = dt;
+    rc = FINESS_OK;
+done:
+    *tio = t;
+    if (st) { st->dt_min = dtmin; st->dt_max = dtmax; st->last_cfl = cfl; }
+    free(qold); free(qstar); free(L); free(smax); free(F); free(G);
+    return rc;
+}
+
+int orc_fin_solve_rk(const finess_params* p, double* q, double* aux, double* t, double tend,
+                     double* dtv, orc_stats* st)
+{
+    return fin_solve(p, 0, q, aux, t, tend, dtv, st);
+}
+
+int orc_fin_solve_lxw(const finess_params* p, double* q, double* aux, double* t, double tend,
+                      double* dtv, orc_stats* st)
+{
+    return fin_solve(p, 1, q, aux, t, tend, dtv, st);
+}
+
+int orc_run_fixed_steps(const finess_params* p, double* q, double* aux, double dt, int nsteps, double* cfl)
+{
+    int rc = check_config(p);
+    if (rc) return rc;
+    const int lxw = p->time_stepping_method == FINESS_TS_LXW;
+    const size_t numel = state_numel(p, p->meqn);
+    double* qstar = (double*)malloc(numel * sizeof(double));
+    double* L = (double*)calloc(numel, sizeof(double));
+    double* smax = (double*)calloc(smax_numel(p), sizeof(double));
+    double* F = lxw ? (double*)calloc(numel, sizeof(double)) : NULL;
+    double* G = lxw ? (double*)calloc(numel, sizeof(double)) : NULL;
+    memcpy(qstar, q, numel * sizeof(double));
+    double c = 0.0;
+    for (int n = 0; n < nsteps && !rc; n++) {
+        memset(smax, 0, smax_numel(p) * sizeof(double));
+        rc = lxw ? lxw_step(p, q, aux, F, G, L, smax, dt) : rk_step(p, q, qstar, aux, L, smax, dt);
+        if (!rc) rc = orc_get_cfl(p, dt, 1e300, smax, &c);
+    }
+    if (cfl) *cfl = c;
+    free(qstar); free(L); free(smax); free(F); free(G);
+    return rc;
+}
+
+/* ====================================================================== */
+/* Lax-Wendroff: time-integrated fluxes                                     */
+/* ====================================================================== */
+
+/* Diff1 / Diff2, lib/WenoReconstruct.cpp:702-716, 791-806 */
+static double diff1(double dx, const double* f)
+{
+    double tmp = (f[0] - f[4]) * (1.0 / 12.0);
+    tmp += (-f[1] + f[3]) * (2.0 / 3.0);
+    return tmp / dx;
+}
+static double diff2(double dx, const double* f)
+{
+    double tmp = (-f[0] - f[4]) * (1.0 / 12.0);
+    tmp += (f[1] + f[3]) * (4.0 / 3.0);
+    tmp += (-f[2]) * (5.0 / 2.0);
+    return tmp / (dx * dx);
+}
+
+/* ConstructIntegratedR (one-stage overload), lib/2d/ConstructIntegratedR.cpp:
+ * 59-296: F = f + dt/2 (f_t + dt/3 f_tt) on cells [1-3, m+3]^2. */
+int orc_construct_integrated_r(const finess_params* p, double dt, const double* q, const double* aux,
+                               double* F, double* G)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    const double dx = p->dx, dy = p->dy;
+    if (mbc < 5 || p->ndims != 2) return FINESS_ERR_CONFIG;
+    /* R = SampleFunction(FluxFunc) on all cells incl. ghosts, :83-84 */
+    const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    double* R = (double*)malloc(ncell * meqn * 2 * sizeof(double));
+    if (!R) return FINESS_ERR_ARG;
+#define RIX(i, j, m, d) ((ix3((i), (j), 1, my, 1, mbc)) * (size_t)(meqn * 2) + (size_t)((m)-1) * 2 + (size_t)((d)-1))
+#pragma omp parallel for schedule(static)
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            double f[MAXEQ], g[MAXEQ];
+            orc_flux(p, &q[ix3(i, j, 1, my, meqn, mbc)], &aux[ix3(i, j, 1, my, na, mbc)], f, g);
+            for (int m = 1; m <= meqn; m++) {
+                R[RIX(i, j, m, 1)] = f[m - 1];
+                R[RIX(i, j, m, 2)] = g[m - 1];
+            }
+        }
+    const int mbc_small = 3, half = 3, npts = 5;
+#pragma omp parallel for schedule(static)
+    for (int i = 1 - mbc_small; i <= mx + mbc_small; i++)
+        for (int j = 1 - mbc_small; j <= my + mbc_small; j++) {
+            double Fv[MAXEQ][5], Gv[MAXEQ][5], qx_[MAXEQ][5], qy_[MAXEQ][5];
+            for (int m = 1; m <= meqn; m++) {
+                int s = -half + 1;
+                for (int r = 1; r <= npts; r++) {
+                    Fv[m - 1][r - 1] = R[RIX(i + s, j, m, 1)];
+                    Gv[m - 1][r - 1] = R[RIX(i, j + s, m, 2)];
+                    qx_[m - 1][r - 1] = q[ix3(i + s, j, m, my, meqn, mbc)];
+                    qy_[m - 1][r - 1] = q[ix3(i, j + s, m, my, meqn, mbc)];
+                    s++;
+                }
+            }
+            double fx[MAXEQ], qx[MAXEQ], gy[MAXEQ], qy[MAXEQ];
+            for (int m = 0; m < meqn; m++) {
+                fx[m] = diff1(dx, Fv[m]);
+                qx[m] = diff1(dx, qx_[m]);
+                gy[m] = diff1(dy, Gv[m]);
+                qy[m] = diff1(dy, qy_[m]);
+            }
+            double A[2 * MAXEQ * MAXEQ];
+            const double* qc = &q[ix3(i, j, 1, my, meqn, mbc)];
+            orc_dflux(p, qc, A);
+#define AA(m1, m2, d) A[(((d)-1) * meqn + (m1)-1) * meqn + (m2)-1]
+            double f_t[MAXEQ], g_t[MAXEQ];
+            for (int m1 = 1; m1 <= meqn; m1++) {
+                double tmp1 = 0., tmp2 = 0.;
+                for (int m2 = 1; m2 <= meqn; m2++) {
+                    tmp1 += -(AA(m1, m2, 1)) * (fx[m2 - 1] + gy[m2 - 1]);
+                    tmp2 += -(AA(m1, m2, 2)) * (fx[m2 - 1] + gy[m2 - 1]);
+                }
+                f_t[m1 - 1] = tmp1;
+                g_t[m1 - 1] = tmp2;
+            }
+            double f_tt[MAXEQ] = {0}, g_tt[MAXEQ] = {0};
+            if (p->time_order > 2) {
+                double Hs[2 * MAXEQ * MAXEQ * MAXEQ];
+                orc_d2flux(p, qc, Hs);
+#define HH(m, m1, m2, d) Hs[((((d)-1) * meqn + (m)-1) * meqn + (m1)-1) * meqn + (m2)-1]
+                double S[MAXEQ], St[MAXEQ] = {0};
+                for (int m = 0; m < meqn; m++) S[m] = fx[m] + gy[m];
+                double fxx[MAXEQ], gyy[MAXEQ], fxy[MAXEQ], gxy[MAXEQ];
+                for (int m = 0; m < meqn; m++) {
+                    fxx[m] = diff2(dx, Fv[m]);
+                    gyy[m] = diff2(dy, Gv[m]);
+                }
+                for (int m = 1; m <= meqn; m++) { /* :202-213 */
+                    double tmp1 = 0.5 * (R[RIX(i + 1, j + 1, m, 1)] - R[RIX(i - 1, j + 1, m, 1)]) / dx;
+                    tmp1 -= 0.5 * (R[RIX(i + 1, j - 1, m, 1)] - R[RIX(i - 1, j - 1, m, 1)]) / dx;
+                    tmp1 *= 0.5 / dy;
+                    fxy[m - 1] = tmp1;
+                    double tmp2 = 0.5 * (R[RIX(i + 1, j + 1, m, 2)] - R[RIX(i - 1, j + 1, m, 2)]) / dx;
+                    tmp2 -= 0.5 * (R[RIX(i + 1, j - 1, m, 2)] - R[RIX(i - 1, j - 1, m, 2)]) / dx;
+                    tmp2 *= 0.5 / dy;
+                    gxy[m - 1] = tmp2;
+                }
+                for (int m = 1; m <= meqn; m++) { /* :215-240 */
+                    double tmp = 0.;
+                    for (int m1 = 1; m1 <= meqn; m1++)
+                        for (int m2 = 1; m2 <= meqn; m2++) {
+                            tmp += HH(m, m1, m2, 1) * qx[m1 - 1] * S[m2 - 1];
+                            tmp += HH(m, m1, m2, 2) * qy[m1 - 1] * S[m2 - 1];
+                        }
+                    for (int m1 = 1; m1 <= meqn; m1++) {
+                        tmp += AA(m, m1, 1) * (fxx[m1 - 1] + gxy[m1 - 1]);
+                        tmp += AA(m, m1, 2) * (fxy[m1 - 1] + gyy[m1 - 1]);
+                    }
+                    St[m - 1] = tmp;
+                }
+                for (int m1 = 1; m1 <= meqn; m1++) { /* :243-256 */
+                    double tmp1 = 0., tmp2 = 0.;
+                    for (int m2 = 1; m2 <= meqn; m2++) {
+                        tmp1 += AA(m1, m2, 1) * St[m2 - 1];
+                        tmp2 += AA(m1, m2, 2) * St[m2 - 1];
+                    }
+                    f_tt[m1 - 1] = tmp1;
+                    g_tt[m1 - 1] = tmp2;
+                }
+                for (int m = 1; m <= meqn; m++) { /* :259-272 */
+                    double tmp1 = 0., tmp2 = 0.;
+                    for (int m1 = 1; m1 <= meqn; m1++)
+                        for (int m2 = 1; m2 <= meqn; m2++) {
+                            tmp1 += HH(m, m1, m2, 1) * S[m1 - 1] * S[m2 - 1];
+                            tmp2 += HH(m, m1, m2, 2) * S[m1 - 1] * S[m2 - 1];
+                        }
+                    f_tt[m - 1] = f_tt[m - 1] + tmp1;
+                    g_tt[m - 1] = g_tt[m - 1] + tmp2;
+                }
+#undef HH
+            }
+#undef AA
+            for (int m = 1; m <= meqn; m++) { /* :287-291 */
+                F[ix3(i, j, m, my, meqn, mbc)] = R[RIX(i, j, m, 1)] + 0.5 * dt * (f_t[m - 1] + dt / 3.0 * f_tt[m - 1]);
+                G[ix3(i, j, m, my, meqn, mbc)] = R[RIX(i, j, m, 2)] + 0.5 * dt * (g_t[m - 1] + dt / 3.0 * g_tt[m - 1]);
+            }
+        }
+#undef RIX
+    free(R);
+    return FINESS_OK;
+}
+
+int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux, const double* F,
+                        const double* G, double* Lstar, double* smax)
+{
+    const int rc = check_config(p);
+    if (rc) return rc;
+    return construct_l_2d(p, q, aux, F, G, Lstar, smax);
+}

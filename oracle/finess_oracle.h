@@ -1,0 +1,77 @@
+/*
+ * finess_oracle.h -- CPU restatement of FINESS's WENO operator and time
+ * steppers.  TEST INFRASTRUCTURE ONLY: it is the checker that the HIP path is
+ * compared against (tests/, __graft_entry__.smoke) and the CPU baseline timed
+ * beside the GPU (bench.py).  No product code links or calls it.
+ *
+ * Arrays use the reference layout (lib/tensors.h:267-273): 2D fields are
+ * dTensorBC3(mx,my,meqn,mbc), 1D fields dTensorBC2(mx,meqn,mbc).
+ * Return codes are those of include/finess_hip.h.
+ */
+#ifndef FINESS_ORACLE_H
+#define FINESS_ORACLE_H
+
+#include "../include/finess_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* WENO reconstructions (lib/WenoReconstruct.cpp:90-141 JS5, :385-438 Z5,
+ * :580-616 FD5): g[0..4] = stencil, returns the value at the right edge. */
+double orc_weno5(int version, double eps, double power, const double* g);
+/* Nonlinear weights of JS5 (omega_k = omt_k/omts, lib/WenoReconstruct.cpp:131-134). */
+void   orc_weno5_js_weights(double eps, double power, const double* g, double* w);
+
+/* Physics callbacks of the configured app. Q, F, G are meqn-vectors. */
+void orc_flux(const finess_params* p, const double* Q, const double* aux, double* F, double* G);
+void orc_project_left (const finess_params* p, int ixy, const double* Qavg, int npts,
+                       const double* Qvals /* [meqn][npts] */, double* Wvals);
+void orc_project_right(const finess_params* p, int ixy, const double* Qavg, int npts,
+                       const double* Wvals /* [meqn][npts] */, double* Qvals);
+void orc_wave_speed(const finess_params* p, int ixy, const double* Ql, const double* Qr,
+                    const double* auxl, const double* auxr, double* s1, double* s2);
+/* Jacobians A[d][m1][m2] = d f_d,m1 / d q_m2 and Hessians H[d][m][m1][m2]. */
+void orc_dflux (const finess_params* p, const double* Q, double* A /* [2][meqn][meqn] */);
+void orc_d2flux(const finess_params* p, const double* Q, double* H /* [2][meqn][meqn][meqn] */);
+
+/* SetBndValues for the configured per-side boundary conditions. */
+void orc_set_bnd(const finess_params* p, double* q, double* aux);
+
+/* ConstructL (2D or 1D by p->ndims). */
+int orc_construct_l(const finess_params* p, const double* q, const double* aux,
+                    double* Lstar, double* smax);
+
+/* GetCFL (lib/2d/GetCFL.cpp:31-74, lib/1d/GetCFL.cpp:9-45). */
+int orc_get_cfl(const finess_params* p, double dt, double dtmax, const double* smax, double* cfl);
+
+typedef struct orc_stats {
+    long long steps_accepted, steps_rejected;
+    double dt_min, dt_max, last_cfl;
+} orc_stats;
+
+/* FinSolveRK / FinSolveLxW: q in/out (reference layout), *t in/out. */
+int orc_fin_solve_rk (const finess_params* p, double* q, double* aux, double* t,
+                      double tend, double* dtv, orc_stats* st);
+int orc_fin_solve_lxw(const finess_params* p, double* q, double* aux, double* t,
+                      double tend, double* dtv, orc_stats* st);
+/* Fixed number of steps of size dt, no controller (for timing and parity at
+ * a stated step count).  Returns the CFL of the last step in *cfl. */
+int orc_run_fixed_steps(const finess_params* p, double* q, double* aux, double dt,
+                        int nsteps, double* cfl);
+
+/* ConstructIntegratedR + ConstructLxWL (lib/2d/ConstructIntegratedR.cpp:59-296,
+ * lib/2d/ConstructLxWL.cpp:30-414). */
+int orc_construct_integrated_r(const finess_params* p, double dt, const double* q,
+                               const double* aux, double* F, double* G);
+int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux,
+                        const double* F, const double* G, double* Lstar, double* smax);
+
+/* Number of OpenMP threads the oracle runs with. */
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

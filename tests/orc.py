@@ -1,0 +1,170 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and to the CPU
+emulation of the kernel phase functions (tests/emu/libfiness_emu.so).
+Test infrastructure only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from finess_amd import Params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+EMU = os.path.join(ROOT, "tests", "emu", "libfiness_emu.so")
+PD = ctypes.POINTER(ctypes.c_double)
+PP = ctypes.POINTER(Params)
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [("steps_accepted", ctypes.c_longlong), ("steps_rejected", ctypes.c_longlong),
+                ("dt_min", ctypes.c_double), ("dt_max", ctypes.c_double), ("last_cfl", ctypes.c_double)]
+
+
+def _build(path, mkdir):
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", mkdir], check=True)
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(PD)
+
+
+_oracle = None
+_emu = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _build(ORACLE, os.path.join(ROOT, "oracle"))
+        lib = ctypes.CDLL(ORACLE)
+        sig = {
+            "orc_weno5": (ctypes.c_double, [ctypes.c_int, ctypes.c_double, ctypes.c_double, PD]),
+            "orc_weno5_js_weights": (None, [ctypes.c_double, ctypes.c_double, PD, PD]),
+            "orc_flux": (None, [PP, PD, PD, PD, PD]),
+            "orc_project_left": (None, [PP, ctypes.c_int, PD, ctypes.c_int, PD, PD]),
+            "orc_project_right": (None, [PP, ctypes.c_int, PD, ctypes.c_int, PD, PD]),
+            "orc_wave_speed": (None, [PP, ctypes.c_int, PD, PD, PD, PD, PD, PD]),
+            "orc_dflux": (None, [PP, PD, PD]),
+            "orc_d2flux": (None, [PP, PD, PD]),
+            "orc_set_bnd": (None, [PP, PD, PD]),
+            "orc_construct_l": (ctypes.c_int, [PP, PD, PD, PD, PD]),
+            "orc_get_cfl": (ctypes.c_int, [PP, ctypes.c_double, ctypes.c_double, PD, PD]),
+            "orc_fin_solve_rk": (ctypes.c_int, [PP, PD, PD, PD, ctypes.c_double, PD, ctypes.POINTER(OrcStats)]),
+            "orc_fin_solve_lxw": (ctypes.c_int, [PP, PD, PD, PD, ctypes.c_double, PD, ctypes.POINTER(OrcStats)]),
+            "orc_run_fixed_steps": (ctypes.c_int, [PP, PD, PD, ctypes.c_double, ctypes.c_int, PD]),
+            "orc_construct_integrated_r": (ctypes.c_int, [PP, ctypes.c_double, PD, PD, PD, PD]),
+            "orc_construct_lxw_l": (ctypes.c_int, [PP, PD, PD, PD, PD, PD, PD]),
+            "orc_num_threads": (ctypes.c_int, []),
+            "orc_set_num_threads": (None, [ctypes.c_int]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(lib, n)
+            f.restype, f.argtypes = r, a
+        _oracle = lib
+    return _oracle
+
+
+def emu():
+    global _emu
+    if _emu is None:
+        _build(EMU, os.path.join(ROOT, "tests", "emu"))
+        lib = ctypes.CDLL(EMU)
+        lib.emu_construct_l.restype = ctypes.c_int
+        lib.emu_construct_l.argtypes = [PP, PD, PD, PD]
+        lib.emu_construct_integrated_r.restype = ctypes.c_int
+        lib.emu_construct_integrated_r.argtypes = [PP, ctypes.c_double, PD, PD, PD]
+        lib.emu_run_fixed_steps.restype = ctypes.c_int
+        lib.emu_run_fixed_steps.argtypes = [PP, PD, ctypes.c_double, ctypes.c_int]
+        _emu = lib
+    return _emu
+
+
+# ---- convenience wrappers ---------------------------------------------------
+def construct_l(p, q, aux=None):
+    lib = oracle()
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    L = np.zeros(p.shape_q())
+    sm = np.zeros(p.shape_smax())
+    rc = lib.orc_construct_l(ctypes.byref(p), ptr(q), ptr(aux), ptr(L), ptr(sm))
+    assert rc == 0, rc
+    return L, sm
+
+
+def set_bnd(p, q, aux=None):
+    aux = np.ones(p.shape_aux()) if aux is None else aux
+    oracle().orc_set_bnd(ctypes.byref(p), ptr(q), ptr(aux))
+    return q
+
+
+def fin_solve(p, q, t, tend, dtv, aux=None, lxw=False):
+    lib = oracle()
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    tt = ctypes.c_double(t)
+    st = OrcStats()
+    fn = lib.orc_fin_solve_lxw if lxw else lib.orc_fin_solve_rk
+    rc = fn(ctypes.byref(p), ptr(q), ptr(aux), ctypes.byref(tt), float(tend), ptr(dtv), ctypes.byref(st))
+    return rc, tt.value, st
+
+
+def fixed_steps(p, q, dt, n, aux=None):
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    cfl = ctypes.c_double()
+    rc = oracle().orc_run_fixed_steps(ctypes.byref(p), ptr(q), ptr(aux), float(dt), int(n), ctypes.byref(cfl))
+    assert rc == 0, rc
+    return cfl.value
+
+
+def integrated_r(p, dt, q, aux=None):
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    F = np.zeros(p.shape_q())
+    G = np.zeros(p.shape_q())
+    rc = oracle().orc_construct_integrated_r(ctypes.byref(p), float(dt), ptr(q), ptr(aux), ptr(F), ptr(G))
+    assert rc == 0, rc
+    return F, G
+
+
+def emu_construct_l(p, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    L = np.zeros(p.shape_q())
+    sm = np.zeros(p.shape_smax())
+    assert emu().emu_construct_l(ctypes.byref(p), ptr(q), ptr(L), ptr(sm)) == 0
+    return L, sm
+
+
+def emu_integrated_r(p, dt, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    F = np.zeros(p.shape_q())
+    G = np.zeros(p.shape_q())
+    assert emu().emu_construct_integrated_r(ctypes.byref(p), float(dt), ptr(q), ptr(F), ptr(G)) == 0
+    return F, G
+
+
+def interior(p, a):
+    g = p.mbc
+    if p.ndims == 1:
+        return a[g:g + p.mx]
+    return a[g:g + p.mx, g:g + p.my]
+
+
+def rel_linf(a, b):
+    """Relative L-infinity per field (last axis): ||a-b|| / max(||b||, tiny);
+    absolute error for an identically-zero field."""
+    a = a.reshape(-1, a.shape[-1])
+    b = b.reshape(-1, b.shape[-1])
+    out = []
+    for m in range(a.shape[1]):
+        nb = np.max(np.abs(b[:, m]))
+        d = np.max(np.abs(a[:, m] - b[:, m]))
+        out.append(d / nb if nb > 0 else d)
+    return np.array(out)
+
+
+def emu_fixed_steps(p, q, dt, n):
+    assert emu().emu_run_fixed_steps(ctypes.byref(p), ptr(q), float(dt), int(n)) == 0
+    return q

@@ -1,0 +1,223 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
+same seeded inputs.  Tolerances: operator-level 1e-12 relative L-inf per
+field; after a stated number of steps 1e-10 relative L-inf per field
+(absolute for the identically-zero rho*w field), the bar of the plain FP64
+form.  FP32 stencil arithmetic is checked at 1e-5 per operator call."""
+import numpy as np
+import pytest
+
+import orc
+from finess_amd import (BC_EXTRAP, BC_PERIODIC, BC_REFLECT, PREC_FP32, WENO_FD, WENO_Z, FinessError, Solver,
+                        problems)
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(12345)
+
+
+def _perturb(q, scale=0.02):
+    q = q.copy()
+    q[..., 0] *= 1.0 + scale * RNG.random(q.shape[:-1])
+    return q
+
+
+def _construct_l_case(p, q, tol=1e-12):
+    orc.set_bnd(p, q)
+    L0, s0 = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("construct_l rel err", err)
+    assert np.all(err <= tol), err
+    assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
+
+
+@pytest.mark.parametrize("shape", [(40, 40), (37, 70), (130, 66)])
+def test_construct_l_euler(shape):
+    p, q, _ = problems.riemann4(mx=shape[0], my=shape[1])
+    _construct_l_case(p, _perturb(q))
+
+
+@pytest.mark.parametrize("weno", [WENO_Z, WENO_FD])
+def test_construct_l_euler_weno_variants(weno):
+    p, q, _ = problems.riemann4(mx=48, my=40, weno_version=weno)
+    _construct_l_case(p, _perturb(q))
+
+
+def test_construct_l_euler_power_param():
+    p, q, _ = problems.riemann4(mx=48, my=40, power_param=1.5)
+    _construct_l_case(p, _perturb(q))
+
+
+def test_construct_l_shallow_water():
+    p, q, _ = problems.sw_bump(nx=45, ny=29)
+    q[..., 1] = 0.1 * RNG.random(q.shape[:-1])
+    q[..., 2] = 0.1 * RNG.random(q.shape[:-1])
+    _construct_l_case(p, q)
+
+
+def test_construct_l_advection_1d():
+    p, q, aux = problems.advection_1d(mx=200)
+    _construct_l_case(p, q)
+
+
+def test_construct_l_fp32_stencil():
+    p, q, _ = problems.riemann4(mx=64, my=64, precision=PREC_FP32)
+    orc.set_bnd(p, q)
+    L0, _ = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, _ = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("fp32 construct_l rel err", err)
+    assert np.all(err <= 1e-5), err
+
+
+def test_construct_integrated_r():
+    p, q, _ = problems.kelvin_helmholtz(mx=48, my=40)
+    q = _perturb(q, 0.05)
+    orc.set_bnd(p, q)
+    dt = 0.004
+    F0, G0 = orc.integrated_r(p, dt, q)
+    with Solver(p) as s:
+        F1, G1 = s.construct_integrated_r(dt, q)
+    g = p.mbc
+    sl = (slice(g - 3, g + p.mx + 3), slice(g - 3, g + p.my + 3))
+    eF = orc.rel_linf(F1[sl], F0[sl])
+    eG = orc.rel_linf(G1[sl], G0[sl])
+    print("integrated R rel err", eF, eG)
+    assert np.all(eF <= 1e-12) and np.all(eG <= 1e-12)
+
+
+def _fixed_steps_case(p, q, dt, n, tol=1e-10):
+    q0 = q.copy()
+    orc.fixed_steps(p, q0, dt, n)
+    with Solver(p) as s:
+        s.set_state(q)
+        s.run_fixed_steps(dt, n)
+        q1, _ = s.get_state()
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("after %d steps rel err" % n, err)
+    assert np.all(err <= tol), err
+
+
+def test_steps_riemann4_128():
+    p, q, _ = problems.riemann4(mx=128)
+    _fixed_steps_case(p, q, 1.5e-3, 20)
+
+
+def test_steps_mixed_boundaries():
+    p, q, _ = problems.riemann4(mx=64, my=96, bc=[BC_EXTRAP, BC_PERIODIC, BC_REFLECT, BC_REFLECT])
+    _fixed_steps_case(p, q, 2e-3, 10)
+
+
+def test_steps_shock_bubble_small():
+    p, q, _ = problems.shock_bubble(mx=256, my=64)
+    _fixed_steps_case(p, q, 1e-3, 10)
+
+
+def test_steps_shallow_water():
+    p, q, _ = problems.sw_bump(nx=96, ny=64)
+    _fixed_steps_case(p, q, 2e-3, 20)
+
+
+def test_steps_lxw_kelvin_helmholtz():
+    p, q, _ = problems.kelvin_helmholtz(mx=96, my=64)
+    _fixed_steps_case(p, q, 1e-3, 10)
+
+
+def test_fin_solve_rk_controller_riemann4():
+    """FinSolveRK with the a-posteriori controller from t=0 to 0.02 at 256^2:
+    same accepted-step count, same dt, state within 1e-10."""
+    p, q, _ = problems.riemann4(mx=256)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.02, dtv0)
+    assert rc == 0
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    q1 = q.copy()
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, 0.02, dtv1)
+    assert abs(t1 - t0) <= 1e-14
+    assert abs(dtv1[1] - dtv0[1]) <= 1e-12 * dtv0[1]
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("controller run: steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err
+
+
+def test_fin_solve_rk_reject_path():
+    """initial_dt far above the stable step: the first attempt is rejected
+    (lib/2d/FinSolveRK.cpp:343-357) and the retry uses dt*desired/cfl."""
+    p, q, _ = problems.riemann4(mx=64, initial_dt=0.01)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, 0.01, 1.0])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.01, dtv0)
+    assert rc == 0 and st0.steps_rejected >= 1
+    q1 = q.copy()
+    dtv1 = np.array([0.0, 0.01, 1.0])
+    with Solver(p) as s:
+        s.set_state(q1)
+        st1 = s.advance(0.01, dtv1)
+        q1, t1 = s.get_state()
+    assert (st1.steps_accepted, st1.steps_rejected) == (st0.steps_accepted, st0.steps_rejected)
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    assert np.all(err <= 1e-10), err
+
+
+def test_fin_solve_rk_advection_full_run():
+    """Workload 1: mx=200, t_final=1, the full run (500 steps at CFL 0.4)."""
+    p, q, aux = problems.advection_1d(mx=200)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 1.0, dtv0, aux=aux)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, 1.0, dtv1, aux=aux)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    exact = np.sin(2 * np.pi * (p.xlow + (np.arange(p.mx) + 0.5) * p.dx - 1.0))
+    l1 = np.mean(np.abs(orc.interior(p, q1)[:, 0] - exact))
+    print("advection: steps", st0.steps_accepted, "rel err vs oracle", err, "L1 vs exact", l1)
+    assert np.all(err <= 1e-10)
+    assert l1 < 1e-4
+
+
+def test_nv_limit_raises():
+    p, q, _ = problems.riemann4(mx=32, nv=2)
+    dtv = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        with pytest.raises(FinessError) as ei:
+            s.fin_solve_rk(q, 0.0, 1.0, dtv)
+    assert ei.value.code == -3
+
+
+def test_symmetry_full_size():
+    """Workload 2 at its full 1024^2 shape: the data is symmetric under
+    (x, y, u, v) -> (y, x, v, u); after 10 steps the GPU state keeps it."""
+    p, q, _ = problems.riemann4(mx=1024)
+    with Solver(p) as s:
+        s.set_state(q)
+        dtv = np.array([0.0, p.initial_dt, p.max_dt])
+        st = s.advance_steps(10, dtv)
+        q1, _ = s.get_state()
+    qi = orc.interior(p, q1)
+    qt = np.transpose(qi, (1, 0, 2))[..., [0, 2, 1, 3, 4]]
+    err = orc.rel_linf(qi, qt)
+    print("symmetry defect", err, "steps", st.steps_accepted)
+    assert np.all(err <= 1e-12), err
+
+
+def test_conservation_periodic_full_size():
+    """Workload 3's shape (2048^2, periodic): sums of conserved fields stay
+    constant to round-off (the flux differences telescope)."""
+    p, q, _ = problems.kelvin_helmholtz(mx=2048)
+    qi0 = orc.interior(p, q).reshape(-1, 5).sum(axis=0)
+    with Solver(p) as s:
+        s.set_state(q)
+        s.run_fixed_steps(2e-4, 5)
+        q1, _ = s.get_state()
+    qi1 = orc.interior(p, q1).reshape(-1, 5).sum(axis=0)
+    scale = np.abs(orc.interior(p, q)).reshape(-1, 5).sum(axis=0) + 1.0
+    print("conservation defect", (qi1 - qi0) / scale)
+    assert np.all(np.abs(qi1 - qi0) / scale <= 1e-12)

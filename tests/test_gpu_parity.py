@@ -180,7 +180,7 @@ def test_fin_solve_rk_advection_full_run():
     l1 = np.mean(np.abs(orc.interior(p, q1)[:, 0] - exact))
     print("advection: steps", st0.steps_accepted, "rel err vs oracle", err, "L1 vs exact", l1)
     assert np.all(err <= 1e-10)
-    assert l1 < 1e-4
+    assert l1 < 2e-6
 
 
 def test_nv_limit_raises():

@@ -37,19 +37,24 @@ struct PhysParams {
 // ------------------------------------------------------------------------
 struct Euler2D {
     static constexpr int MEQN = 5;
-    static constexpr int NV = 9;
+    static constexpr int NV = 10;
     static constexpr int NORMAL_X = 1;  // momentum component normal to x-walls
     static constexpr int NORMAL_Y = 2;
 
-    template <typename R>
-    __host__ __device__ __forceinline__ static void cell_record(const R q[5], const PhysParams& pp, R v[9])
+    // cell record: q0..q4, u, v, w, p, c  with c = sqrt(|gamma p / rho|),
+    // the cell sound speed SetWaveSpd uses for the left and right states
+    // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Always formed in FP64 (the
+    // pressure recovery keeps FP64 in the FP32 form as well).
+    __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[10])
     {
-        const R gamma = R(pp.gamma);
-        const R rho = q[0];
-        const R u1 = q[1] / rho, u2 = q[2] / rho, u3 = q[3] / rho;
-        const R press = (gamma - R(1)) * (q[4] - R(0.5) * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+        const double gamma = pp.gamma;
+        const double rho = q[0];
+        const double irho = 1.0 / rho;
+        const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
+        const double press = (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
         v[5] = u1; v[6] = u2; v[7] = u3; v[8] = press;
+        v[9] = sqrt(fabs(gamma * press * irho));
     }
 
     // LDS variable index of rotated component k (dir 0: x, 1: y)
@@ -58,9 +63,9 @@ struct Euler2D {
         return (dir && (k == 1 || k == 2)) ? 3 - k : k;
     }
 
-    // Rotated state and flux of one cell record.  c[] is the record.
+    // Rotated state and flux of one cell record (FluxFunc.cpp:20-40).
     template <typename R>
-    __host__ __device__ __forceinline__ static void state_flux(const R c[9], int dir, R q[5], R f[5])
+    __host__ __device__ __forceinline__ static void state_flux(const R c[10], int dir, R q[5], R f[5])
     {
         const R rho = c[0], E = c[4], p = c[8];
         const R mn = dir ? c[2] : c[1], mt = dir ? c[1] : c[2];
@@ -74,13 +79,19 @@ struct Euler2D {
         f[4] = un * (E + p);
     }
 
+    // Eigen-system at the arithmetic-mean state (lib/2d/ConstructL.cpp:72-77)
+    // with the left-eigenvector rows of ProjectLeftEig.cpp:49-65 pre-scaled by
+    // their denominators:
+    //   w1 = (a1 q_mu + b1 q_1 + cc s),  w5 = (a5 q_mu + b5 q_1 + cc s),
+    //   s  = q_5 - u2 q_mv - u3 q_mw,
+    //   w_mu = k2 q_1 + e1 q_mu + e2 q_mv + e3 q_mw + m5 q_5,
+    //   w_mv = q_mv - u2 q_1,  w_mw = q_mw - u3 q_1.
     template <typename R>
     struct Eig {
-        R u1, u2, u3, umag2, H, c, cR;
-        R k1a, k1b, k5a, k5b, k2, inv15, inv2;
+        R u1, u2, u3, hu, H, cR;
+        R a1, b1, a5, b5, cc, k2, e1, e2, e3, m5;
     };
 
-    // Eigen-system at the arithmetic-mean state (lib/2d/ConstructL.cpp:72-77)
     template <typename R>
     __host__ __device__ __forceinline__ static void eig(const R qa[5], const PhysParams& pp, Eig<R>& e)
     {
@@ -89,32 +100,36 @@ struct Euler2D {
         const R irho = R(1) / rho;
         e.u1 = qa[1] * irho; e.u2 = qa[2] * irho; e.u3 = qa[3] * irho;
         const R energy = qa[4];
-        e.umag2 = e.u1 * e.u1 + e.u2 * e.u2 + e.u3 * e.u3;
-        const R press = (gamma - R(1)) * (energy - R(0.5) * rho * e.umag2);
+        const R umag2 = e.u1 * e.u1 + e.u2 * e.u2 + e.u3 * e.u3;
+        const R press = (gamma - R(1)) * (energy - R(0.5) * rho * umag2);
         const R c2 = gamma * press * irho;
-        e.c = sqrt(c2);          // ProjectLeftEig.cpp:43
-        e.cR = sqrt(fabs(c2));   // ProjectRightEig.cpp:42
+        e.cR = sqrt(fabs(c2));                                   // ProjectRightEig.cpp:42
+        const R c = c2 >= R(0) ? e.cR : R(__builtin_nan(""));   // ProjectLeftEig.cpp:43
         e.H = (energy + press) * irho;
-        const R hu = e.umag2 * R(0.5);
-        e.k1a = hu - e.H - e.u1 * e.c;
-        e.k1b = hu * (e.c - e.u1) + e.H * e.u1;
-        e.k5a = e.H - hu - e.u1 * e.c;
-        e.k5b = hu * (e.c + e.u1) - e.H * e.u1;
-        e.k2 = e.H - e.umag2;
-        const R den = R(2) * e.H - e.umag2;
-        e.inv2 = R(2) / den;
-        e.inv15 = R(1) / (e.c * den);
+        e.hu = umag2 * R(0.5);
+        const R den = R(2) * e.H - umag2;
+        const R r = R(1) / (c * den);
+        const R inv2 = R(2) * c * r;                             // 2 / den
+        e.a1 = (e.hu - e.H - e.u1 * c) * r;
+        e.b1 = (e.hu * (c - e.u1) + e.H * e.u1) * r;
+        e.a5 = (e.H - e.hu - e.u1 * c) * r;
+        e.b5 = (e.hu * (c + e.u1) - e.H * e.u1) * r;
+        e.cc = c * r;
+        e.k2 = (e.H - umag2) * inv2;
+        e.e1 = e.u1 * inv2; e.e2 = e.u2 * inv2; e.e3 = e.u3 * inv2;
+        e.m5 = -inv2;
     }
 
     template <typename R>
     __host__ __device__ __forceinline__ static void left(const Eig<R>& e, const R q[5], R w[5])
     {
-        const R t = e.c * (q[4] - e.u2 * q[2] - e.u3 * q[3]);
-        w[0] = (e.k1a * q[1] + e.k1b * q[0] + t) * e.inv15;
-        w[1] = ((e.k2 * q[0] + e.u1 * q[1] + e.u2 * q[2] + e.u3 * q[3]) - q[4]) * e.inv2;
+        const R s = q[4] - e.u2 * q[2] - e.u3 * q[3];
+        const R t = e.cc * s;
+        w[0] = e.a1 * q[1] + e.b1 * q[0] + t;
+        w[1] = e.k2 * q[0] + e.e1 * q[1] + e.e2 * q[2] + e.e3 * q[3] + e.m5 * q[4];
         w[2] = q[2] - e.u2 * q[0];
         w[3] = q[3] - e.u3 * q[0];
-        w[4] = (e.k5a * q[1] + e.k5b * q[0] + t) * e.inv15;
+        w[4] = e.a5 * q[1] + e.b5 * q[0] + t;
     }
 
     template <typename R>
@@ -126,25 +141,22 @@ struct Euler2D {
         q[1] = (e.u1 - e.cR) * w[0] + e.u1 * w[1] + (e.u1 + e.cR) * w[4];
         q[2] = e.u2 * s + w[2];
         q[3] = e.u3 * s + w[3];
-        q[4] = (e.H - uc) * w[0] + (e.umag2 * R(0.5)) * w[1] + e.u2 * w[2] + e.u3 * w[3] + (e.H + uc) * w[4];
+        q[4] = (e.H - uc) * w[0] + e.hu * w[1] + e.u2 * w[2] + e.u3 * w[3] + (e.H + uc) * w[4];
     }
 
     // SetWaveSpd with nvec = the face normal; cl, cr are the cell records of
     // the left and right cells.  Returns max(|s1|,|s2|).
     template <typename R>
-    __host__ __device__ __forceinline__ static R wave_speed(const R cl[9], const R cr[9], int dir, const PhysParams& pp)
+    __host__ __device__ __forceinline__ static R wave_speed(const R cl[10], const R cr[10], int dir, const PhysParams& pp)
     {
         const R gamma = R(pp.gamma);
-        const R rhol = cl[0], rhor = cr[0], pl = cl[8], pr = cr[8];
         const R unl = dir ? cl[6] : cl[5], unr = dir ? cr[6] : cr[5];
-        const R rho = R(0.5) * (rhol + rhor);
+        const R rho = R(0.5) * (cl[0] + cr[0]);
         const R un = R(0.5) * (unl + unr);
-        const R press = R(0.5) * (pl + pr);
-        const R c_l = sqrt(fabs(gamma * pl / rhol));
-        const R c_r = sqrt(fabs(gamma * pr / rhor));
+        const R press = R(0.5) * (cl[8] + cr[8]);
         const R c = sqrt(fabs(gamma * press / rho));
-        const R s1 = rmin(unl - c_l, un - c);
-        const R s2 = rmax(unr + c_r, un + c);
+        const R s1 = rmin(unl - cl[9], un - c);
+        const R s2 = rmax(unr + cr[9], un + c);
         return rmax(fabs(s1), fabs(s2));
     }
 };
@@ -159,12 +171,11 @@ struct ShallowWater2D {
     static constexpr int NORMAL_X = 1;
     static constexpr int NORMAL_Y = 2;
 
-    template <typename R>
-    __host__ __device__ __forceinline__ static void cell_record(const R q[3], const PhysParams&, R v[5])
+    __host__ __device__ __forceinline__ static void cell_record(const double q[3], const PhysParams&, double v[5])
     {
-        const R h = q[0];
+        const double ih = 1.0 / q[0];
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
-        v[3] = q[1] / h; v[4] = q[2] / h;
+        v[3] = q[1] * ih; v[4] = q[2] * ih;
     }
 
     __host__ __device__ __forceinline__ static int rot_q(int k, int dir)

@@ -153,22 +153,32 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     for (int k = 0; k < M; k++) qa[k] = R(0.5) * (qs[3][k] + qs[2][k]);
     typename P::template Eig<R> e;
     P::eig(qa, pp, e);
-    R w[6][M], g[6][M];
-    for (int s = 0; s < 6; s++) {
-        P::left(e, qs[s], w[s]);
-        P::left(e, fs[s], g[s]);
-    }
     const R alpha = P::wave_speed(cl, cr, dir, pp);
     const R la = ascale * alpha;
-    R gh[M];
-    for (int k = 0; k < M; k++) {
-        R gp[5], gm[5];
-        for (int s = 0; s < 5; s++) {
-            gp[s] = R(0.5) * (g[s][k] + la * w[s][k]);
-            gm[s] = R(0.5) * (g[5 - s][k] - la * w[5 - s][k]);
+    // Lax-Friedrichs split projected in one go: the reference forms
+    // gp = (L f + la L q)/2 and gm = (L f - la L q)/2 (ConstructL.cpp:177-185);
+    // L is linear, so L(f + la q) and L(f - la q) are projected instead, and
+    // the 1/2 is applied after the reconstruction: every WENO5 variant
+    // satisfies W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.
+    R gp[M][5], gm[M][5];
+    for (int s = 0; s < 6; s++) {
+        R vp[M], vm[M], t[M];
+        for (int k = 0; k < M; k++) {
+            vp[k] = fs[s][k] + la * qs[s][k];
+            vm[k] = fs[s][k] - la * qs[s][k];
         }
-        gh[k] = weno5<WK>(gp, wp) + weno5<WK>(gm, wp);
+        if (s < 5) {
+            P::left(e, vp, t);
+            for (int k = 0; k < M; k++) gp[k][s] = t[k];
+        }
+        if (s > 0) {
+            P::left(e, vm, t);
+            for (int k = 0; k < M; k++) gm[k][5 - s] = t[k];
+        }
     }
+    const WenoParams<R> wp4{R(4) * wp.eps, wp.power};
+    R gh[M];
+    for (int k = 0; k < M; k++) gh[k] = R(0.5) * (weno5<WK>(gp[k], wp4) + weno5<WK>(gm[k], wp4));
     R fr[M];
     P::right(e, gh, fr);
     for (int k = 0; k < M; k++) fh[P::rot_q(k, dir)] = fr[k];
@@ -232,10 +242,11 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
     const long long idx = (long long)gi * a.my + gj;
     const int c = (ci + 3) * S::HJ + (cj + 3);
     const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
+    const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
     for (int m = 0; m < M; m++) {
         // Lstar = -(Fhat(i+1)-Fhat(i))/dx - (Ghat(j+1)-Ghat(j))/dy  (ConstructL.cpp:388-399)
-        double L = -(double(sm.fx[m][(ci + 1) * TJ + cj]) - double(sm.fx[m][ci * TJ + cj])) / a.dx;
-        L = L - (double(sm.gy[m][ci * (TJ + 1) + cj + 1]) - double(sm.gy[m][ci * (TJ + 1) + cj])) / a.dy;
+        double L = -(double(sm.fx[m][(ci + 1) * TJ + cj]) - double(sm.fx[m][ci * TJ + cj])) * idx_;
+        L = L - (double(sm.gy[m][ci * (TJ + 1) + cj + 1]) - double(sm.gy[m][ci * (TJ + 1) + cj])) * idy_;
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
         if (!a.out) continue;
         const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : a.qa[m * a.plane + idx];

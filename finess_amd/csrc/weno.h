@@ -53,14 +53,6 @@ __host__ __device__ __forceinline__ R fmin3(R a, R b, R c)
     return fmin(fmin(a, b), c);
 }
 
-// Unnormalised JS5 weights for power_param == 2 (see the header comment).
-__host__ __device__ __forceinline__ void js5_p2_terms(double b0, double b1, double b2, double eps,
-                                                      double& t0, double& t1, double& t2)
-{
-    double a0 = eps + b0, a1 = eps + b1, a2 = eps + b2;
-    a0 *= a0; a1 *= a1; a2 *= a2;
-    t0 = 0.1 * (a1 * a2); t1 = 0.6 * (a0 * a2); t2 = 0.3 * (a0 * a1);
-}
 __host__ __device__ __forceinline__ float frcp(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -68,6 +60,29 @@ __host__ __device__ __forceinline__ float frcp(float x)
 #else
     return 1.0f / x;
 #endif
+}
+
+// 1/x to full FP64 accuracy (within an ulp) without the IEEE division
+// sequence: hardware reciprocal estimate refined by two Newton steps.
+__host__ __device__ __forceinline__ double rcp_nr(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+#else
+    return 1.0 / x;
+#endif
+}
+__host__ __device__ __forceinline__ float rcp_nr(float x) { return frcp(x); }
+
+// Unnormalised JS5 weights for power_param == 2 (see the header comment).
+__host__ __device__ __forceinline__ void js5_p2_terms(double b0, double b1, double b2, double eps,
+                                                      double& t0, double& t1, double& t2)
+{
+    double a0 = eps + b0, a1 = eps + b1, a2 = eps + b2;
+    a0 *= a0; a1 *= a1; a2 *= a2;
+    t0 = 0.1 * (a1 * a2); t1 = 0.6 * (a0 * a2); t2 = 0.3 * (a0 * a1);
 }
 __host__ __device__ __forceinline__ void js5_p2_terms(float b0, float b1, float b2, float eps,
                                                       float& t0, float& t1, float& t2)
@@ -118,7 +133,7 @@ __host__ __device__ __forceinline__ R weno5(const R g[5], const WenoParams<R>& w
     if (KIND == WENO_JS5_P2) {
         R t0, t1, t2;
         js5_p2_terms(b0, b1, b2, wp.eps, t0, t1, t2);
-        return (t0 * u1 + t1 * u2 + t2 * u3) / (t0 + t1 + t2);
+        return (t0 * u1 + t1 * u2 + t2 * u3) * rcp_nr(t0 + t1 + t2);
     } else if (KIND == WENO_JS5_POW) {
         const R t0 = R(0.1) * pow(wp.eps + b0, -wp.power);
         const R t1 = R(0.6) * pow(wp.eps + b1, -wp.power);

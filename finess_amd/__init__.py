@@ -19,7 +19,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfiness_hip.so")
+LIB_PATH = os.environ.get("FINESS_HIP_LIB") or os.path.join(_HERE, "libfiness_hip.so")
 
 # status codes (include/finess_hip.h)
 OK, ERR_CONFIG, ERR_NONFINITE, ERR_NV, ERR_HIP, ERR_DT, ERR_CFL, ERR_ARG, ERR_NODEVICE = 0, -1, -2, -3, -4, -5, -6, -7, -8

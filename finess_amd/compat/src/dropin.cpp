@@ -1,0 +1,364 @@
+// dropin.cpp -- ConstructL / FinSolveRK / FinSolveLxW with the reference's
+// C++ signatures, running on the GPU through the C-ABI of libfiness_hip.so.
+// State crosses the boundary through the tensors' public numel/vget/vset, so
+// this file compiles against the reference's own lib/tensors.h as well.
+//
+// Error behaviour follows the reference: an exceeded step count, dt > max_dt
+// and CFL > 1e8 print the reference's messages and exit(1)
+// (lib/2d/FinSolveRK.cpp:84-92, lib/2d/GetCFL.cpp:43-67); an unsupported
+// configuration throws std::logic_error like GetWenoReconstruct
+// (lib/WenoReconstruct.cpp:70); device failures throw std::runtime_error.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <vector>
+
+#include "../../../include/finess_hip.h"
+#include "IniParams.h"
+#include "finess_dropin.h"
+
+// The app callbacks are optional for the library: weak references resolve to
+// null when an app (or a test) does not link them.
+#if NDIMS == 1
+#pragma weak FluxFunc
+#pragma weak SetWaveSpd
+#else
+#pragma weak FluxFunc
+#pragma weak ProjectLeftEig
+#pragma weak ProjectRightEig
+#pragma weak SetWaveSpd
+#endif
+#pragma weak SetBndValues
+
+namespace finess_dropin {
+
+namespace {
+finess_ctx* g_ctx = nullptr;
+finess_params g_params;
+ProbeReport g_report;
+bool g_probed = false;
+
+[[noreturn]] void fail_like_reference(int code, const char* what)
+{
+    switch (code) {
+    case FINESS_ERR_NV:
+        printf(" Error in FinSolveRK.cpp: ");
+        printf("Exceeded allowed # of time steps \n");
+        printf("Terminating program.\n");
+        exit(1);
+    case FINESS_ERR_DT:
+        std::cout << std::endl << " Error: dt is out of bounds ... " << std::endl << std::endl;
+        exit(1);
+    case FINESS_ERR_CFL:
+        std::cout << std::endl << " Error: CFL number is out of bounds ... " << std::endl << std::endl;
+        exit(1);
+    case FINESS_ERR_CONFIG:
+        throw std::logic_error(std::string("finess-hip: configuration not supported on the device: ") + what);
+    default:
+        throw std::runtime_error(std::string("finess-hip: ") + what);
+    }
+}
+
+int bc_word(const std::string& w)
+{
+    if (w == "periodic") return FINESS_BC_PERIODIC;
+    if (w == "reflective" || w == "wall" || w == "reflect") return FINESS_BC_REFLECT;
+    if (w == "extrapolation" || w == "transmissive" || w == "outflow") return FINESS_BC_EXTRAP;
+    throw std::logic_error("gpu.boundary: unknown boundary kind " + w);
+}
+
+// ---------------------------------------------------------------- physics
+double rel(double a, double b) { return std::fabs(a - b) / (1.0 + std::fabs(b)); }
+
+#if NDIMS == 2
+bool euler_matches(double gamma, std::string& notes)
+{
+    if (!&FluxFunc) return false;
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const int n = 64;
+    dTensor2 x(n, 2), Q(n, 5), A(n, 1);
+    dTensor3 F(n, 5, 2);
+    std::vector<double> qs(5 * n);
+    for (int i = 1; i <= n; i++) {
+        const double rho = 0.3 + U(rng), u = U(rng) - 0.5, v = U(rng) - 0.5, w = 0.0, p = 0.2 + U(rng);
+        const double E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w);
+        const double q[5] = {rho, rho * u, rho * v, rho * w, E};
+        for (int m = 1; m <= 5; m++) Q.set(i, m, q[m - 1]);
+        x.set(i, 1, U(rng)); x.set(i, 2, U(rng)); A.set(i, 1, 0.0);
+    }
+    FluxFunc(x, Q, A, F);
+    double err = 0;
+    for (int i = 1; i <= n; i++) {
+        const double rho = Q.get(i, 1), u = Q.get(i, 2) / rho, v = Q.get(i, 3) / rho, E = Q.get(i, 5);
+        const double p = (gamma - 1) * (E - 0.5 * rho * (u * u + v * v));
+        const double f[5] = {rho * u, rho * u * u + p, rho * u * v, 0.0, u * (E + p)};
+        const double g[5] = {rho * v, rho * v * u, rho * v * v + p, 0.0, v * (E + p)};
+        for (int m = 1; m <= 5; m++) err = std::max(err, std::max(rel(F.get(i, m, 1), f[m - 1]), rel(F.get(i, m, 2), g[m - 1])));
+    }
+    if (err > 1e-12) { notes += "FluxFunc is not the Euler flux; "; return false; }
+    if (&ProjectLeftEig && &ProjectRightEig) {
+        // left and right eigenvector projections must be inverse to each other
+        dTensor1 aux(1), qa(5);
+        for (int m = 1; m <= 5; m++) qa.set(m, Q.get(1, m));
+        aux.set(1, 0.0);
+        dTensor2 V(5, 5), W(5, 5), B(5, 5);
+        for (int a = 1; a <= 5; a++)
+            for (int b = 1; b <= 5; b++) V.set(a, b, a == b ? 1.0 : 0.0);
+        for (int ixy = 1; ixy <= 2; ixy++) {
+            ProjectLeftEig(ixy, aux, qa, V, W);
+            ProjectRightEig(ixy, aux, qa, W, B);
+            for (int a = 1; a <= 5; a++)
+                for (int b = 1; b <= 5; b++)
+                    if (std::fabs(B.get(a, b) - (a == b ? 1.0 : 0.0)) > 1e-10) {
+                        notes += "ProjectRightEig is not the inverse of ProjectLeftEig; ";
+                        return false;
+                    }
+        }
+    }
+    return true;
+}
+
+bool sw_matches(std::string& notes)
+{
+    if (!&FluxFunc) return false;
+    std::mt19937_64 rng(9);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const int n = 64;
+    dTensor2 x(n, 2), Q(n, 3), A(n, 1);
+    dTensor3 F(n, 3, 2);
+    for (int i = 1; i <= n; i++) {
+        const double h = 0.5 + U(rng), u = U(rng) - 0.5, v = U(rng) - 0.5;
+        Q.set(i, 1, h); Q.set(i, 2, h * u); Q.set(i, 3, h * v);
+        x.set(i, 1, 0.0); x.set(i, 2, 0.0); A.set(i, 1, 0.0);
+    }
+    FluxFunc(x, Q, A, F);
+    double err = 0;
+    for (int i = 1; i <= n; i++) {
+        const double h = Q.get(i, 1), u = Q.get(i, 2) / h, v = Q.get(i, 3) / h;
+        const double f[3] = {h * u, h * u * u + 0.5 * h * h, h * u * v};
+        const double g[3] = {h * v, h * v * u, h * v * v + 0.5 * h * h};
+        for (int m = 1; m <= 3; m++) err = std::max(err, std::max(rel(F.get(i, m, 1), f[m - 1]), rel(F.get(i, m, 2), g[m - 1])));
+    }
+    if (err > 1e-12) { notes += "FluxFunc is not the shallow-water flux (g = 1); "; return false; }
+    return true;
+}
+#else
+bool advection_matches(std::string& notes)
+{
+    if (!&FluxFunc) return false;
+    const int n = 16;
+    dTensor1 x(n);
+    dTensor2 Q(n, 1), A(n, 1), F(n, 1);
+    for (int i = 1; i <= n; i++) { x.set(i, 0.1 * i); Q.set(i, 1, 0.3 * i - 1.0); A.set(i, 1, 0.5 + 0.1 * i); }
+    FluxFunc(x, Q, A, F);
+    for (int i = 1; i <= n; i++)
+        if (rel(F.get(i, 1), Q.get(i, 1) * A.get(i, 1)) > 1e-14) { notes += "FluxFunc is not aux(1)*q; "; return false; }
+    return true;
+}
+#endif
+
+// ---------------------------------------------------------- boundary probe
+// Cells carry distinct positive codes; after the app's SetBndValues every
+// ghost cell (corners excluded) names its source, which identifies the rule
+// on each side.
+#if NDIMS == 2
+double code(int i, int j, int m, int my) { return 1.0 + m + 8.0 * (double)(j + 64 + (my + 200) * (double)(i + 64)); }
+
+bool classify_side(StateVars& Q, int side, int mx, int my, int mbc, int meqn, int nrm, int& kind)
+{
+    const dTensorBC3& q = Q.const_ref_q();
+    int votes[3] = {1, 1, 1};  // extrap, periodic, reflect all possible until contradicted
+    auto check = [&](int gi, int gj, int ei, int ej, int pi, int pj, int ri, int rj) {
+        for (int m = 1; m <= meqn; m++) {
+            const double v = q.get(gi, gj, m);
+            if (v != code(ei, ej, m, my)) votes[0] = 0;
+            if (v != code(pi, pj, m, my)) votes[1] = 0;
+            const double rv = (m == nrm ? -1.0 : 1.0) * code(ri, rj, m, my);
+            if (v != rv) votes[2] = 0;
+        }
+    };
+    for (int g = 1; g <= mbc; g++) {
+        if (side == 0) for (int j = 1; j <= my; j++) check(1 - g, j, 1, j, mx + 1 - g, j, g, j);
+        if (side == 1) for (int j = 1; j <= my; j++) check(mx + g, j, mx, j, g, j, mx + 1 - g, j);
+        if (side == 2) for (int i = 1; i <= mx; i++) check(i, 1 - g, i, 1, i, my + 1 - g, i, g);
+        if (side == 3) for (int i = 1; i <= mx; i++) check(i, my + g, i, my, i, g, i, my + 1 - g);
+    }
+    const int n = votes[0] + votes[1] + votes[2];
+    if (n == 0) return false;
+    kind = votes[1] ? FINESS_BC_PERIODIC : votes[0] ? FINESS_BC_EXTRAP : FINESS_BC_REFLECT;
+    return true;
+}
+#else
+double code(int i, int m) { return 1.0 + m + 8.0 * (double)(i + 64); }
+#endif
+
+} // namespace
+
+ProbeReport probe_app()
+{
+    const IniParams& ini = global_ini_params;
+    ProbeReport r;
+    r.physics = -1;
+    r.physics_checked = false;
+    r.bc_checked = false;
+    for (int k = 0; k < 4; k++) r.bc[k] = FINESS_BC_EXTRAP;
+    const std::string phys = ini.get_gpu("physics", "");
+    const int meqn = ini.get_meqn();
+#if NDIMS == 2
+    if (phys == "euler" || (phys.empty() && meqn == 5)) r.physics = FINESS_PHYS_EULER;
+    else if (phys == "shallow_water" || (phys.empty() && meqn == 3)) r.physics = FINESS_PHYS_SHALLOW_WATER;
+    if (&FluxFunc) {
+        if (r.physics == FINESS_PHYS_EULER) r.physics_checked = euler_matches(ini.get_gamma(), r.notes);
+        else if (r.physics == FINESS_PHYS_SHALLOW_WATER) r.physics_checked = sw_matches(r.notes);
+        if (!r.physics_checked) throw std::logic_error("finess-hip: the app's physics matches no device physics: " + r.notes);
+    }
+#else
+    if (phys == "advection" || (phys.empty() && meqn == 1)) r.physics = FINESS_PHYS_ADVECTION;
+    if (&FluxFunc) {
+        r.physics_checked = advection_matches(r.notes);
+        if (!r.physics_checked) throw std::logic_error("finess-hip: the app's physics matches no device physics: " + r.notes);
+    }
+#endif
+    if (r.physics < 0) throw std::logic_error("finess-hip: no device physics for meqn = " + std::to_string(meqn));
+
+    const std::string words = ini.get_gpu("boundary", "");
+    if (!words.empty()) {
+        std::istringstream in(words);
+        std::string w;
+        for (int k = 0; k < 4 && (in >> w); k++) r.bc[k] = bc_word(w);
+    } else if (&SetBndValues) {
+        const int mx = ini.get_mx(), mbc = ini.get_mbc(), maux = ini.get_maux();
+#if NDIMS == 2
+        const int my = ini.get_my();
+        StateVars Q(0.0, mx, my, meqn, maux, mbc);
+        dTensorBC3& q = Q.ref_q();
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int j = 1 - mbc; j <= my + mbc; j++)
+                for (int m = 1; m <= meqn; m++)
+                    q.set(i, j, m, (i >= 1 && i <= mx && j >= 1 && j <= my) ? code(i, j, m, my) : NAN);
+        SetBndValues(Q);
+        const int nx = r.physics == FINESS_PHYS_ADVECTION ? -1 : 2, ny = r.physics == FINESS_PHYS_ADVECTION ? -1 : 3;
+        for (int side = 0; side < 4; side++) {
+            int kind = 0;
+            if (!classify_side(Q, side, mx, my, mbc, meqn, side < 2 ? nx : ny, kind))
+                throw std::logic_error("finess-hip: SetBndValues side " + std::to_string(side) +
+                                       " is not an extrapolation, periodic or reflecting rule");
+            r.bc[side] = kind;
+        }
+#else
+        StateVars Q(0.0, mx, meqn, maux, mbc);
+        dTensorBC2& q = Q.ref_q();
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= meqn; m++) q.set(i, m, (i >= 1 && i <= mx) ? code(i, m) : NAN);
+        SetBndValues(Q);
+        for (int side = 0; side < 2; side++) {
+            int votes[2] = {1, 1};
+            for (int g = 1; g <= mbc; g++)
+                for (int m = 1; m <= meqn; m++) {
+                    const int gi = side ? mx + g : 1 - g;
+                    const double v = q.get(gi, m);
+                    if (v != code(side ? mx : 1, m)) votes[0] = 0;
+                    if (v != code(side ? g : mx + 1 - g, m)) votes[1] = 0;
+                }
+            if (!votes[0] && !votes[1])
+                throw std::logic_error("finess-hip: 1D SetBndValues is neither periodic nor extrapolation");
+            r.bc[side] = votes[1] ? FINESS_BC_PERIODIC : FINESS_BC_EXTRAP;
+        }
+        r.bc[2] = r.bc[3] = r.bc[0];
+#endif
+        r.bc_checked = true;
+    }
+    return r;
+}
+
+void release()
+{
+    if (g_ctx) finess_destroy(g_ctx);
+    g_ctx = nullptr;
+    g_probed = false;
+}
+
+namespace {
+finess_ctx* context()
+{
+    if (!g_probed) {
+        g_report = probe_app();
+        g_probed = true;
+    }
+    finess_params p;
+    global_ini_params.to_finess_params(&p);
+    p.physics = g_report.physics;
+    for (int k = 0; k < 4; k++) p.bc[k] = g_report.bc[k];
+    int rc = finess_params_finalize(&p);
+    if (rc) fail_like_reference(rc, "parameter combination");
+    if (g_ctx && std::memcmp(&p, &g_params, sizeof p) == 0) return g_ctx;
+    if (g_ctx) finess_destroy(g_ctx);
+    g_ctx = nullptr;
+    rc = finess_create(&p, &g_ctx);
+    if (rc) fail_like_reference(rc, "finess_create failed");
+    g_params = p;
+    return g_ctx;
+}
+
+template <class T>
+std::vector<double> gather(const T& t)
+{
+    std::vector<double> v((size_t)t.numel());
+    for (int k = 0; k < t.numel(); k++) v[k] = t.vget(k);
+    return v;
+}
+template <class T>
+void scatter(const std::vector<double>& v, T& t)
+{
+    for (int k = 0; k < t.numel(); k++) t.vset(k, v[k]);
+}
+
+void fin_solve(StateVars& Qnew, double tend, double dtv[], bool lxw)
+{
+    finess_ctx* c = context();
+    std::vector<double> q = gather(Qnew.const_ref_q()), aux = gather(Qnew.const_ref_aux());
+    double t = Qnew.get_t();
+    const int rc = lxw ? finess_fin_solve_lxw(c, q.data(), aux.data(), &t, tend, dtv)
+                       : finess_fin_solve_rk(c, q.data(), aux.data(), &t, tend, dtv);
+    if (rc) fail_like_reference(rc, finess_last_error(c));
+    scatter(q, Qnew.ref_q());
+    Qnew.set_t(t);
+}
+} // namespace
+} // namespace finess_dropin
+
+using namespace finess_dropin;
+
+#if NDIMS == 2
+void ConstructL(StateVars& Q, dTensorBC3& Lstar, dTensorBC3& smax)
+{
+    finess_ctx* c = context();
+    std::vector<double> q = gather(Q.const_ref_q()), aux = gather(Q.const_ref_aux());
+    std::vector<double> L = gather(Lstar), sm = gather(smax);
+    const int rc = finess_construct_l(c, q.data(), aux.data(), L.data(), sm.data());
+    if (rc) fail_like_reference(rc, finess_last_error(c));
+    scatter(L, Lstar);
+    scatter(sm, smax);
+}
+
+void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, false); }
+void FinSolveLxW(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, true); }
+#else
+void ConstructL(const StateVars& Q, dTensorBC2& Lstar, dTensorBC1& smax)
+{
+    finess_ctx* c = context();
+    std::vector<double> q = gather(Q.const_ref_q()), aux = gather(Q.const_ref_aux());
+    std::vector<double> L = gather(Lstar), sm = gather(smax);
+    const int rc = finess_construct_l(c, q.data(), aux.data(), L.data(), sm.data());
+    if (rc) fail_like_reference(rc, finess_last_error(c));
+    scatter(L, Lstar);
+    scatter(sm, smax);
+}
+
+void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, false); }
+#endif

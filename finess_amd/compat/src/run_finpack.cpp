@@ -1,0 +1,134 @@
+// run_finpack.cpp -- a host frame loop for apps built against this library:
+// parameters.ini -> initial data -> nout frames of FinSolveRK / FinSolveLxW ->
+// q%04d.dat files.  It follows lib/2d/RunFinpack.cpp:17-130 (sampling at
+// cell centres over the whole grid including ghosts, dtv carried across
+// frames, tend = tstart + tfinal/nout) and the ASCII frame format of
+// lib/2d/Output.cpp:26-44 (time, then q in m -> j -> i order, %24.16e).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+
+#include "IniParams.h"
+#include "finess_dropin.h"
+
+#if NDIMS == 1
+void QinitFunc(const dTensor1& xpts, dTensor2& qvals);
+void AuxFunc(const dTensor1& xpts, dTensor2& auxvals);
+#else
+void QinitFunc(const dTensor2& xpts, dTensor2& qvals);
+void AuxFunc(const dTensor2& xpts, dTensor2& auxvals);
+#endif
+#pragma weak AuxFunc
+
+namespace {
+
+void write_frame(const StateVars& Q, int n)
+{
+    const IniParams& ini = global_ini_params;
+    std::ostringstream fn;
+    fn << ini.get_output_dir() << "/q" << std::setfill('0') << std::setw(4) << n << ".dat";
+    std::ofstream f(fn.str().c_str());
+    f << std::setprecision(16);
+    f << std::setw(24) << std::scientific << Q.get_t() << "\n";
+    const int meqn = ini.get_meqn(), mx = ini.get_mx();
+#if NDIMS == 1
+    for (int m = 1; m <= meqn; m++)
+        for (int i = 1; i <= mx; i++) f << std::setw(24) << std::scientific << Q.const_ref_q().get(i, m) << "\n";
+#else
+    const int my = ini.get_my();
+    for (int m = 1; m <= meqn; m++)
+        for (int j = 1; j <= my; j++)
+            for (int i = 1; i <= mx; i++) f << std::setw(24) << std::scientific << Q.const_ref_q().get(i, j, m) << "\n";
+#endif
+}
+
+void sample(StateVars& Q)
+{
+    const IniParams& ini = global_ini_params;
+    const int mx = ini.get_mx(), mbc = ini.get_mbc(), meqn = ini.get_meqn(), na = std::max(1, ini.get_maux());
+    const double dx = ini.get_dx(), xlow = ini.get_xlow();
+#if NDIMS == 1
+    const int n = mx + 2 * mbc;
+    dTensor1 x(n);
+    for (int i = 1 - mbc; i <= mx + mbc; i++) x.set(i + mbc, xlow + (i - 0.5) * dx);
+    dTensor2 qv(n, meqn), av(n, na);
+    QinitFunc(x, qv);
+    if (&AuxFunc) AuxFunc(x, av); else av.setall(1.0);
+    for (int i = 1 - mbc; i <= mx + mbc; i++) {
+        for (int m = 1; m <= meqn; m++) Q.ref_q().set(i, m, qv.get(i + mbc, m));
+        for (int m = 1; m <= na; m++) Q.ref_aux().set(i, m, av.get(i + mbc, m));
+    }
+#else
+    const int my = ini.get_my();
+    const double dy = ini.get_dy(), ylow = ini.get_ylow();
+    const int nyt = my + 2 * mbc, n = (mx + 2 * mbc) * nyt;
+    dTensor2 x(n, 2), qv(n, meqn), av(n, na);
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            const int k = (i + mbc - 1) * nyt + (j + mbc - 1) + 1;
+            x.set(k, 1, xlow + (i - 0.5) * dx);
+            x.set(k, 2, ylow + (j - 0.5) * dy);
+        }
+    QinitFunc(x, qv);
+    if (&AuxFunc) AuxFunc(x, av); else av.setall(0.0);
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            const int k = (i + mbc - 1) * nyt + (j + mbc - 1) + 1;
+            for (int m = 1; m <= meqn; m++) Q.ref_q().set(i, j, m, qv.get(k, m));
+            for (int m = 1; m <= na; m++) Q.ref_aux().set(i, j, m, av.get(k, m));
+        }
+#endif
+}
+
+} // namespace
+
+// main_global equivalent (lib/main_global.cpp:24-75): returns the exit code.
+int finess_run(int argc, char* argv[])
+{
+    const std::string ini_file = argc > 1 ? argv[1] : "parameters.ini";
+    try {
+        global_ini_params.init(ini_file);
+        const IniParams& ini = global_ini_params;
+        mkdir(ini.get_output_dir().c_str(), 0755);
+#if NDIMS == 1
+        StateVars Q(0.0, ini.get_mx(), ini.get_meqn(), ini.get_maux(), ini.get_mbc());
+#else
+        StateVars Q(0.0, ini.get_mx(), ini.get_my(), ini.get_meqn(), ini.get_maux(), ini.get_mbc());
+#endif
+        sample(Q);
+        write_frame(Q, 0);
+        double dtv[3] = {0.0, ini.get_initial_dt(), ini.get_max_dt()};
+        const double dtout = ini.get_tfinal() / double(ini.get_nout());
+        double tend = 0.0;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int n = 1; n <= ini.get_nout(); n++) {
+            const double tstart = tend;
+            tend = tstart + dtout;
+#if NDIMS == 1
+            FinSolveRK(Q, tend, dtv);
+#else
+            if (ini.get_time_stepping_method() == IniParams::TimeSteppingMethod::LxW)
+                FinSolveLxW(Q, tend, dtv);
+            else
+                FinSolveRK(Q, tend, dtv);
+#endif
+            write_frame(Q, n);
+            std::cout << "FINESS: Frame " << std::setw(3) << n << ": plot files done at time t ="
+                      << std::setw(12) << std::scientific << std::setprecision(5) << tend << std::endl;
+        }
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << " finess-hip: frames done in " << std::fixed << std::setprecision(3) << el << " s" << std::endl;
+        finess_dropin::release();
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

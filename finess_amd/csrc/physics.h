@@ -37,15 +37,15 @@ struct PhysParams {
 // ------------------------------------------------------------------------
 struct Euler2D {
     static constexpr int MEQN = 5;
-    static constexpr int NV = 10;
+    static constexpr int NV = 8;
     static constexpr int NORMAL_X = 1;  // momentum component normal to x-walls
     static constexpr int NORMAL_Y = 2;
 
-    // cell record: q0..q4, u, v, w, p, c  with c = sqrt(|gamma p / rho|),
-    // the cell sound speed SetWaveSpd uses for the left and right states
+    // cell record: q0..q4, 1/rho, p, c  with c = sqrt(|gamma p / rho|), the
+    // cell sound speed SetWaveSpd uses for the left and right states
     // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Always formed in FP64 (the
     // pressure recovery keeps FP64 in the FP32 form as well).
-    __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[10])
+    __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[8])
     {
         const double gamma = pp.gamma;
         const double rho = q[0];
@@ -53,8 +53,8 @@ struct Euler2D {
         const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
         const double press = (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
-        v[5] = u1; v[6] = u2; v[7] = u3; v[8] = press;
-        v[9] = sqrt(fabs(gamma * press * irho));
+        v[5] = irho; v[6] = press;
+        v[7] = sqrt(fabs(gamma * press * irho));
     }
 
     // LDS variable index of rotated component k (dir 0: x, 1: y)
@@ -63,20 +63,25 @@ struct Euler2D {
         return (dir && (k == 1 || k == 2)) ? 3 - k : k;
     }
 
-    // Rotated state and flux of one cell record (FluxFunc.cpp:20-40).
+    // Rotated state of one cell record.
     template <typename R>
-    __host__ __device__ __forceinline__ static void state_flux(const R c[10], int dir, R q[5], R f[5])
+    __host__ __device__ __forceinline__ static void state(const R c[8], int dir, R q[5])
     {
-        const R rho = c[0], E = c[4], p = c[8];
-        const R mn = dir ? c[2] : c[1], mt = dir ? c[1] : c[2];
-        const R un = dir ? c[6] : c[5], ut = dir ? c[5] : c[6], uw = c[7];
-        q[0] = rho; q[1] = mn; q[2] = mt; q[3] = c[3]; q[4] = E;
-        const R run = rho * un;
+        q[0] = c[0]; q[1] = dir ? c[2] : c[1]; q[2] = dir ? c[1] : c[2]; q[3] = c[3]; q[4] = c[4];
+    }
+
+    // Rotated flux of a rotated state with its record (FluxFunc.cpp:20-40).
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux(const R q[5], const R c[8], R f[5])
+    {
+        const R irho = c[5], p = c[6];
+        const R un = q[1] * irho, ut = q[2] * irho, uw = q[3] * irho;
+        const R run = q[0] * un;
         f[0] = run;
         f[1] = run * un + p;
         f[2] = run * ut;
         f[3] = run * uw;
-        f[4] = un * (E + p);
+        f[4] = un * (q[4] + p);
     }
 
     // Eigen-system at the arithmetic-mean state (lib/2d/ConstructL.cpp:72-77)
@@ -144,19 +149,20 @@ struct Euler2D {
         q[4] = (e.H - uc) * w[0] + e.hu * w[1] + e.u2 * w[2] + e.u3 * w[3] + (e.H + uc) * w[4];
     }
 
-    // SetWaveSpd with nvec = the face normal; cl, cr are the cell records of
-    // the left and right cells.  Returns max(|s1|,|s2|).
+    // SetWaveSpd with nvec = the face normal; ql, qr the rotated states and
+    // cl, cr the records of the left and right cells.  Returns max(|s1|,|s2|).
     template <typename R>
-    __host__ __device__ __forceinline__ static R wave_speed(const R cl[10], const R cr[10], int dir, const PhysParams& pp)
+    __host__ __device__ __forceinline__ static R wave_speed(const R ql[5], const R qr[5], const R cl[8], const R cr[8],
+                                                            const PhysParams& pp)
     {
         const R gamma = R(pp.gamma);
-        const R unl = dir ? cl[6] : cl[5], unr = dir ? cr[6] : cr[5];
-        const R rho = R(0.5) * (cl[0] + cr[0]);
+        const R unl = ql[1] * cl[5], unr = qr[1] * cr[5];
+        const R rho = R(0.5) * (ql[0] + qr[0]);
         const R un = R(0.5) * (unl + unr);
-        const R press = R(0.5) * (cl[8] + cr[8]);
+        const R press = R(0.5) * (cl[6] + cr[6]);
         const R c = sqrt(fabs(gamma * press / rho));
-        const R s1 = rmin(unl - cl[9], un - c);
-        const R s2 = rmax(unr + cr[9], un + c);
+        const R s1 = rmin(unl - cl[7], un - c);
+        const R s2 = rmax(unr + cr[7], un + c);
         return rmax(fabs(s1), fabs(s2));
     }
 };
@@ -184,12 +190,17 @@ struct ShallowWater2D {
     }
 
     template <typename R>
-    __host__ __device__ __forceinline__ static void state_flux(const R c[5], int dir, R q[3], R f[3])
+    __host__ __device__ __forceinline__ static void state(const R c[5], int dir, R q[3])
     {
-        const R h = c[0];
-        const R mn = dir ? c[2] : c[1], mt = dir ? c[1] : c[2];
-        const R un = dir ? c[4] : c[3], ut = dir ? c[3] : c[4];
-        q[0] = h; q[1] = mn; q[2] = mt;
+        q[0] = c[0]; q[1] = dir ? c[2] : c[1]; q[2] = dir ? c[1] : c[2];
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux(const R q[3], const R c[5], R f[3])
+    {
+        (void)c;
+        const R h = q[0];
+        const R un = q[1] / h, ut = q[2] / h;
         const R hun = h * un;
         f[0] = hun;
         f[1] = hun * un + R(0.5) * h * h;
@@ -231,24 +242,22 @@ struct ShallowWater2D {
     }
 
     template <typename R>
-    __host__ __device__ __forceinline__ static R wave_speed(const R cl[5], const R cr[5], int dir, const PhysParams& pp)
+    __host__ __device__ __forceinline__ static R wave_speed(const R ql[3], const R qr[3], const R cl[5], const R cr[5],
+                                                            const PhysParams& pp)
     {
-        const R hl = cl[0], hr = cr[0];
-        const R unl = dir ? cl[4] : cl[3];
+        (void)cl; (void)cr;
+        const R hl = ql[0], hr = qr[0];
+        const R unl = ql[1] / hl, utl = ql[2] / hl;
         // apps/2d/shallow_water/lib/SetWaveSpd.cpp:27-28 divides the right
-        // momentum by h_l; kept when sw_compat_hl is set.
-        const R mr1 = cr[1], mr2 = cr[2];
+        // momenta by h_l; kept when sw_compat_hl is set.
         const R hden = pp.sw_compat_hl ? hl : hr;
-        const R u1r = mr1 / hden, u2r = mr2 / hden;
-        const R unr = dir ? u2r : u1r;
-        const R u1l = cl[3], u2l = cl[4];
+        const R unr = qr[1] / hden, utr = qr[2] / hden;
         const R h = R(0.5) * (hl + hr);
         const R shl = sqrt(hl), shr = sqrt(hr);
         const R idn = R(1) / (shl + shr);
-        const R u1 = (shl * u1l + shr * u1r) * idn;
-        const R u2 = (shl * u2l + shr * u2r) * idn;
-        const R un = dir ? u2 : u1;
+        const R un = (shl * unl + shr * unr) * idn;
         const R sh = sqrt(h);
+        (void)utl; (void)utr;
         const R s1 = rmin(un - sh, unl - shl);
         const R s2 = rmax(un + sh, unr + shr);
         return rmax(fabs(s1), fabs(s2));

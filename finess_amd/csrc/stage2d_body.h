@@ -6,16 +6,18 @@
 //   UpdateSoln(a1, a2, b, dt, ...)     (lib/2d/UpdateSoln.cpp:14-38)
 // and the wave-speed part of GetCFL (lib/2d/GetCFL.cpp:54-58).
 //
-// A workgroup owns a TI x TJ tile of cells (j, the contiguous index, along
-// TJ).  Phase 1 stages the tile plus a 3-cell cross-shaped halo in LDS,
-// mapping halo cells through the boundary condition on the fly (the halo
-// fill) and computing each cell's record (state + primitives) once.
-// Phase 2 computes every x-face and y-face flux of the tile: characteristic
-// projection at the mean state, local Lax-Friedrichs splitting, WENO5 on the
-// 5 characteristic fields for both split parts, projection back.  Faces are
-// dealt to threads as one list, x-faces first, so each wave runs the same
-// instructions whatever the face direction.  Phase 3 differences the face
-// fluxes into L and writes  out = a1*qa + a2*qb + (b dt)*L  in FP64.
+// A workgroup of 256 threads owns a TI x TJ = 7 x 32 tile of cells (j, the
+// contiguous index, along TJ).  Phase 1 stages the tile plus a 3-cell
+// cross-shaped halo in LDS, mapping halo cells through the boundary condition
+// on the fly (the halo fill) and computing each cell's record (state +
+// primitives) once.  The (TI+1) x TJ = 256 x-faces are then computed one per
+// thread -- characteristic projection at the mean state, local Lax-Friedrichs
+// splitting, WENO5 on every characteristic field for both split parts,
+// projection back -- and differenced into the x part of L; the same LDS face
+// buffer then takes the TI x (TJ+1) = 231 y-faces.  The last phase forms L
+// and writes  out = a1*qa + a2*qb + (b dt)*L  in FP64.  One face per thread
+// and direction keeps every wave equally busy, and sharing one face buffer
+// between the directions keeps the LDS footprint at three workgroups per CU.
 // The per-face wave speeds feed a workgroup max that the caller reduces into
 // the step's CFL slot.
 //
@@ -83,9 +85,9 @@ template <class P, typename R, int TI, int TJ>
 struct StageSmem {
     static constexpr int HI = TI + 6, HJ = TJ + 6, NC = HI * HJ;
     static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1);
+    static constexpr int NF = NFX > NFY ? NFX : NFY;
     R cv[P::NV][NC];
-    R fx[P::MEQN][NFX];
-    R gy[P::MEQN][NFY];
+    R face[P::MEQN][NF];
 };
 
 // Cell load with the boundary mapping (phase 1).
@@ -136,36 +138,42 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
                                                 const WenoParams<R>& wp, R ascale, R fh[P::MEQN])
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    R qs[6][M], fs[6][M];
-    R cl[NV], cr[NV];
-    for (int s = 0; s < 6; s++) {
-        R rec[NV];
-        for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
-        P::state_flux(rec, dir, qs[s], fs[s]);
-        if (s == 2) for (int v = 0; v < NV; v++) cl[v] = rec[v];
-        if (s == 3) for (int v = 0; v < NV; v++) cr[v] = rec[v];
-        if (LXW) {
-            for (int k = 0; k < M; k++)
-                fs[s][k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
-        }
+    // mean state and wave speed from the two cells beside the face
+    R cl[NV], cr[NV], ql[M], qr[M];
+    for (int v = 0; v < NV; v++) {
+        cl[v] = cv[v][base + 2 * stride];
+        cr[v] = cv[v][base + 3 * stride];
     }
+    P::state(cl, dir, ql);
+    P::state(cr, dir, qr);
     R qa[M];
-    for (int k = 0; k < M; k++) qa[k] = R(0.5) * (qs[3][k] + qs[2][k]);
+    for (int k = 0; k < M; k++) qa[k] = R(0.5) * (qr[k] + ql[k]);
     typename P::template Eig<R> e;
     P::eig(qa, pp, e);
-    const R alpha = P::wave_speed(cl, cr, dir, pp);
+    const R alpha = P::wave_speed(ql, qr, cl, cr, pp);
     const R la = ascale * alpha;
     // Lax-Friedrichs split projected in one go: the reference forms
     // gp = (L f + la L q)/2 and gm = (L f - la L q)/2 (ConstructL.cpp:177-185);
     // L is linear, so L(f + la q) and L(f - la q) are projected instead, and
     // the 1/2 is applied after the reconstruction: every WENO5 variant
     // satisfies W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.
+    // Each stencil point is projected as soon as it is read, so only the
+    // projected values stay live.
     R gp[M][5], gm[M][5];
+#pragma unroll
     for (int s = 0; s < 6; s++) {
+        R rec[NV], q[M], f[M];
+        for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
+        P::state(rec, dir, q);
+        if (LXW) {
+            for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
+        } else {
+            P::flux(q, rec, f);
+        }
         R vp[M], vm[M], t[M];
         for (int k = 0; k < M; k++) {
-            vp[k] = fs[s][k] + la * qs[s][k];
-            vm[k] = fs[s][k] - la * qs[s][k];
+            vp[k] = f[k] + la * q[k];
+            vm[k] = f[k] - la * q[k];
         }
         if (s < 5) {
             P::left(e, vp, t);
@@ -185,54 +193,66 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     return alpha;
 }
 
+// One face per thread for direction dir (0: x-faces, thread t -> face
+// (t / TJ, t % TJ), i.e. between cells fi-1 and fi; 1: y-faces, thread t ->
+// (t / (TJ+1), t % (TJ+1))).  Writes the face flux to sm.face and folds its
+// wave speed into amax when GetCFL counts it (faces mx+1/2 and my+1/2 are
+// skipped, lib/2d/GetCFL.cpp:54-58).
 template <class P, typename R, int WK, bool LXW, int TI, int TJ>
-__host__ __device__ __forceinline__ void phase_faces(int tid, int nthreads, int i0, int j0, const StageArgs& a,
-                                                     StageSmem<P, R, TI, TJ>& sm, double& ax, double& ay)
+__host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, int j0, const StageArgs& a,
+                                                     StageSmem<P, R, TI, TJ>& sm, double& amax)
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
-    const WenoParams<R> wp{R(a.eps), R(a.power)};
-    const R ascale = R(a.alpha_scaling);
     const long long lld = a.my + 6, lpl = (long long)(a.mx + 6) * lld;
-    for (int f = tid; f < S::NFX + S::NFY; f += nthreads) {
-        int dir, fi, fj, base, stride;
-        long long lbase = 0, lstride = 0;
-        bool valid, count;
-        if (f < S::NFX) {
-            dir = 0; fi = f / TJ; fj = f - fi * TJ;
-            base = fi * S::HJ + (fj + 3); stride = S::HJ;
-            const int gi = i0 + fi, gj = j0 + fj;
-            valid = gi <= a.mx && gj < a.my;
-            count = gi < a.mx;                         // GetCFL skips face mx+1/2
-            lbase = (long long)gi * lld + (gj + 3); lstride = lld;
-        } else {
-            const int h = f - S::NFX;
-            dir = 1; fi = h / (TJ + 1); fj = h - fi * (TJ + 1);
-            base = (fi + 3) * S::HJ + fj; stride = 1;
-            const int gi = i0 + fi, gj = j0 + fj;
-            valid = gi < a.mx && gj <= a.my;
-            count = gj < a.my;
-            lbase = (long long)(gi + 3) * lld + gj; lstride = 1;
-        }
-        if (!valid) continue;
-        R fh[M];
-        const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F,
-                                                        lbase, lstride, lpl, a.pp, wp, ascale, fh);
-        if (dir == 0) {
-            for (int m = 0; m < M; m++) sm.fx[m][f] = fh[m];
-            if (count) ax = rmax(ax, double(alpha));
-            if (a.smax_faces_x) a.smax_faces_x[(long long)(i0 + fi) * a.my + (j0 + fj)] = double(alpha);
-        } else {
-            for (int m = 0; m < M; m++) sm.gy[m][f - S::NFX] = fh[m];
-            if (count) ay = rmax(ay, double(alpha));
-            if (a.smax_faces_y) a.smax_faces_y[(long long)(i0 + fi) * (a.my + 1) + (j0 + fj)] = double(alpha);
-        }
+    int fi, fj, base, stride;
+    long long lbase, lstride;
+    bool valid, count;
+    if (dir == 0) {
+        if (tid >= S::NFX) return;
+        fi = tid / TJ; fj = tid - fi * TJ;
+        base = fi * S::HJ + (fj + 3); stride = S::HJ;
+        const int gi = i0 + fi, gj = j0 + fj;
+        valid = gi <= a.mx && gj < a.my;
+        count = gi < a.mx;
+        lbase = (long long)gi * lld + (gj + 3); lstride = lld;
+    } else {
+        if (tid >= S::NFY) return;
+        fi = tid / (TJ + 1); fj = tid - fi * (TJ + 1);
+        base = (fi + 3) * S::HJ + fj; stride = 1;
+        const int gi = i0 + fi, gj = j0 + fj;
+        valid = gi < a.mx && gj <= a.my;
+        count = gj < a.my;
+        lbase = (long long)(gi + 3) * lld + gj; lstride = 1;
     }
+    if (!valid) return;
+    const WenoParams<R> wp{R(a.eps), R(a.power)};
+    R fh[M];
+    const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F, lbase,
+                                                    lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh);
+    for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
+    if (count) amax = rmax(amax, double(alpha));
+    if (dir == 0 && a.smax_faces_x) a.smax_faces_x[(long long)(i0 + fi) * a.my + (j0 + fj)] = double(alpha);
+    if (dir == 1 && a.smax_faces_y) a.smax_faces_y[(long long)(i0 + fi) * (a.my + 1) + (j0 + fj)] = double(alpha);
 }
 
+// x part of L for the thread's cell: -(Fhat(i+1) - Fhat(i))/dx
+// (ConstructL.cpp:388-399).
+template <class P, typename R, int TI, int TJ>
+__host__ __device__ __forceinline__ void phase_lx(int tid, const StageArgs& a, const StageSmem<P, R, TI, TJ>& sm,
+                                                  double Lx[P::MEQN])
+{
+    const int ci = tid / TJ, cj = tid - ci * TJ;
+    if (ci >= TI) return;
+    const double idx_ = 1.0 / a.dx;
+    for (int m = 0; m < P::MEQN; m++)
+        Lx[m] = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
+}
+
+// L = Lx - (Ghat(j+1) - Ghat(j))/dy and the stage update in FP64.
 template <class P, typename R, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, const StageArgs& a,
-                                                      StageSmem<P, R, TI, TJ>& sm)
+                                                      const StageSmem<P, R, TI, TJ>& sm, const double Lx[P::MEQN])
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
@@ -242,11 +262,9 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
     const long long idx = (long long)gi * a.my + gj;
     const int c = (ci + 3) * S::HJ + (cj + 3);
     const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
-    const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+    const double idy_ = 1.0 / a.dy;
     for (int m = 0; m < M; m++) {
-        // Lstar = -(Fhat(i+1)-Fhat(i))/dx - (Ghat(j+1)-Ghat(j))/dy  (ConstructL.cpp:388-399)
-        double L = -(double(sm.fx[m][(ci + 1) * TJ + cj]) - double(sm.fx[m][ci * TJ + cj])) * idx_;
-        L = L - (double(sm.gy[m][ci * (TJ + 1) + cj + 1]) - double(sm.gy[m][ci * (TJ + 1) + cj])) * idy_;
+        const double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
         if (!a.out) continue;
         const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : a.qa[m * a.plane + idx];

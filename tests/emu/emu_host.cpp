@@ -14,7 +14,8 @@
 using namespace finess;
 
 namespace {
-constexpr int TI = 8, TJ = 32;
+constexpr int TI = 7, TJ = 32, NT = 256;
+constexpr int LTI = 8, LTJ = 32;  // tile of the Lax-Wendroff flux kernel
 
 int wk_of(const finess_params* p)
 {
@@ -32,22 +33,27 @@ std::vector<double> to_planes_ghosted(const finess_params* p, const double* q, i
     return out;
 }
 
-template <class P, typename R, int WK>
+template <class P, typename R, int WK, bool LXW = false>
+void run_tile(const StageArgs& a, int i0, int j0, StageSmem<P, R, TI, TJ>& sm)
+{
+    memset(&sm, 0, sizeof sm);
+    std::vector<double> Lx((size_t)NT * P::MEQN, 0.0);
+    for (int t = 0; t < NT; t++) phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
+    double ax = 0, ay = 0;
+    for (int t = 0; t < NT; t++) phase_faces<P, R, WK, LXW, TI, TJ>(t, 0, i0, j0, a, sm, ax);
+    for (int t = 0; t < NT; t++) phase_lx<P, R, TI, TJ>(t, a, sm, &Lx[(size_t)t * P::MEQN]);
+    for (int t = 0; t < NT; t++) phase_faces<P, R, WK, LXW, TI, TJ>(t, 1, i0, j0, a, sm, ay);
+    for (int t = 0; t < NT; t++) phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, &Lx[(size_t)t * P::MEQN]);
+}
+
+template <class P, typename R, int WK, bool LXW = false>
 void run_tiles(const StageArgs& a0)
 {
     StageArgs a = a0;
     const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
     a.ntj = ntj;
     std::vector<StageSmem<P, R, TI, TJ>> smv(1);
-    auto& sm = smv[0];
-    for (int t = 0; t < nti * ntj; t++) {
-        const int i0 = (t / ntj) * TI, j0 = (t % ntj) * TJ;
-        memset(&sm, 0, sizeof sm);
-        for (int tid = 0; tid < TI * TJ; tid++) phase_load<P, R, TI, TJ>(tid, TI * TJ, i0, j0, a, sm);
-        double ax = 0, ay = 0;
-        for (int tid = 0; tid < TI * TJ; tid++) phase_faces<P, R, WK, false, TI, TJ>(tid, TI * TJ, i0, j0, a, sm, ax, ay);
-        for (int tid = 0; tid < TI * TJ; tid++) phase_update<P, R, TI, TJ>(tid, i0, j0, a, sm);
-    }
+    for (int t = 0; t < nti * ntj; t++) run_tile<P, R, WK, LXW>(a, (t / ntj) * TI, (t % ntj) * TJ, smv[0]);
 }
 
 template <class P, typename R>
@@ -137,12 +143,12 @@ int emu_construct_integrated_r(const finess_params* p, double dt, const double* 
     a.plane = (long long)p->mx * p->my; a.time_order = p->time_order; a.dt = dt; a.dx = p->dx; a.dy = p->dy;
     a.pp.gamma = p->gamma;
     for (int k = 0; k < 4; k++) a.bc[k] = p->bc[k];
-    const int nti = (p->mx + 6 + TI - 1) / TI, ntj = (p->my + 6 + TJ - 1) / TJ;
-    std::vector<LxwSmem<TI, TJ>> smv(1);
+    const int nti = (p->mx + 6 + LTI - 1) / LTI, ntj = (p->my + 6 + LTJ - 1) / LTJ;
+    std::vector<LxwSmem<LTI, LTJ>> smv(1);
     for (int t = 0; t < nti * ntj; t++) {
-        const int e0i = (t / ntj) * TI, e0j = (t % ntj) * TJ;
-        for (int tid = 0; tid < TI * TJ; tid++) lxw_phase_load<Euler2D, TI, TJ>(tid, TI * TJ, e0i, e0j, a, smv[0]);
-        for (int tid = 0; tid < TI * TJ; tid++) lxw_phase_cell<Euler2D, TI, TJ>(tid, e0i, e0j, a, smv[0]);
+        const int e0i = (t / ntj) * LTI, e0j = (t % ntj) * LTJ;
+        for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_load<Euler2D, LTI, LTJ>(tid, LTI * LTJ, e0i, e0j, a, smv[0]);
+        for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_cell<Euler2D, LTI, LTJ>(tid, e0i, e0j, a, smv[0]);
     }
     const long long ldq = p->my + 2 * g;
     for (int i = -3; i < p->mx + 3; i++)
@@ -184,29 +190,11 @@ int emu_run_fixed_steps(const finess_params* p, double* q, double dt, int nsteps
         a.dx = p->dx; a.dy = p->dy; a.alpha_scaling = p->alpha_scaling; a.eps = p->epsilon; a.power = p->power_param;
         a.pp.gamma = p->gamma; a.pp.sw_compat_hl = p->sw_compat_hl;
         if (lxw) { a.lxw_F = Fo.data(); a.lxw_G = Go.data(); }
-        const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
-        a.ntj = ntj;
         if (p->physics == FINESS_PHYS_EULER) {
-            std::vector<StageSmem<Euler2D, double, TI, TJ>> smv(1);
-            for (int t = 0; t < nti * ntj; t++) {
-                const int i0 = (t / ntj) * TI, j0 = (t % ntj) * TJ;
-                for (int tid = 0; tid < TI * TJ; tid++) phase_load<Euler2D, double, TI, TJ>(tid, TI * TJ, i0, j0, a, smv[0]);
-                double ax = 0, ay = 0;
-                for (int tid = 0; tid < TI * TJ; tid++) {
-                    if (lxw) phase_faces<Euler2D, double, WENO_JS5_P2, true, TI, TJ>(tid, TI * TJ, i0, j0, a, smv[0], ax, ay);
-                    else phase_faces<Euler2D, double, WENO_JS5_P2, false, TI, TJ>(tid, TI * TJ, i0, j0, a, smv[0], ax, ay);
-                }
-                for (int tid = 0; tid < TI * TJ; tid++) phase_update<Euler2D, double, TI, TJ>(tid, i0, j0, a, smv[0]);
-            }
+            if (lxw) run_tiles<Euler2D, double, WENO_JS5_P2, true>(a);
+            else run_tiles<Euler2D, double, WENO_JS5_P2, false>(a);
         } else {
-            std::vector<StageSmem<ShallowWater2D, double, TI, TJ>> smv(1);
-            for (int t = 0; t < nti * ntj; t++) {
-                const int i0 = (t / ntj) * TI, j0 = (t % ntj) * TJ;
-                for (int tid = 0; tid < TI * TJ; tid++) phase_load<ShallowWater2D, double, TI, TJ>(tid, TI * TJ, i0, j0, a, smv[0]);
-                double ax = 0, ay = 0;
-                for (int tid = 0; tid < TI * TJ; tid++) phase_faces<ShallowWater2D, double, WENO_JS5_P2, false, TI, TJ>(tid, TI * TJ, i0, j0, a, smv[0], ax, ay);
-                for (int tid = 0; tid < TI * TJ; tid++) phase_update<ShallowWater2D, double, TI, TJ>(tid, i0, j0, a, smv[0]);
-            }
+            run_tiles<ShallowWater2D, double, WENO_JS5_P2, false>(a);
         }
     };
     for (int n = 0; n < nsteps; n++) {
@@ -217,12 +205,12 @@ int emu_run_fixed_steps(const finess_params* p, double* q, double dt, int nsteps
             la.q = buf[A].data(); la.F = Fo.data(); la.G = Go.data(); la.mx = p->mx; la.my = p->my; la.plane = plane;
             for (int k = 0; k < 4; k++) la.bc[k] = p->bc[k];
             la.time_order = p->time_order; la.dt = dt; la.dx = p->dx; la.dy = p->dy; la.pp.gamma = p->gamma;
-            const int nti = (p->mx + 6 + TI - 1) / TI, ntj = (p->my + 6 + TJ - 1) / TJ;
-            std::vector<LxwSmem<TI, TJ>> smv(1);
+            const int nti = (p->mx + 6 + LTI - 1) / LTI, ntj = (p->my + 6 + LTJ - 1) / LTJ;
+            std::vector<LxwSmem<LTI, LTJ>> smv(1);
             for (int t = 0; t < nti * ntj; t++) {
-                const int e0i = (t / ntj) * TI, e0j = (t % ntj) * TJ;
-                for (int tid = 0; tid < TI * TJ; tid++) lxw_phase_load<Euler2D, TI, TJ>(tid, TI * TJ, e0i, e0j, la, smv[0]);
-                for (int tid = 0; tid < TI * TJ; tid++) lxw_phase_cell<Euler2D, TI, TJ>(tid, e0i, e0j, la, smv[0]);
+                const int e0i = (t / ntj) * LTI, e0j = (t % ntj) * LTJ;
+                for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_load<Euler2D, LTI, LTJ>(tid, LTI * LTJ, e0i, e0j, la, smv[0]);
+                for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_cell<Euler2D, LTI, LTJ>(tid, e0i, e0j, la, smv[0]);
             }
             stage(A, A, -1, B, 1.0, 0.0, 1.0);
             cur = B;

@@ -112,6 +112,8 @@ class Stats(ctypes.Structure):
 
 _lib = None
 _PD = ctypes.POINTER(ctypes.c_double)
+_PF = ctypes.POINTER(ctypes.c_float)
+MLP_NPARAM = 1889
 
 # every symbol include/finess_hip.h declares, with its ctypes signature
 SIGNATURES = {
@@ -133,6 +135,20 @@ SIGNATURES = {
     "finess_last_kernel_time": (ctypes.c_int, [ctypes.c_void_p, _PD, ctypes.POINTER(ctypes.c_longlong)]),
     "finess_version": (ctypes.c_char_p, []),
     "finess_device_count": (ctypes.c_int, []),
+    "finess_set_indicator": (ctypes.c_int, [ctypes.c_void_p, _PF, ctypes.c_float]),
+    "finess_route_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                          ctypes.POINTER(ctypes.c_longlong)]),
+    "finess_mlp_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]),
+    "finess_mlp_destroy": (None, [ctypes.c_void_p]),
+    "finess_mlp_harvest": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_longlong)]),
+    "finess_mlp_get_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, _PF, _PF, _PF]),
+    "finess_mlp_set_data": (ctypes.c_int, [ctypes.c_void_p, _PF, _PF, ctypes.c_longlong]),
+    "finess_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _PF]),
+    "finess_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _PF]),
+    "finess_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _PF]),
+    "finess_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, _PF, _PF, ctypes.c_int, _PF, _PF]),
+    "finess_mlp_eval": (ctypes.c_int, [ctypes.c_void_p, _PF, ctypes.c_longlong, ctypes.c_int, _PF]),
 }
 
 
@@ -177,6 +193,13 @@ def _ptr(a):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_PD)
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PF)
 
 
 class Solver:
@@ -287,3 +310,95 @@ class Solver:
         n = ctypes.c_longlong()
         self._check(self.lib.finess_last_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)), "last_kernel_time")
         return ms.value, n.value
+
+    # ---- troubled-interface indicator ------------------------------------
+    def set_indicator(self, params, threshold):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.shape == (MLP_NPARAM,)
+        self._check(self.lib.finess_set_indicator(self.h, _fptr(params), float(threshold)), "set_indicator")
+
+    def route_stats(self):
+        t = ctypes.c_longlong()
+        n = ctypes.c_longlong()
+        self._check(self.lib.finess_route_stats(self.h, ctypes.byref(t), ctypes.byref(n)), "route_stats")
+        return t.value, n.value
+
+
+class Mlp:
+    """The 24-32-32-1 troubled-interface network (``finess_mlp``): a device
+    training set harvested from solver states, Adam training and evaluation
+    on the matrix cores."""
+
+    def __init__(self, seed=1, capacity=1 << 22):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.finess_mlp_create(int(seed), int(capacity), ctypes.byref(h))
+        if rc:
+            raise FinessError(rc, "finess_mlp_create")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.finess_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise FinessError(rc, what)
+
+    def harvest(self, solver, stride=1, tau=0.05):
+        n = ctypes.c_longlong()
+        self._check(self.lib.finess_mlp_harvest(self.h, solver.h, int(stride), float(tau), ctypes.byref(n)), "harvest")
+        self.n = n.value
+        return self.n
+
+    def get_data(self, first=0, n=None):
+        n = self.n - first if n is None else n
+        X = np.zeros((n, 24), np.float32)
+        y = np.zeros(n, np.float32)
+        d = np.zeros(n, np.float32)
+        self._check(self.lib.finess_mlp_get_data(self.h, int(first), int(n), _fptr(X), _fptr(y), _fptr(d)), "get_data")
+        return X, y, d
+
+    def set_data(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        self._check(self.lib.finess_mlp_set_data(self.h, _fptr(X), _fptr(y), len(y)), "set_data")
+        self.n = len(y)
+
+    def train(self, epochs=4, batch=4096, lr=2e-3):
+        loss = np.zeros(epochs, np.float32)
+        self._check(self.lib.finess_mlp_train(self.h, int(epochs), int(batch), float(lr), _fptr(loss)), "train")
+        return loss
+
+    @property
+    def params(self):
+        P = np.zeros(MLP_NPARAM, np.float32)
+        self._check(self.lib.finess_mlp_get_params(self.h, _fptr(P)), "get_params")
+        return P
+
+    @params.setter
+    def params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        self._check(self.lib.finess_mlp_set_params(self.h, _fptr(P)), "set_params")
+
+    def gradient(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.finess_mlp_gradient(self.h, _fptr(X), _fptr(y), len(y), _fptr(g), _fptr(loss)), "gradient")
+        return g, float(loss[0])
+
+    def logits(self, X, path=0):
+        X = np.ascontiguousarray(X, np.float32)
+        z = np.zeros(len(X), np.float32)
+        self._check(self.lib.finess_mlp_eval(self.h, _fptr(X), len(X), int(path), _fptr(z)), "eval")
+        return z

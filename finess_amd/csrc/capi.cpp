@@ -28,6 +28,7 @@
 #include "../../include/finess_hip.h"
 #include "kernels.h"
 #include "lxw_body.h"
+#include "mlp.h"
 #include "stage1d_body.h"
 #include "stage2d_body.h"
 
@@ -50,6 +51,8 @@ struct finess_ctx {
     double* fhat1d = nullptr;
     unsigned long long* smax = nullptr;    // device CFL slots [8][2]
     unsigned long long* smax_h = nullptr;  // pinned copy
+    MlpFrags* frags = nullptr;                    // device copy of the installed network
+    unsigned long long* route_counts = nullptr;   // device [2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double t = 0.0;
     int cur = 0;
@@ -137,6 +140,20 @@ Stage1DArgs base_stage1d_args(finess_ctx* c)
     return a;
 }
 
+int stage_routed(finess_ctx* c, StageArgs& a)
+{
+    const finess_params& p = c->p;
+    if (p.time_stepping_method != FINESS_TS_RK) return fail(c, FINESS_ERR_CONFIG, "routing needs the RK stepper");
+    if (p.routing == FINESS_ROUTE_LEARNED && !c->frags)
+        return fail(c, FINESS_ERR_CONFIG, "routing = learned but no indicator installed (finess_set_indicator)");
+    a.frags = c->frags;
+    a.route_exact = p.routing == FINESS_ROUTE_ORACLE;
+    a.route_tau = p.route_tau;
+    a.route_counts = c->route_counts;
+    const int rc = launch_stage2d_routed(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), a, c->st);
+    return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "routed stage launch failed") : 0;
+}
+
 // One stage: out = a1*qa + a2*qb + (beta*dt) L(src).  slot: CFL slot.
 int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2, double beta, double dt, int slot)
 {
@@ -172,6 +189,7 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.lxw_F = c->lxwF;
         a.lxw_G = c->lxwG;
     }
+    if (p.routing != FINESS_ROUTE_NONE) return stage_routed(c, a);
     const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), lxw, a, c->st);
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
 }
@@ -378,6 +396,10 @@ int finess_params_finalize(finess_params* p)
         return FINESS_ERR_CONFIG;
     }
     if (p->precision == FINESS_PREC_FP32 && weno_kind(*p) != WENO_JS5_P2) return FINESS_ERR_CONFIG;
+    if (p->routing != FINESS_ROUTE_NONE &&
+        (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || weno_kind(*p) != WENO_JS5_P2 ||
+         p->time_stepping_method != FINESS_TS_RK))
+        return FINESS_ERR_CONFIG;
     for (int k = 0; k < 4; k++)
         if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_REFLECT) return FINESS_ERR_CONFIG;
     return FINESS_OK;
@@ -430,6 +452,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         rc = alloc(c, (void**)&c->lxwF, lb);
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
     }
+    if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * sizeof(unsigned long long));
+    if (!rc) rc = launch_fill_u64(c->route_counts, 2, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (!rc) rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (rc) {
         finess_destroy(c);
@@ -453,6 +477,8 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->faces_y);
     hipFree(c->fhat1d);
     hipFree(c->smax);
+    hipFree(c->frags);
+    hipFree(c->route_counts);
     if (c->smax_h) hipHostFree(c->smax_h);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -747,6 +773,37 @@ int finess_run_fixed_steps(finess_ctx* c, double dt, int nsteps, double* last_cf
     }
     return FINESS_OK;
 }
+
+int finess_set_indicator(finess_ctx* c, const float* params, float threshold)
+{
+    if (!c || !params) return FINESS_ERR_ARG;
+    MlpFrags h;
+    mlp_make_frags(params, threshold, &h);
+    if (!c->frags) {
+        int rc;
+        if ((rc = alloc(c, (void**)&c->frags, sizeof(MlpFrags)))) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->frags, &h, sizeof h, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FINESS_OK;
+}
+
+int finess_route_stats(finess_ctx* c, long long* troubled, long long* faces)
+{
+    if (!c) return FINESS_ERR_ARG;
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, c->route_counts, sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (launch_fill_u64(c->route_counts, 2, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "reset failed");
+    if (troubled) *troubled = (long long)h[0];
+    if (faces) *faces = (long long)h[1];
+    return FINESS_OK;
+}
+
+// Accessors for the network object (mlp_api.cpp).
+hipStream_t finess_ctx_stream(finess_ctx* c) { return c->st; }
+const double* finess_ctx_state(finess_ctx* c) { return c->have_state ? c->buf[c->cur] : nullptr; }
+const finess_params* finess_ctx_params(finess_ctx* c) { return &c->p; }
 
 int finess_sync(finess_ctx* c)
 {

@@ -18,6 +18,18 @@ int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
 // Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].
 int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st);
 
+int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st);
+
+struct HarvestArgs;
+struct MlpFrags;
+int launch_harvest(const HarvestArgs& a, hipStream_t st);
+int launch_mlp_eval(const float* X, long long n, const MlpFrags* fr, float* logits, hipStream_t st);
+int mlp_forward(const float* params, const float* X, int B, float* H1, float* H2, float* z, hipStream_t st);
+int mlp_backward(const float* params, const float* X, const float* y, int B, float* H1, float* H2, float* z,
+                 float* dz, float* dH2, float* dH1, float* grad, float* loss_sum, hipStream_t st);
+int mlp_adam(float* p, float* g, float* m, float* v, float lr, int t, hipStream_t st);
+int mlp_zero(float* p, long long n, hipStream_t st);
+
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st);
 int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],

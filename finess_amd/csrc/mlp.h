@@ -1,0 +1,68 @@
+// mlp.h -- BF16 matrix-core building blocks of the troubled-interface MLP
+// (24 -> 32 -> 32 -> 1, ReLU, logistic output): operand rounding and the
+// 16x16x16 MFMA tile with FP32 accumulation.  No counterpart in the
+// reference.
+//
+// Lane layout of v_mfma_f32_16x16x16_bf16 (wave64), used everywhere below:
+//   A (16 x 16, M x K): lane l holds A[l % 16][4*(l/16) + v], v = 0..3
+//   B (16 x 16, K x N): lane l holds B[4*(l/16) + v][l % 16]
+//   D (16 x 16, M x N): lane l holds D[4*(l/16) + v][l % 16]
+// so a D tile is, lane for lane, the B operand of a following product whose
+// K index is D's row index.  The inference inside the stage kernel computes
+// the transposed network H^T = W^T X^T for that reason: each layer's
+// accumulators feed the next layer's MFMA without moving data between lanes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace finess {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__host__ __device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float x;
+    __builtin_memcpy(&x, &u, 4);
+    return x;
+}
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
+{
+    bf16x4 r;
+    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    return r;
+}
+
+// Weight fragments of the transposed network for the in-kernel inference,
+// prepared on the host from the FP32 parameter vector (finess_mlp layout):
+//   w1[mb][kb][lane][4]  W1^T rows 16mb.., cols (inputs) 16kb..  (2 x 2)
+//   w2[mb][kb][lane][4]  W2^T                                      (2 x 2)
+//   w3[kb][lane][4]      w3^T in row 0 of a 16-row tile            (2)
+//   b1[32], b2[32], b3, threshold on the logit (troubled if above)
+struct MlpFrags {
+    uint16_t w1[2][2][64][4];
+    uint16_t w2[2][2][64][4];
+    uint16_t w3[2][64][4];
+    float b1[32], b2[32];
+    float b3;
+    float threshold;
+};
+
+void mlp_make_frags(const float* params, float threshold, MlpFrags* f);
+
+} // namespace finess

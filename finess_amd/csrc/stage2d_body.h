@@ -75,6 +75,12 @@ struct StageArgs {
     PhysParams pp;
     // wave-speed maxima (bit patterns of non-negative doubles), [2] per slot
     unsigned long long* smax;
+    // troubled-interface routing (routed kernel only): network fragments,
+    // or the exact weight-departure test when route_exact is set
+    const struct MlpFrags* frags;
+    int route_exact;
+    double route_tau;
+    unsigned long long* route_counts;  // [0] troubled faces, [1] faces
     // optional face-flux / L output for the drop-in ConstructL (may be null):
     double* Lout;          // L planes (mx*my)
     double* smax_faces_x;  // (mx+1) x my alphas of x-faces, row-major i*(my)+j
@@ -198,38 +204,54 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
 // (t / (TJ+1), t % (TJ+1))).  Writes the face flux to sm.face and folds its
 // wave speed into amax when GetCFL counts it (faces mx+1/2 and my+1/2 are
 // skipped, lib/2d/GetCFL.cpp:54-58).
+// Geometry of face number f of direction dir in the tile (see phase_faces).
+struct FaceGeom {
+    int fi, fj, base, stride;
+    long long lbase, lstride;
+    bool valid, count;
+};
+
+template <int TI, int TJ>
+__host__ __device__ __forceinline__ FaceGeom face_geom(int f, int dir, int i0, int j0, const StageArgs& a)
+{
+    constexpr int HJ = TJ + 6;
+    const long long lld = a.my + 6;
+    FaceGeom g;
+    if (dir == 0) {
+        g.fi = f / TJ; g.fj = f - g.fi * TJ;
+        g.base = g.fi * HJ + (g.fj + 3); g.stride = HJ;
+        const int gi = i0 + g.fi, gj = j0 + g.fj;
+        g.valid = f < (TI + 1) * TJ && gi <= a.mx && gj < a.my;
+        g.count = gi < a.mx;
+        g.lbase = (long long)gi * lld + (gj + 3); g.lstride = lld;
+    } else {
+        g.fi = f / (TJ + 1); g.fj = f - g.fi * (TJ + 1);
+        g.base = (g.fi + 3) * HJ + g.fj; g.stride = 1;
+        const int gi = i0 + g.fi, gj = j0 + g.fj;
+        g.valid = f < TI * (TJ + 1) && gi < a.mx && gj <= a.my;
+        g.count = gj < a.my;
+        g.lbase = (long long)(gi + 3) * lld + gj; g.lstride = 1;
+    }
+    return g;
+}
+
 template <class P, typename R, int WK, bool LXW, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, int j0, const StageArgs& a,
                                                      StageSmem<P, R, TI, TJ>& sm, double& amax)
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
-    const long long lld = a.my + 6, lpl = (long long)(a.mx + 6) * lld;
-    int fi, fj, base, stride;
-    long long lbase, lstride;
-    bool valid, count;
-    if (dir == 0) {
-        if (tid >= S::NFX) return;
-        fi = tid / TJ; fj = tid - fi * TJ;
-        base = fi * S::HJ + (fj + 3); stride = S::HJ;
-        const int gi = i0 + fi, gj = j0 + fj;
-        valid = gi <= a.mx && gj < a.my;
-        count = gi < a.mx;
-        lbase = (long long)gi * lld + (gj + 3); lstride = lld;
-    } else {
-        if (tid >= S::NFY) return;
-        fi = tid / (TJ + 1); fj = tid - fi * (TJ + 1);
-        base = (fi + 3) * S::HJ + fj; stride = 1;
-        const int gi = i0 + fi, gj = j0 + fj;
-        valid = gi < a.mx && gj <= a.my;
-        count = gj < a.my;
-        lbase = (long long)(gi + 3) * lld + gj; lstride = 1;
-    }
-    if (!valid) return;
+    const long long lpl = (long long)(a.mx + 6) * (a.my + 6);
+    const FaceGeom g = face_geom<TI, TJ>(tid, dir, i0, j0, a);
+    if (!g.valid) return;
+    const int fi = g.fi, fj = g.fj, base = g.base, stride = g.stride;
+    const long long lbase = g.lbase, lstride = g.lstride;
+    const bool count = g.count;
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
     const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F, lbase,
                                                     lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh);
+    (void)sizeof(S);
     for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
     if (count) amax = rmax(amax, double(alpha));
     if (dir == 0 && a.smax_faces_x) a.smax_faces_x[(long long)(i0 + fi) * a.my + (j0 + fj)] = double(alpha);

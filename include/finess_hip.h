@@ -178,6 +178,38 @@ int finess_sync(finess_ctx* ctx);
  * context's stream, and the number of stage launches it covered. */
 int finess_last_kernel_time(finess_ctx* ctx, double* ms, long long* launches);
 
+/* ---- troubled-interface indicator (no counterpart in the reference) ------ */
+
+/* Installs the network (finess_mlp parameter vector, FINESS_MLP_NPARAM
+ * floats) on a context whose routing is FINESS_ROUTE_LEARNED: faces whose
+ * logit exceeds `threshold` take characteristic WENO5, the others the
+ * component-wise linear fifth-order flux. */
+int finess_set_indicator(finess_ctx* ctx, const float* params, float threshold);
+/* Troubled and total face counts accumulated since the last call (and reset). */
+int finess_route_stats(finess_ctx* ctx, long long* troubled, long long* faces);
+
+#define FINESS_MLP_NPARAM 1889
+typedef struct finess_mlp finess_mlp;
+
+/* A 24-32-32-1 network with FP32 master parameters (He-initialised from
+ * `seed`) and a device training set of up to `capacity` interfaces. */
+int  finess_mlp_create(unsigned long long seed, long long capacity, finess_mlp** out);
+void finess_mlp_destroy(finess_mlp* mlp);
+/* Appends every `stride`-th face of the context's current 2D Euler state:
+ * features and the label [max weight departure > tau]. *count = set size. */
+int  finess_mlp_harvest(finess_mlp* mlp, finess_ctx* ctx, int stride, double tau, long long* count);
+/* Copies up to n samples of the set (X: n x 24, y: n, departure: n; any may be NULL). */
+int  finess_mlp_get_data(finess_mlp* mlp, long long first, long long n, float* X, float* y, float* departure);
+int  finess_mlp_set_data(finess_mlp* mlp, const float* X, const float* y, long long n);
+/* Adam on the mean binary cross-entropy; loss[e] receives epoch e's mean loss. */
+int  finess_mlp_train(finess_mlp* mlp, int epochs, int batch, float lr, float* loss);
+int  finess_mlp_get_params(finess_mlp* mlp, float* params);
+int  finess_mlp_set_params(finess_mlp* mlp, const float* params);
+/* Gradient of the mean BCE loss over (X, y) (n <= 65536); *loss = mean loss. */
+int  finess_mlp_gradient(finess_mlp* mlp, const float* X, const float* y, int n, float* grad, float* loss);
+/* Logits of n interfaces: layer GEMMs (path 0) or the in-kernel wave routine (path 1). */
+int  finess_mlp_eval(finess_mlp* mlp, const float* X, long long n, int path, float* logits);
+
 /* ---- library info --------------------------------------------------------- */
 const char* finess_version(void);
 int finess_device_count(void);

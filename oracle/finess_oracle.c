@@ -22,6 +22,7 @@
 #include "finess_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -976,6 +977,236 @@ int orc_construct_integrated_r(const finess_params* p, double dt, const double* 
 #undef RIX
     free(R);
     return FINESS_OK;
+}
+
+/* ====================================================================== */
+/* Troubled-interface indicator (new functionality, no reference code)      */
+/* ====================================================================== */
+
+/* Rotated primitives (rho, u_n, u_t, p) of an Euler state. */
+static void euler_prims(const finess_params* p, int ixy, const double* q, double* v)
+{
+    const double rho = q[0];
+    const double un = (ixy == 1 ? q[1] : q[2]) / rho, ut = (ixy == 1 ? q[2] : q[1]) / rho, uw = q[3] / rho;
+    v[0] = rho;
+    v[1] = un;
+    v[2] = ut;
+    v[3] = (p->gamma - 1.0) * (q[4] - 0.5 * rho * (un * un + ut * ut + uw * uw));
+}
+
+void orc_face_features(const finess_params* p, int ixy, const double* qs, float* feat)
+{
+    double v[6][4], mean[4], lo[4], hi[4];
+    for (int s = 0; s < 6; s++) euler_prims(p, ixy, qs + s * p->meqn, v[s]);
+    for (int k = 0; k < 4; k++) {
+        mean[k] = 0.0;
+        lo[k] = hi[k] = v[0][k];
+        for (int s = 0; s < 6; s++) {
+            mean[k] += v[s][k];
+            if (v[s][k] < lo[k]) lo[k] = v[s][k];
+            if (v[s][k] > hi[k]) hi[k] = v[s][k];
+        }
+        mean[k] /= 6.0;
+    }
+    const double cref = sqrt(fabs(p->gamma * mean[3] / mean[0]));
+    const double ref[4] = {fabs(mean[0]), cref, cref, fabs(mean[3])};
+    for (int k = 0; k < 4; k++)
+        for (int s = 0; s < 6; s++) feat[s * 4 + k] = (float)((v[s][k] - mean[k]) / (hi[k] - lo[k] + 0.01 * ref[k] + 1e-300));
+}
+
+/* The reference's split characteristic stencils (lib/2d/ConstructL.cpp:
+ * 72-185) and the JS weights of each (lib/WenoReconstruct.cpp:131-134). */
+double orc_face_departure(const finess_params* p, int ixy, const double* qs)
+{
+    const int meqn = p->meqn;
+    double Qavg[MAXEQ], qv[MAXEQ * 6], fv[MAXEQ * 6], wv[MAXEQ * 6], gv[MAXEQ * 6], F[MAXEQ], G[MAXEQ], aux[1] = {0};
+    for (int m = 0; m < meqn; m++) Qavg[m] = 0.5 * (qs[3 * meqn + m] + qs[2 * meqn + m]);
+    for (int s = 0; s < 6; s++) {
+        orc_flux(p, qs + s * meqn, aux, F, G);
+        for (int m = 0; m < meqn; m++) {
+            qv[m * 6 + s] = qs[s * meqn + m];
+            fv[m * 6 + s] = ixy == 1 ? F[m] : G[m];
+        }
+    }
+    orc_project_left(p, ixy, Qavg, 6, qv, wv);
+    orc_project_left(p, ixy, Qavg, 6, fv, gv);
+    double s1, s2;
+    orc_wave_speed(p, ixy, qs + 2 * meqn, qs + 3 * meqn, aux, aux, &s1, &s2);
+    const double la = p->alpha_scaling * Max(fabs(s1), fabs(s2));
+    const double d[3] = {0.1, 0.6, 0.3};
+    double dep = 0.0;
+    for (int m = 0; m < meqn; m++) {
+        double gp[5], gm[5], w[3];
+        for (int s = 1; s <= 5; s++) {
+            gp[s - 1] = 0.5 * (gv[m * 6 + s - 1] + la * wv[m * 6 + s - 1]);
+            gm[s - 1] = 0.5 * (gv[m * 6 + 6 - s] - la * wv[m * 6 + 6 - s]);
+        }
+        orc_weno5_js_weights(p->epsilon, 2.0, gp, w);
+        for (int j = 0; j < 3; j++) dep = Max(dep, fabs(w[j] - d[j]));
+        orc_weno5_js_weights(p->epsilon, 2.0, gm, w);
+        for (int j = 0; j < 3; j++) dep = Max(dep, fabs(w[j] - d[j]));
+    }
+    return dep;
+}
+
+/* Component-wise LLF with the linear fifth-order reconstruction
+ * (WenoReconstruct_FD5 on the conserved components, identity projection). */
+static double interface_flux_linear(const finess_params* p, int ixy, const double qs[6][MAXEQ],
+                                    const double fs[6][MAXEQ], double* fhat)
+{
+    double s1, s2, aux[1] = {0};
+    orc_wave_speed(p, ixy, qs[2], qs[3], aux, aux, &s1, &s2);
+    const double alpha = Max(0.0, Max(fabs(s1), fabs(s2)));
+    const double la = p->alpha_scaling * alpha;
+    for (int m = 0; m < p->meqn; m++) {
+        double gp[5], gm[5];
+        for (int s = 1; s <= 5; s++) {
+            gp[s - 1] = 0.5 * (fs[s - 1][m] + la * qs[s - 1][m]);
+            gm[s - 1] = 0.5 * (fs[6 - s][m] - la * qs[6 - s][m]);
+        }
+        fhat[m] = orc_weno5(FINESS_WENO_FD, p->epsilon, p->power_param, gp) +
+                  orc_weno5(FINESS_WENO_FD, p->epsilon, p->power_param, gm);
+    }
+    return alpha;
+}
+
+int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
+                           double* Lstar, double* smax, long long* n_troubled)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
+    double* Fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn, sizeof(double));
+    double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
+    long long nt = 0;
+    for (int dir = 1; dir <= 2; dir++) {
+        const int imax = dir == 1 ? mx + 1 : mx, jmax = dir == 1 ? my : my + 1;
+#pragma omp parallel for schedule(static) reduction(+ : nt)
+        for (int i = 1; i <= imax; i++)
+            for (int j = 1; j <= jmax; j++) {
+                double qs[6][MAXEQ], fs[6][MAXEQ], F[MAXEQ], G[MAXEQ], fh[MAXEQ];
+                for (int s = 1; s <= 6; s++) {
+                    const int is = dir == 1 ? i - 1 + s - 3 : i, js = dir == 1 ? j : j - 1 + s - 3;
+                    for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix3(is, js, m, my, meqn, mbc)];
+                    orc_flux(p, qs[s - 1], &aux[ix3(is, js, 1, my, na, mbc)], F, G);
+                    for (int m = 0; m < meqn; m++) fs[s - 1][m] = dir == 1 ? F[m] : G[m];
+                }
+                int full = mode == 0;
+                if (mode == 2) full = orc_face_departure(p, dir, &qs[0][0]) > tau;
+                nt += full;
+                double alpha;
+                if (full) {
+                    const int il = dir == 1 ? i - 1 : i, jl = dir == 1 ? j : j - 1;
+                    alpha = interface_flux(p, dir, qs, fs, &aux[ix3(il, jl, 1, my, na, mbc)],
+                                           &aux[ix3(i, j, 1, my, na, mbc)], NULL, fh);
+                } else {
+                    alpha = interface_flux_linear(p, dir, qs, fs, fh);
+                }
+                double* sm = &smax[ix3(i, j, dir, my + 1, 2, mbc)];
+                *sm = Max(*sm, alpha);
+                for (int m = 1; m <= meqn; m++) {
+                    if (dir == 1) Fhat[ix3(i, j, m, my, meqn, mbc)] = fh[m - 1];
+                    else Ghat[ix3(i, j, m, my + 1, meqn, mbc)] = fh[m - 1];
+                }
+            }
+    }
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my; j++)
+            for (int m = 1; m <= meqn; m++) {
+                double tmp = -(Fhat[ix3(i + 1, j, m, my, meqn, mbc)] - Fhat[ix3(i, j, m, my, meqn, mbc)]) / p->dx;
+                tmp = tmp - (Ghat[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Ghat[ix3(i, j, m, my + 1, meqn, mbc)]) / p->dy;
+                Lstar[ix3(i, j, m, my, meqn, mbc)] = tmp;
+            }
+    free(Fhat);
+    free(Ghat);
+    if (n_troubled) *n_troubled = nt;
+    return FINESS_OK;
+}
+
+/* ---- MLP with BF16 operand rounding ------------------------------------- */
+#define NIN 24
+#define NH 32
+#define OFF_W1 0
+#define OFF_B1 (OFF_W1 + NIN * NH)
+#define OFF_W2 (OFF_B1 + NH)
+#define OFF_B2 (OFF_W2 + NH * NH)
+#define OFF_W3 (OFF_B2 + NH)
+#define OFF_B3 (OFF_W3 + NH)
+#define NPARAM (OFF_B3 + 1)
+
+float orc_bf16(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+static int g_round_bf16 = 1;
+void orc_mlp_set_rounding(int on) { g_round_bf16 = on; }
+static double rb(double x) { return g_round_bf16 ? (double)orc_bf16((float)x) : x; }
+
+/* One sample: activations in FP32 as the GPU stores them. */
+static void mlp_sample(const float* P, const float* x, float* h1, float* h2, float* z)
+{
+    for (int n = 0; n < NH; n++) {
+        double a = 0.0;
+        for (int k = 0; k < NIN; k++) a += rb(x[k]) * rb(P[OFF_W1 + k * NH + n]);
+        const float v = (float)a + P[OFF_B1 + n];
+        h1[n] = v > 0.f ? v : 0.f;
+    }
+    for (int n = 0; n < NH; n++) {
+        double a = 0.0;
+        for (int k = 0; k < NH; k++) a += rb(h1[k]) * rb(P[OFF_W2 + k * NH + n]);
+        const float v = (float)a + P[OFF_B2 + n];
+        h2[n] = v > 0.f ? v : 0.f;
+    }
+    double a = 0.0;
+    for (int k = 0; k < NH; k++) a += rb(h2[k]) * rb(P[OFF_W3 + k]);
+    *z = (float)a + P[OFF_B3];
+}
+
+void orc_mlp_forward(const float* P, const float* X, int n, float* logits)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        float h1[NH], h2[NH];
+        mlp_sample(P, X + (size_t)i * NIN, h1, h2, &logits[i]);
+    }
+}
+
+double orc_mlp_gradient(const float* P, const float* X, const float* y, int n, double* grad)
+{
+    memset(grad, 0, NPARAM * sizeof(double));
+    double loss = 0.0;
+    for (int i = 0; i < n; i++) {
+        const float* x = X + (size_t)i * NIN;
+        float h1[NH], h2[NH], z;
+        mlp_sample(P, x, h1, h2, &z);
+        const double s = 1.0 / (1.0 + exp(-(double)z));
+        loss += fmax((double)z, 0.0) - z * (double)y[i] + log1p(exp(-fabs((double)z)));
+        const float dz = (float)((s - y[i]) / n);
+        /* gW3 = H2^T dz, gb3 = sum dz */
+        for (int k = 0; k < NH; k++) grad[OFF_W3 + k] += rb(h2[k]) * rb(dz);
+        grad[OFF_B3] += dz;
+        /* dH2 = dz w3^T . [H2 > 0] */
+        float dh2[NH], dh1[NH];
+        for (int k = 0; k < NH; k++) dh2[k] = h2[k] > 0.f ? (float)(rb(dz) * rb(P[OFF_W3 + k])) : 0.f;
+        for (int k = 0; k < NH; k++)
+            for (int m = 0; m < NH; m++) grad[OFF_W2 + k * NH + m] += rb(h1[k]) * rb(dh2[m]);
+        for (int m = 0; m < NH; m++) grad[OFF_B2 + m] += dh2[m];
+        /* dH1 = dH2 W2^T . [H1 > 0] */
+        for (int k = 0; k < NH; k++) {
+            double a = 0.0;
+            for (int m = 0; m < NH; m++) a += rb(dh2[m]) * rb(P[OFF_W2 + k * NH + m]);
+            dh1[k] = h1[k] > 0.f ? (float)a : 0.f;
+        }
+        for (int k = 0; k < NIN; k++)
+            for (int m = 0; m < NH; m++) grad[OFF_W1 + k * NH + m] += rb(x[k]) * rb(dh1[m]);
+        for (int m = 0; m < NH; m++) grad[OFF_B1 + m] += dh1[m];
+    }
+    return loss / n;
 }
 
 int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux, const double* F,

@@ -67,6 +67,25 @@ int orc_construct_integrated_r(const finess_params* p, double dt, const double* 
 int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux,
                         const double* F, const double* G, double* Lstar, double* smax);
 
+/* ---- troubled-interface indicator (no counterpart in the reference) ----
+ * qs: the 6 stencil states q(i-3..i+2) across a face, [6][meqn]. */
+void   orc_face_features(const finess_params* p, int ixy, const double* qs, float* feat /* [24] */);
+double orc_face_departure(const finess_params* p, int ixy, const double* qs);
+/* ConstructL with routing: mode 0 characteristic WENO everywhere, 1 the
+ * component-wise linear fifth-order flux everywhere, 2 characteristic WENO
+ * where orc_face_departure > tau else linear.  *n_troubled counts faces. */
+int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
+                           double* Lstar, double* smax, long long* n_troubled);
+/* MLP 24-32-32-1 (parameter layout of include/finess_hip.h's finess_mlp),
+ * every matrix product with both operands rounded to BF16 as the matrix cores
+ * do, accumulated in FP64. */
+float orc_bf16(float x);
+/* 0: exact products (for finite-difference checks of the gradient). */
+void  orc_mlp_set_rounding(int on);
+void  orc_mlp_forward(const float* params, const float* X, int n, float* logits);
+/* Gradient of the mean binary cross-entropy; returns the mean loss. */
+double orc_mlp_gradient(const float* params, const float* X, const float* y, int n, double* grad);
+
 /* Number of OpenMP threads the oracle runs with. */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

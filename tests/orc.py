@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
 EMU = os.path.join(ROOT, "tests", "emu", "libfiness_emu.so")
 PD = ctypes.POINTER(ctypes.c_double)
+PF = ctypes.POINTER(ctypes.c_float)
 PP = ctypes.POINTER(Params)
 
 
@@ -60,6 +61,14 @@ def oracle():
             "orc_construct_integrated_r": (ctypes.c_int, [PP, ctypes.c_double, PD, PD, PD, PD]),
             "orc_construct_lxw_l": (ctypes.c_int, [PP, PD, PD, PD, PD, PD, PD]),
             "orc_num_threads": (ctypes.c_int, []),
+            "orc_face_features": (None, [PP, ctypes.c_int, PD, PF]),
+            "orc_face_departure": (ctypes.c_double, [PP, ctypes.c_int, PD]),
+            "orc_construct_l_routed": (ctypes.c_int, [PP, PD, PD, ctypes.c_int, ctypes.c_double, PD, PD,
+                                                      ctypes.POINTER(ctypes.c_longlong)]),
+            "orc_bf16": (ctypes.c_float, [ctypes.c_float]),
+            "orc_mlp_set_rounding": (None, [ctypes.c_int]),
+            "orc_mlp_forward": (None, [PF, PF, ctypes.c_int, PF]),
+            "orc_mlp_gradient": (ctypes.c_double, [PF, PF, PF, ctypes.c_int, PD]),
             "orc_set_num_threads": (None, [ctypes.c_int]),
         }
         for n, (r, a) in sig.items():
@@ -168,3 +177,51 @@ def rel_linf(a, b):
 def emu_fixed_steps(p, q, dt, n):
     assert emu().emu_run_fixed_steps(ctypes.byref(p), ptr(q), float(dt), int(n)) == 0
     return q
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PF)
+
+
+def construct_l_routed(p, q, mode, tau=0.05):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    aux = np.zeros(p.shape_aux())
+    L = np.zeros(p.shape_q())
+    sm = np.zeros(p.shape_smax())
+    nt = ctypes.c_longlong()
+    assert oracle().orc_construct_l_routed(ctypes.byref(p), ptr(q), ptr(aux), int(mode), float(tau), ptr(L), ptr(sm),
+                                           ctypes.byref(nt)) == 0
+    return L, sm, nt.value
+
+
+def face_stencil(p, q, dir_, i, j):
+    """States q(i-3..i+2) across x-face (i-1/2, j) or y-face (i, j-1/2), 1-based."""
+    g = p.mbc
+    if dir_ == 1:
+        return np.ascontiguousarray(q[g + i - 4:g + i + 2, g + j - 1, :])
+    return np.ascontiguousarray(q[g + i - 1, g + j - 4:g + j + 2, :])
+
+
+def features(p, dir_, qs):
+    f = np.zeros(24, np.float32)
+    oracle().orc_face_features(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)), fptr(f))
+    return f
+
+
+def departure(p, dir_, qs):
+    return oracle().orc_face_departure(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)))
+
+
+def mlp_forward(P, X):
+    X = np.ascontiguousarray(X, np.float32)
+    z = np.zeros(len(X), np.float32)
+    oracle().orc_mlp_forward(fptr(np.ascontiguousarray(P, np.float32)), fptr(X), len(X), fptr(z))
+    return z
+
+
+def mlp_gradient(P, X, y):
+    g = np.zeros(1889)
+    loss = oracle().orc_mlp_gradient(fptr(np.ascontiguousarray(P, np.float32)), fptr(np.ascontiguousarray(X, np.float32)),
+                                     fptr(np.ascontiguousarray(y, np.float32)), len(y), ptr(g))
+    return g, loss

@@ -1,0 +1,76 @@
+"""The example apps, linked against the C++ drop-in operators, run on the GPU
+end to end (parameters.ini -> QinitFunc -> frames of FinSolveRK/FinSolveLxW ->
+q%04d.dat) and their frames match the oracle run with the same parameters."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from finess_amd import problems
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APPS = os.path.join(ROOT, "finess_amd", "compat", "apps")
+
+
+def run_app(name, tmp_path):
+    shutil.copy(os.path.join(APPS, name, "parameters.ini"), tmp_path / "parameters.ini")
+    print("running app", name, flush=True)
+    r = subprocess.run([os.path.join(APPS, name, "finess.exe")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+
+
+def read_frame(path, shape2d=None, meqn=1):
+    v = np.loadtxt(path)
+    t, q = v[0], v[1:]
+    if shape2d is None:
+        return t, q.reshape(meqn, -1).T
+    mx, my = shape2d
+    return t, np.transpose(q.reshape(meqn, my, mx), (2, 1, 0))
+
+
+def oracle_frames(p, q, aux, nout, tfinal, lxw=False):
+    dtv = np.array([0.0, p.initial_dt, p.max_dt])
+    t = 0.0
+    for n in range(1, nout + 1):
+        rc, t, _ = orc.fin_solve(p, q, t, tfinal * n / nout, dtv, aux=aux, lxw=lxw)
+        assert rc == 0
+    return t, q
+
+
+def test_euler_quadrants_app(tmp_path):
+    run_app("euler_quadrants", tmp_path)
+    p, q, aux = problems.riemann4(mx=64, initial_dt=1e-3, max_dt=1.0, nv=500000)
+    t0, q0 = oracle_frames(p, q, aux, 2, 0.02)
+    t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
+    assert abs(t1 - t0) < 1e-15
+    err = orc.rel_linf(q1, orc.interior(p, q0))
+    print("app vs oracle", err)
+    assert np.all(err <= 1e-10)
+
+
+def test_advection_app(tmp_path):
+    run_app("advection_sine", tmp_path)
+    p, q, aux = problems.advection_1d(mx=200, initial_dt=0.002, max_dt=1.0, nv=500000)
+    t0, q0 = oracle_frames(p, q, aux, 1, 1.0)
+    t1, q1 = read_frame(tmp_path / "output" / "q0001.dat", None, 1)
+    assert abs(t1 - t0) < 1e-15
+    err = orc.rel_linf(q1, orc.interior(p, q0))
+    assert np.all(err <= 1e-10), err
+
+
+def test_kelvin_helmholtz_lxw_app(tmp_path):
+    run_app("kelvin_helmholtz", tmp_path)
+    p, q, aux = problems.kelvin_helmholtz(mx=64, seed=20240611, cfl=0.3, max_cfl=0.35, initial_dt=1e-3,
+                                          max_dt=1.0, nv=500000)
+    t0, q0 = oracle_frames(p, q, aux, 2, 0.02, lxw=True)
+    t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
+    assert abs(t1 - t0) < 1e-15
+    err = orc.rel_linf(q1, orc.interior(p, q0))
+    print("LxW app vs oracle", err)
+    assert np.all(err <= 1e-10), err

@@ -1,0 +1,127 @@
+"""GPU tests of the troubled-interface indicator: the matrix-core network
+against the oracle (forward through both the training GEMMs and the wave
+routine the stage kernel uses, gradients), the harvested features and labels
+against the oracle, and the routed stage kernel against the routed oracle."""
+import numpy as np
+import pytest
+
+import orc
+from finess_amd import PREC_FP32, PREC_FP64, ROUTE_ORACLE, Mlp, Solver, problems
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(33)
+
+
+def _rand_params(scale=0.3):
+    return (RNG.normal(size=1889) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_forward_matches_oracle(path):
+    """path 0: the training GEMMs; path 1: the in-kernel wave routine.  Both
+    round operands to BF16 at the same points as the oracle, so only FP32
+    versus FP64 accumulation separates them."""
+    net = Mlp(seed=5, capacity=1024)
+    P = _rand_params()
+    net.params = P
+    X = RNG.normal(size=(5000, 24)).astype(np.float32)
+    z_gpu = net.logits(X, path=path)
+    z_orc = orc.mlp_forward(P, X)
+    d = np.abs(z_gpu - z_orc)
+    print("forward path", path, "max", d.max(), "p99.9", np.quantile(d, 0.999))
+    # an FP32-vs-FP64 sum that lands on the other side of a BF16 rounding
+    # boundary of a hidden activation moves a few logits by ~1e-3
+    assert np.quantile(d, 0.99) <= 1e-4 * (1 + np.abs(z_orc).max())
+    assert d.max() <= 2e-2 * (1 + np.abs(z_orc).max())
+
+
+def test_gradient_matches_oracle():
+    net = Mlp(seed=6, capacity=1024)
+    P = _rand_params()
+    net.params = P
+    X = RNG.normal(size=(3000, 24)).astype(np.float32)
+    y = (RNG.random(3000) > 0.7).astype(np.float32)
+    g_gpu, loss_gpu = net.gradient(X, y)
+    g_orc, loss_orc = orc.mlp_gradient(P, X, y)
+    rel = np.linalg.norm(g_gpu - g_orc) / np.linalg.norm(g_orc)
+    print("gradient rel L2", rel, "loss", loss_gpu, loss_orc)
+    assert abs(loss_gpu - loss_orc) < 1e-5
+    assert rel < 2e-3
+
+
+def test_training_reduces_loss():
+    net = Mlp(seed=7, capacity=1 << 16)
+    X = RNG.normal(size=(1 << 16, 24)).astype(np.float32)
+    y = ((X[:, 0] - X[:, 5] + 0.5 * X[:, 9]) > 0.3).astype(np.float32)   # a learnable rule
+    net.set_data(X, y)
+    loss = net.train(epochs=4, batch=2048, lr=3e-3)
+    acc = np.mean((net.logits(X) > 0) == (y > 0.5))
+    print("loss per epoch", loss, "accuracy", acc)
+    assert loss[-1] < 0.5 * loss[0] and acc > 0.95
+
+
+def test_harvest_matches_oracle():
+    p, q, _ = problems.riemann4(mx=48, my=40)
+    with Solver(p) as s:
+        s.set_state(q)
+        s.run_fixed_steps(2e-3, 5)
+        qs, _ = s.get_state()
+        orc.set_bnd(p, qs)
+        net = Mlp(seed=1, capacity=1 << 16)
+        n = net.harvest(s, stride=1, tau=0.05)
+    X, y, dep = net.get_data()
+    nfx = (p.mx + 1) * p.my
+    assert n == nfx + p.mx * (p.my + 1)
+    rng = np.random.default_rng(0)
+    for k in rng.choice(n, 300, replace=False):
+        if k < nfx:
+            i, j, d = k // p.my + 1, k % p.my + 1, 1
+        else:
+            h = k - nfx
+            i, j, d = h // (p.my + 1) + 1, h % (p.my + 1) + 1, 2
+        st = orc.face_stencil(p, qs, d, i, j)
+        assert np.allclose(X[k], orc.features(p, d, st), atol=2e-6), (k, d)
+        assert abs(dep[k] - orc.departure(p, d, st)) < 1e-6
+    assert 0 < y.sum() < n / 3
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_FP32])
+def test_routed_extremes(prec):
+    """Exact-departure routing with tau < 0 routes every face to
+    characteristic WENO (= the plain kernel), with tau > 1 every face to the
+    linear flux (= the oracle's all-linear ConstructL)."""
+    tol = 1e-12 if prec == PREC_FP64 else 1e-5
+    p, q, _ = problems.riemann4(mx=70, my=64, precision=prec)
+    q[..., 0] *= 1 + 0.01 * RNG.random(q.shape[:-1])
+    orc.set_bnd(p, q)
+    L_full, _, _ = orc.construct_l_routed(p, q, 0)
+    L_lin, _, _ = orc.construct_l_routed(p, q, 1)
+    for tau, ref in ((-1.0, L_full), (2.0, L_lin)):
+        pr = p.copy()
+        pr.routing = ROUTE_ORACLE
+        pr.route_tau = tau
+        with Solver(pr) as s:
+            L, _ = s.construct_l(q)
+            nt, nf = s.route_stats()
+        err = orc.rel_linf(orc.interior(p, L), orc.interior(p, ref))
+        print("tau", tau, "troubled", nt, "of", nf, "err", err)
+        assert np.all(err <= tol), err
+        assert nt == (nf if tau < 0 else 0)
+
+
+def test_routed_exact_matches_oracle():
+    """Mixed routing on a piecewise-constant state (only the jump faces are
+    troubled): the routed kernel reproduces the routed oracle."""
+    p, q, _ = problems.riemann4(mx=64, my=72)
+    orc.set_bnd(p, q)
+    L0, _, n0 = orc.construct_l_routed(p, q, 2, tau=0.05)
+    pr = p.copy()
+    pr.routing = ROUTE_ORACLE
+    pr.route_tau = 0.05
+    with Solver(pr) as s:
+        L1, _ = s.construct_l(q)
+        nt, nf = s.route_stats()
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("troubled", nt, n0, "of", nf, "err", err)
+    assert nt == n0
+    assert np.all(err <= 1e-12)

@@ -74,6 +74,7 @@ class Params(ctypes.Structure):
         ("routing", ctypes.c_int),
         ("route_tau", ctypes.c_double),
         ("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int),
+        ("route_per_step", ctypes.c_int),
     ]
 
     def copy(self):

@@ -53,6 +53,8 @@ struct finess_ctx {
     unsigned long long* smax_h = nullptr;  // pinned copy
     MlpFrags* frags = nullptr;                    // device copy of the installed network
     unsigned long long* route_counts = nullptr;   // device [2]
+    unsigned char* route_mask = nullptr;          // per-face routing of the current step
+    bool route_fresh = true;                      // next routed stage starts a step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double t = 0.0;
     int cur = 0;
@@ -140,7 +142,10 @@ Stage1DArgs base_stage1d_args(finess_ctx* c)
     return a;
 }
 
-int stage_routed(finess_ctx* c, StageArgs& a)
+// stepping: the stage belongs to a time step (the first stage of a step runs
+// the indicator and records its routing for the later ones when
+// route_per_step is set); otherwise a standalone ConstructL.
+int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
 {
     const finess_params& p = c->p;
     if (p.time_stepping_method != FINESS_TS_RK) return fail(c, FINESS_ERR_CONFIG, "routing needs the RK stepper");
@@ -150,6 +155,11 @@ int stage_routed(finess_ctx* c, StageArgs& a)
     a.route_exact = p.routing == FINESS_ROUTE_ORACLE;
     a.route_tau = p.route_tau;
     a.route_counts = c->route_counts;
+    if (stepping && p.route_per_step && c->route_mask) {
+        a.route_mask = c->route_mask;
+        a.route_mask_read = c->route_fresh ? 0 : 1;
+        c->route_fresh = false;
+    }
     const int rc = launch_stage2d_routed(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), a, c->st);
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "routed stage launch failed") : 0;
 }
@@ -189,7 +199,7 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.lxw_F = c->lxwF;
         a.lxw_G = c->lxwG;
     }
-    if (p.routing != FINESS_ROUTE_NONE) return stage_routed(c, a);
+    if (p.routing != FINESS_ROUTE_NONE) return stage_routed(c, a, true);
     const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), lxw, a, c->st);
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
 }
@@ -224,6 +234,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
     int rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st);
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
+    c->route_fresh = true;
     if (p.time_stepping_method == FINESS_TS_LXW) {
         if ((rc = lxw_fluxes(c, A, dt))) return rc;
         if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
@@ -356,6 +367,7 @@ void finess_params_default(finess_params* p)
     p->precision = FINESS_PREC_FP64;
     p->routing = FINESS_ROUTE_NONE;
     p->route_tau = 0.05;
+    p->route_per_step = 1;
     finess_params_finalize(p);
 }
 
@@ -453,6 +465,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
     }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * sizeof(unsigned long long));
+    if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2)
+        rc = alloc(c, (void**)&c->route_mask, (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1));
     if (!rc) rc = launch_fill_u64(c->route_counts, 2, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (!rc) rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (rc) {
@@ -479,6 +493,7 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->smax);
     hipFree(c->frags);
     hipFree(c->route_counts);
+    hipFree(c->route_mask);
     if (c->smax_h) hipHostFree(c->smax_h);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -581,8 +596,12 @@ int finess_construct_l(finess_ctx* c, const double* q, const double* aux, double
     a.Lout = Ld;
     a.smax_faces_x = c->faces_x;
     a.smax_faces_y = c->faces_y;
-    rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 0, a, c->st);
-    if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "stage launch failed");
+    if (p.routing != FINESS_ROUTE_NONE) {
+        if ((rc = stage_routed(c, a, false))) return rc;
+    } else {
+        rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 0, a, c->st);
+        if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "stage launch failed");
+    }
     const size_t nL = (size_t)c->M * c->plane;
     std::vector<double> L(nL), ax((size_t)(p.mx + 1) * p.my), ay((size_t)p.mx * (p.my + 1));
     HIPCHK(c, hipMemcpyAsync(L.data(), Ld, nL * sizeof(double), hipMemcpyDeviceToHost, c->st));

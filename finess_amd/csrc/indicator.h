@@ -161,40 +161,35 @@ __host__ __device__ __forceinline__ R face_departure(const R (*rec)[P::NV], int 
 // Smooth-interface flux: component-wise LLF + linear fifth-order upwind
 // reconstruction of the split conserved fluxes.  Returns alpha (the same
 // wave speed the characteristic path uses, so the CFL is unchanged).
+// The FD5 weights (2, -13, 47, 27, -3)/60 on f + la q over cells 0..4 and on
+// f - la q over cells 5..1 fold into one symmetric stencil on f and one
+// antisymmetric stencil on q:
+//   fhat = [(f0+f5) - 8(f1+f4) + 37(f2+f3)]/60 + la [(q0-q5) - 5(q1-q4) + 10(q2-q3)]/60.
 template <class P, typename R, int NC>
 __host__ __device__ __forceinline__ R face_flux_linear(const R (*cv)[NC], int base, int stride, int dir,
                                                        const PhysParams& pp, R ascale, R fh[P::MEQN])
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    R cl[NV], cr[NV], ql[M], qr[M];
-    for (int v = 0; v < NV; v++) {
-        cl[v] = cv[v][base + 2 * stride];
-        cr[v] = cv[v][base + 3 * stride];
-    }
-    P::state(cl, dir, ql);
-    P::state(cr, dir, qr);
-    const R alpha = P::wave_speed(ql, qr, cl, cr, pp);
-    const R la = ascale * alpha;
-    // linear weights folded into one 6-point stencil per split part:
-    // FD5(u_{i-2..i+2}) = (2u_{i-2} - 13u_{i-1} + 47u_i + 27u_{i+1} - 3u_{i+2}) / 60
-    const R c0 = R(2) / R(60), c1 = R(-13) / R(60), c2 = R(47) / R(60), c3 = R(27) / R(60), c4 = R(-3) / R(60);
-    R hp[M], hm[M];
-    for (int k = 0; k < M; k++) { hp[k] = R(0); hm[k] = R(0); }
+    R q[6][M], f[6][M], cl[NV], cr[NV];
 #pragma unroll
     for (int s = 0; s < 6; s++) {
-        R rec[NV], q[M], f[M];
+        R rec[NV];
         for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
-        P::state(rec, dir, q);
-        P::flux(q, rec, f);
-        // gp uses cells s = 0..4 with coefficients c0..c4; gm uses s = 5..1
-        const R cp = s == 0 ? c0 : s == 1 ? c1 : s == 2 ? c2 : s == 3 ? c3 : s == 4 ? c4 : R(0);
-        const R cm = s == 5 ? c0 : s == 4 ? c1 : s == 3 ? c2 : s == 2 ? c3 : s == 1 ? c4 : R(0);
-        for (int k = 0; k < M; k++) {
-            hp[k] += cp * (f[k] + la * q[k]);
-            hm[k] += cm * (f[k] - la * q[k]);
-        }
+        P::state(rec, dir, q[s]);
+        P::flux(q[s], rec, f[s]);
+        if (s == 2)
+            for (int v = 0; v < NV; v++) cl[v] = rec[v];
+        if (s == 3)
+            for (int v = 0; v < NV; v++) cr[v] = rec[v];
     }
-    for (int k = 0; k < M; k++) fh[P::rot_q(k, dir)] = R(0.5) * (hp[k] + hm[k]);
+    const R alpha = P::wave_speed(q[2], q[3], cl, cr, pp);
+    const R la = ascale * alpha;
+    const R k60 = R(1) / R(60);
+    for (int k = 0; k < M; k++) {
+        const R fa = (f[0][k] + f[5][k]) - R(8) * (f[1][k] + f[4][k]) + R(37) * (f[2][k] + f[3][k]);
+        const R qb = (q[0][k] - q[5][k]) - R(5) * (q[1][k] - q[4][k]) + R(10) * (q[2][k] - q[3][k]);
+        fh[P::rot_q(k, dir)] = (fa + la * qb) * k60;
+    }
     return alpha;
 }
 

@@ -41,10 +41,24 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Two FP32 values to a packed BF16 pair (low half = a) with the gfx950
+// conversion instruction, which rounds to nearest even like f2bf.
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    bf16x2_t v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    return u;
+}
+
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 {
+    const uint32_t u[2] = {bf16_pair(a, b), bf16_pair(c, d)};
     bf16x4 r;
-    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    __builtin_memcpy(&r, u, 8);
     return r;
 }
 
@@ -54,14 +68,16 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 //   w2[mb][kb][lane][4]  W2^T                                      (2 x 2)
 //   w3[kb][lane][4]      w3^T in row 0 of a 16-row tile            (2)
 //   b1[32], b2[32], b3, threshold on the logit (troubled if above)
-struct MlpFrags {
+struct __attribute__((aligned(16))) MlpFrags {
     uint16_t w1[2][2][64][4];
     uint16_t w2[2][2][64][4];
     uint16_t w3[2][64][4];
     float b1[32], b2[32];
     float b3;
     float threshold;
+    float pad[2];  // whole 16-byte chunks, so a workgroup can stage it in LDS with vector copies
 };
+static_assert(sizeof(MlpFrags) % 16 == 0, "MlpFrags is copied in 16-byte chunks");
 
 void mlp_make_frags(const float* params, float threshold, MlpFrags* f);
 

@@ -81,6 +81,11 @@ struct StageArgs {
     int route_exact;
     double route_tau;
     unsigned long long* route_counts;  // [0] troubled faces, [1] faces
+    // per-face routing flags, x faces [(mx+1) my] then y faces [mx (my+1)]:
+    // written by the indicator when route_mask_read is 0, else read instead
+    // of running the indicator (null: indicator only, nothing stored)
+    unsigned char* route_mask;
+    int route_mask_read;
     // optional face-flux / L output for the drop-in ConstructL (may be null):
     double* Lout;          // L planes (mx*my)
     double* smax_faces_x;  // (mx+1) x my alphas of x-faces, row-major i*(my)+j

@@ -36,10 +36,11 @@ __device__ __forceinline__ double wave_max_r(double v)
 }
 
 struct RouteSmem {
+    MlpFrags fr;  // the network, staged once per workgroup
     uint16_t feat[4][64][MLP_FEAT_LD];
     float lo[4][64];
     int list[256];
-    int nt, ns;
+    int nt[2], ns[2];  // troubled / smooth counts per face direction
 };
 
 template <class P, typename R, int WK, int TI, int TJ, bool EXACT>
@@ -50,16 +51,19 @@ __device__ __forceinline__ void routed_faces(int t, int dir, int i0, int j0, con
     constexpr int M = P::MEQN, NV = P::NV;
     const int lane = t & 63, w = t >> 6;
     const FaceGeom g = face_geom<TI, TJ>(t, dir, i0, j0, a);
-    if (t == 0) { rs.nt = 0; rs.ns = 0; }
+    const long long nfx = (long long)(a.mx + 1) * a.my;
+    const long long mi = dir == 0 ? (long long)(i0 + g.fi) * a.my + (j0 + g.fj)
+                                  : nfx + (long long)(i0 + g.fi) * (a.my + 1) + (j0 + g.fj);
     bool troubled = false;
-    if (EXACT) {
+    if (a.route_mask_read) {  // routing recorded by this step's first stage
+        if (g.valid) troubled = a.route_mask[mi] != 0;
+    } else if (EXACT) {
         if (g.valid) {
             R rec[6][NV];
             for (int s = 0; s < 6; s++)
                 for (int v = 0; v < NV; v++) rec[s][v] = sm.cv[v][g.base + s * g.stride];
             troubled = face_departure<P, R>(rec, dir, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
         }
-        __syncthreads();
     } else {
         float f[MLP_IN];
         for (int k = 0; k < MLP_IN; k++) f[k] = 0.f;
@@ -69,15 +73,16 @@ __device__ __forceinline__ void routed_faces(int t, int dir, int i0, int j0, con
                 for (int v = 0; v < NV; v++) rec[s][v] = sm.cv[v][g.base + s * g.stride];
             face_features<P, R>(rec, dir, a.pp, f);
         }
-        const float z = mlp_wave_logit(f, *a.frags, rs.feat[w], rs.lo[w], lane);  // synchronises
-        troubled = z > a.frags->threshold;
+        const float z = mlp_wave_logit(f, rs.fr, rs.feat[w], rs.lo[w], lane);
+        troubled = z > rs.fr.threshold;
     }
+    if (a.route_mask && !a.route_mask_read && g.valid) a.route_mask[mi] = troubled ? 1 : 0;
     if (g.valid) {
-        if (troubled) rs.list[atomicAdd(&rs.nt, 1)] = t;
-        else rs.list[255 - atomicAdd(&rs.ns, 1)] = t;
+        if (troubled) rs.list[atomicAdd(&rs.nt[dir], 1)] = t;
+        else rs.list[255 - atomicAdd(&rs.ns[dir], 1)] = t;
     }
     __syncthreads();
-    const int nt = rs.nt, ns = rs.ns;
+    const int nt = rs.nt[dir], ns = rs.ns[dir];
     if (t == 0 && a.route_counts) {
         atomicAdd(&a.route_counts[0], (unsigned long long)nt);
         atomicAdd(&a.route_counts[1], (unsigned long long)(nt + ns));
@@ -114,6 +119,8 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
     const int t = threadIdx.x;
+    if (!EXACT) mlp_stage_frags(a.frags, &rs.fr, t, NT);
+    if (t < 2) { rs.nt[t] = 0; rs.ns[t] = 0; }  // ordered before the appends by the load barrier
     phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
     __syncthreads();
     double ax = 0.0, ay = 0.0;

@@ -1070,8 +1070,23 @@ static double interface_flux_linear(const finess_params* p, int ixy, const doubl
     return alpha;
 }
 
-int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
-                           double* Lstar, double* smax, long long* n_troubled)
+double orc_face_flux_linear(const finess_params* p, int ixy, const double* qs, double* fhat)
+{
+    double q6[6][MAXEQ], f6[6][MAXEQ], F[MAXEQ], G[MAXEQ], aux[1] = {0};
+    for (int s = 0; s < 6; s++) {
+        for (int m = 0; m < p->meqn; m++) q6[s][m] = qs[s * p->meqn + m];
+        orc_flux(p, q6[s], aux, F, G);
+        for (int m = 0; m < p->meqn; m++) f6[s][m] = ixy == 1 ? F[m] : G[m];
+    }
+    return interface_flux_linear(p, ixy, q6, f6, fhat);
+}
+
+/* mask (may be NULL): per-face routing, x faces (i-1)*my + j-1 then y faces
+ * (mx+1)*my + (i-1)*(my+1) + j-1; mask_read: take mode-2 decisions from it,
+ * else record them there. */
+static int construct_l_routed_mask(const finess_params* p, const double* q, const double* aux, int mode,
+                                   double tau, unsigned char* mask, int mask_read, double* Lstar, double* smax,
+                                   long long* n_troubled)
 {
     const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
     if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
@@ -1091,7 +1106,13 @@ int orc_construct_l_routed(const finess_params* p, const double* q, const double
                     for (int m = 0; m < meqn; m++) fs[s - 1][m] = dir == 1 ? F[m] : G[m];
                 }
                 int full = mode == 0;
-                if (mode == 2) full = orc_face_departure(p, dir, &qs[0][0]) > tau;
+                const long long mi = dir == 1 ? (long long)(i - 1) * my + (j - 1)
+                                              : (long long)(mx + 1) * my + (long long)(i - 1) * (my + 1) + (j - 1);
+                if (mode == 2) {
+                    if (mask && mask_read) full = mask[mi] != 0;
+                    else full = orc_face_departure(p, dir, &qs[0][0]) > tau;
+                    if (mask && !mask_read) mask[mi] = (unsigned char)full;
+                }
                 nt += full;
                 double alpha;
                 if (full) {
@@ -1120,6 +1141,47 @@ int orc_construct_l_routed(const finess_params* p, const double* q, const double
     free(Ghat);
     if (n_troubled) *n_troubled = nt;
     return FINESS_OK;
+}
+
+int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
+                           double* Lstar, double* smax, long long* n_troubled)
+{
+    return construct_l_routed_mask(p, q, aux, mode, tau, NULL, 0, Lstar, smax, n_troubled);
+}
+
+int orc_run_fixed_steps_routed(const finess_params* p, double* q, double* aux, double dt, int nsteps, double tau,
+                               int per_step, double* cfl)
+{
+    int rc = check_config(p);
+    if (rc) return rc;
+    if (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->time_order != 3) return FINESS_ERR_CONFIG;
+    const size_t numel = state_numel(p, p->meqn);
+    double* qstar = (double*)malloc(numel * sizeof(double));
+    double* L = (double*)calloc(numel, sizeof(double));
+    double* smax = (double*)calloc(smax_numel(p), sizeof(double));
+    unsigned char* mask = (unsigned char*)calloc((size_t)(p->mx + 1) * p->my + (size_t)p->mx * (p->my + 1), 1);
+    const int k = 2;
+    double c = 0.0;
+    for (int n = 0; n < nsteps && !rc; n++) {
+        memset(smax, 0, smax_numel(p) * sizeof(double));
+        unsigned char* m = per_step ? mask : NULL;
+        orc_set_bnd(p, q, aux);
+        rc = construct_l_routed_mask(p, q, aux, 2, tau, m, 0, L, smax, NULL);
+        if (rc) break;
+        update_soln(numel, RK_A1[k][0], RK_A2[k][0], RK_B[k][0], dt, q, L, qstar);
+        orc_set_bnd(p, qstar, aux);
+        rc = construct_l_routed_mask(p, qstar, aux, 2, tau, m, 1, L, smax, NULL);
+        if (rc) break;
+        update_soln(numel, RK_A1[k][1], RK_A2[k][1], RK_B[k][1], dt, q, L, qstar);
+        orc_set_bnd(p, qstar, aux);
+        rc = construct_l_routed_mask(p, qstar, aux, 2, tau, m, 1, L, smax, NULL);
+        if (rc) break;
+        update_soln(numel, RK_A1[k][2], RK_A2[k][2], RK_B[k][2], dt, qstar, L, q);
+        rc = orc_get_cfl(p, dt, 1e300, smax, &c);
+    }
+    if (cfl) *cfl = c;
+    free(qstar); free(L); free(smax); free(mask);
+    return rc;
 }
 
 /* ---- MLP with BF16 operand rounding ------------------------------------- */

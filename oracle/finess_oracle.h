@@ -71,11 +71,18 @@ int orc_construct_lxw_l(const finess_params* p, const double* q, const double* a
  * qs: the 6 stencil states q(i-3..i+2) across a face, [6][meqn]. */
 void   orc_face_features(const finess_params* p, int ixy, const double* qs, float* feat /* [24] */);
 double orc_face_departure(const finess_params* p, int ixy, const double* qs);
+/* The smooth-interface flux (component-wise LLF, FD5 of lib/WenoReconstruct.cpp:580-616
+ * on each split part); returns alpha. */
+double orc_face_flux_linear(const finess_params* p, int ixy, const double* qs, double* fhat);
 /* ConstructL with routing: mode 0 characteristic WENO everywhere, 1 the
  * component-wise linear fifth-order flux everywhere, 2 characteristic WENO
  * where orc_face_departure > tau else linear.  *n_troubled counts faces. */
 int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
                            double* Lstar, double* smax, long long* n_troubled);
+/* Fixed SSP-RK3 steps with mode-2 routing; per_step: the first stage of
+ * each step decides the routing and the later stages reuse it. */
+int orc_run_fixed_steps_routed(const finess_params* p, double* q, double* aux, double dt, int nsteps, double tau,
+                               int per_step, double* cfl);
 /* MLP 24-32-32-1 (parameter layout of include/finess_hip.h's finess_mlp),
  * every matrix product with both operands rounded to BF16 as the matrix cores
  * do, accumulated in FP64. */

@@ -10,6 +10,7 @@
 #include "../../finess_amd/csrc/lxw_body.h"
 #include "../../finess_amd/csrc/stage1d_body.h"
 #include "../../finess_amd/csrc/stage2d_body.h"
+#include "../../finess_amd/csrc/indicator.h"
 
 using namespace finess;
 
@@ -226,6 +227,56 @@ int emu_run_fixed_steps(const finess_params* p, double* q, double dt, int nsteps
             for (int m = 0; m < M; m++)
                 q[((i + g) * ldq + (j + g)) * M + m] = buf[cur][(size_t)m * plane + (size_t)i * p->my + j];
     return 0;
+}
+
+// ---- troubled-interface indicator: the device functions on one 6-cell
+// stencil qs[6][5] (states q(i-3..i+2) across the face), dir 1 = x, 2 = y.
+static void emu_records(const finess_params* p, const double* qs, double cv[8][6])
+{
+    PhysParams pp{};
+    pp.gamma = p->gamma;
+    for (int s = 0; s < 6; s++) {
+        double rec[8];
+        Euler2D::cell_record(qs + 5 * s, pp, rec);
+        for (int v = 0; v < 8; v++) cv[v][s] = rec[v];
+    }
+}
+
+void emu_face_features(const finess_params* p, int dir, const double* qs, int fp32, float* feat)
+{
+    PhysParams pp{};
+    pp.gamma = p->gamma;
+    double cv[8][6];
+    emu_records(p, qs, cv);
+    if (fp32) {
+        float rec[6][8];
+        for (int s = 0; s < 6; s++) for (int v = 0; v < 8; v++) rec[s][v] = (float)cv[v][s];
+        face_features<Euler2D, float>(rec, dir - 1, pp, feat);
+    } else {
+        double rec[6][8];
+        for (int s = 0; s < 6; s++) for (int v = 0; v < 8; v++) rec[s][v] = cv[v][s];
+        face_features<Euler2D, double>(rec, dir - 1, pp, feat);
+    }
+}
+
+double emu_face_departure(const finess_params* p, int dir, const double* qs)
+{
+    PhysParams pp{};
+    pp.gamma = p->gamma;
+    double cv[8][6], rec[6][8];
+    emu_records(p, qs, cv);
+    for (int s = 0; s < 6; s++) for (int v = 0; v < 8; v++) rec[s][v] = cv[v][s];
+    return face_departure<Euler2D, double>(rec, dir - 1, pp, p->epsilon, p->alpha_scaling);
+}
+
+// fh in the unrotated component order, as the stage kernel stores it
+double emu_face_flux_linear(const finess_params* p, int dir, const double* qs, double* fh)
+{
+    PhysParams pp{};
+    pp.gamma = p->gamma;
+    double cv[8][6];
+    emu_records(p, qs, cv);
+    return face_flux_linear<Euler2D, double, 6>(cv, 0, 1, dir - 1, pp, p->alpha_scaling, fh);
 }
 
 } // extern "C"

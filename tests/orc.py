@@ -63,6 +63,9 @@ def oracle():
             "orc_num_threads": (ctypes.c_int, []),
             "orc_face_features": (None, [PP, ctypes.c_int, PD, PF]),
             "orc_face_departure": (ctypes.c_double, [PP, ctypes.c_int, PD]),
+            "orc_face_flux_linear": (ctypes.c_double, [PP, ctypes.c_int, PD, PD]),
+            "orc_run_fixed_steps_routed": (ctypes.c_int, [PP, PD, PD, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                                          ctypes.c_int, PD]),
             "orc_construct_l_routed": (ctypes.c_int, [PP, PD, PD, ctypes.c_int, ctypes.c_double, PD, PD,
                                                       ctypes.POINTER(ctypes.c_longlong)]),
             "orc_bf16": (ctypes.c_float, [ctypes.c_float]),
@@ -89,6 +92,12 @@ def emu():
         lib.emu_construct_integrated_r.argtypes = [PP, ctypes.c_double, PD, PD, PD]
         lib.emu_run_fixed_steps.restype = ctypes.c_int
         lib.emu_run_fixed_steps.argtypes = [PP, PD, ctypes.c_double, ctypes.c_int]
+        lib.emu_face_features.restype = None
+        lib.emu_face_features.argtypes = [PP, ctypes.c_int, PD, ctypes.c_int, PF]
+        lib.emu_face_departure.restype = ctypes.c_double
+        lib.emu_face_departure.argtypes = [PP, ctypes.c_int, PD]
+        lib.emu_face_flux_linear.restype = ctypes.c_double
+        lib.emu_face_flux_linear.argtypes = [PP, ctypes.c_int, PD, PD]
         _emu = lib
     return _emu
 
@@ -225,3 +234,34 @@ def mlp_gradient(P, X, y):
     loss = oracle().orc_mlp_gradient(fptr(np.ascontiguousarray(P, np.float32)), fptr(np.ascontiguousarray(X, np.float32)),
                                      fptr(np.ascontiguousarray(y, np.float32)), len(y), ptr(g))
     return g, loss
+
+
+def face_flux_linear(p, dir_, qs):
+    fh = np.zeros(5)
+    a = oracle().orc_face_flux_linear(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)), ptr(fh))
+    return fh, a
+
+
+def emu_face_features(p, dir_, qs, fp32=False):
+    f = np.zeros(24, np.float32)
+    emu().emu_face_features(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)), int(fp32), fptr(f))
+    return f
+
+
+def emu_face_departure(p, dir_, qs):
+    return emu().emu_face_departure(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)))
+
+
+def emu_face_flux_linear(p, dir_, qs):
+    fh = np.zeros(5)
+    a = emu().emu_face_flux_linear(ctypes.byref(p), int(dir_), ptr(np.ascontiguousarray(qs, np.float64)), ptr(fh))
+    return fh, a
+
+
+def fixed_steps_routed(p, q, dt, n, tau, per_step):
+    """SSP-RK3 steps with exact-departure routing; q (reference layout) in place."""
+    aux = np.zeros(p.shape_aux())
+    c = ctypes.c_double()
+    assert oracle().orc_run_fixed_steps_routed(ctypes.byref(p), ptr(q), ptr(aux), float(dt), int(n), float(tau),
+                                               int(per_step), ctypes.byref(c)) == 0
+    return c.value

@@ -125,3 +125,22 @@ def test_routed_exact_matches_oracle():
     print("troubled", nt, n0, "of", nf, "err", err)
     assert nt == n0
     assert np.all(err <= 1e-12)
+
+
+@pytest.mark.parametrize("per_step", [0, 1])
+def test_routed_stepping_matches_oracle(per_step):
+    """SSP-RK3 steps with exact-departure routing decided at every stage or
+    once per step (recorded at the first stage, reused by the later ones)."""
+    p, q, _ = problems.riemann4(mx=64, my=72, routing=ROUTE_ORACLE, route_tau=0.05, route_per_step=per_step)
+    dt = 0.3 / 72 / 2.5
+    q_ref = q.copy()
+    orc.fixed_steps_routed(p, q_ref, dt, 5, 0.05, per_step)
+    with Solver(p) as s:
+        s.set_state(q)
+        s.run_fixed_steps(dt, 5)
+        q_gpu, _ = s.get_state()
+        nt, nf = s.route_stats()
+    err = orc.rel_linf(orc.interior(p, q_gpu), orc.interior(p, q_ref))
+    print("per_step", per_step, "troubled", nt, "of", nf, "err", err)
+    assert 0 < nt < nf
+    assert np.all(err <= 1e-10), err

@@ -127,3 +127,59 @@ def test_linear_flux_is_fifth_order():
         errs.append(np.abs(orc.interior(p, L)[..., 0] - orc.interior(p, ex[..., None])[..., 0]).max())
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert rates[-1] > 4.7, (errs, rates)
+
+
+def test_linear_flux_equals_weno_on_linear_data():
+    """With u, v, p constant and rho linear in x, both split fluxes are linear
+    along the stencil, every smoothness indicator is equal, the JS weights are
+    exactly the linear weights, and characteristic WENO5 reduces to the linear
+    flux: interior L of the two routes agree and the departure vanishes."""
+    p, _, _ = problems.riemann4(mx=24, my=12)
+    X, _ = problems.cell_centres(p)
+    rho = 1.0 + 0.5 * X
+    q = problems.euler_conserved(rho, np.full_like(X, 0.4), np.full_like(X, -0.2), np.full_like(X, 1.3))
+    orc.set_bnd(p, q)
+    Lw, sw, _ = orc.construct_l_routed(p, q, 0)
+    Ll, sl, _ = orc.construct_l_routed(p, q, 1)
+    g = p.mbc
+    sl_ = (slice(g + 4, g + p.mx - 4), slice(g, g + p.my))   # cells whose faces see no boundary ghost
+    assert np.abs(Lw[sl_] - Ll[sl_]).max() <= 1e-12 * np.abs(Lw[sl_]).max()
+    assert np.array_equal(sw, sl)
+    qs = orc.face_stencil(p, q, 1, 10, 5)
+    assert orc.departure(p, 1, qs) < 1e-12
+
+
+@pytest.mark.parametrize("dir_", [1, 2])
+def test_device_indicator_functions_match_oracle(dir_):
+    """The stage kernel's feature, label and linear-flux routines, run on the
+    host, against the oracle on random stencils."""
+    p, _, _ = problems.riemann4(mx=16)
+    for _ in range(50):
+        qs, _ = _euler_stencil()
+        f_o = orc.features(p, dir_, qs)
+        assert np.allclose(orc.emu_face_features(p, dir_, qs), f_o, atol=1e-6)
+        assert np.allclose(orc.emu_face_features(p, dir_, qs, fp32=True), f_o, atol=2e-5)
+        assert abs(orc.emu_face_departure(p, dir_, qs) - orc.departure(p, dir_, qs)) < 1e-12
+        fh_o, a_o = orc.face_flux_linear(p, dir_, qs)
+        fh_e, a_e = orc.emu_face_flux_linear(p, dir_, qs)
+        assert abs(a_e - a_o) <= 1e-14 * a_o
+        assert np.allclose(fh_e, fh_o, rtol=1e-13, atol=1e-13 * np.abs(fh_o).max())
+
+
+def test_routed_stepping_identities():
+    """Routed SSP-RK3 stepping: with every face troubled it is the plain
+    stepper bit for bit, whether the routing is decided per stage or per step;
+    mixed routing decided once per step stays close to deciding per stage."""
+    p, q0, _ = problems.riemann4(mx=24, my=20)
+    dt = 0.3 / 24 / 2.5
+    qa = q0.copy()
+    orc.fixed_steps(p, qa, dt, 3)
+    for per_step in (0, 1):
+        qb = q0.copy()
+        orc.fixed_steps_routed(p, qb, dt, 3, -1.0, per_step)
+        assert np.array_equal(orc.interior(p, qb), orc.interior(p, qa))
+    qc, qd = q0.copy(), q0.copy()
+    orc.fixed_steps_routed(p, qc, dt, 3, 0.05, 0)
+    orc.fixed_steps_routed(p, qd, dt, 3, 0.05, 1)
+    dif = orc.rel_linf(orc.interior(p, qd), orc.interior(p, qc))
+    assert 0 < dif.max() < 0.05, dif

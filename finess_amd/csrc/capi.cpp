@@ -49,10 +49,10 @@ struct finess_ctx {
     double* faces_x = nullptr;  // per-face wave speeds (drop-in ConstructL)
     double* faces_y = nullptr;
     double* fhat1d = nullptr;
-    unsigned long long* smax = nullptr;    // device CFL slots [8][2]
+    unsigned long long* smax = nullptr;    // device CFL slots [NSLOT][2][CFL_SPREAD]
     unsigned long long* smax_h = nullptr;  // pinned copy
     MlpFrags* frags = nullptr;                    // device copy of the installed network
-    unsigned long long* route_counts = nullptr;   // device [2]
+    unsigned long long* route_counts = nullptr;   // device [2][CFL_SPREAD]
     unsigned char* route_mask = nullptr;          // per-face routing of the current step
     bool route_fresh = true;                      // next routed stage starts a step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -178,7 +178,7 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.a1 = a1;
         a.a2 = a2;
         a.bdt = beta * dt;
-        a.smax_slot = c->smax + 2 * slot;
+        a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
         const int rc = launch_stage1d(weno_kind(p), a, c->st);
         return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D stage launch failed") : 0;
     }
@@ -193,7 +193,7 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
     a.a1 = a1;
     a.a2 = a2;
     a.bdt = beta * dt;
-    a.smax = c->smax + 2 * slot;
+    a.smax = c->smax + 2 * CFL_SPREAD * slot;
     const bool lxw = p.time_stepping_method == FINESS_TS_LXW;
     if (lxw) {
         a.lxw_F = c->lxwF;
@@ -232,7 +232,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
 {
     const finess_params& p = c->p;
     const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
-    int rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st);
+    int rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st);
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
     c->route_fresh = true;
     if (p.time_stepping_method == FINESS_TS_LXW) {
@@ -270,11 +270,16 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
 // GetCFL from the slot's maxima (lib/2d/GetCFL.cpp:31-74, lib/1d/GetCFL.cpp:9-45).
 int read_cfl(finess_ctx* c, int slot, double dt, double dtmax, double* cfl_out)
 {
-    HIPCHK(c, hipMemcpyAsync(c->smax_h, c->smax + 2 * slot, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->st));
+    HIPCHK(c, hipMemcpyAsync(c->smax_h, c->smax + 2 * CFL_SPREAD * slot, 2 * CFL_SPREAD * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     const finess_params& p = c->p;
-    const double sx = bits_to_double(c->smax_h[0]), sy = bits_to_double(c->smax_h[1]);
+    unsigned long long bx = 0, by = 0;  // bit patterns of non-negative doubles (or +inf)
+    for (int k = 0; k < CFL_SPREAD; k++) {
+        bx = c->smax_h[k] > bx ? c->smax_h[k] : bx;
+        by = c->smax_h[CFL_SPREAD + k] > by ? c->smax_h[CFL_SPREAD + k] : by;
+    }
+    const double sx = bits_to_double(bx), sy = bits_to_double(by);
     if (dt > dtmax) return fail(c, FINESS_ERR_DT, "dt is out of bounds (dt > max_dt)");
     if (!isfinite(sx) || !isfinite(sy)) return fail(c, FINESS_ERR_NONFINITE, "non-finite wave speed");
     double cfl = -100.0;
@@ -445,8 +450,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
     }
     const size_t sbytes = (size_t)c->M * c->plane * sizeof(double);
     for (int k = 0; k < 3 && !rc; k++) rc = alloc(c, (void**)&c->buf[k], sbytes);
-    if (!rc) rc = alloc(c, (void**)&c->smax, 2 * NSLOT * sizeof(unsigned long long));
-    if (!rc && hipHostMalloc((void**)&c->smax_h, 2 * NSLOT * sizeof(unsigned long long), hipHostMallocDefault) !=
+    if (!rc) rc = alloc(c, (void**)&c->smax, 2 * CFL_SPREAD * NSLOT * sizeof(unsigned long long));
+    if (!rc && hipHostMalloc((void**)&c->smax_h, 2 * CFL_SPREAD * NSLOT * sizeof(unsigned long long), hipHostMallocDefault) !=
                    hipSuccess)
         rc = FINESS_ERR_HIP;
     if (!rc) rc = alloc(c, (void**)&c->staging, ref_numel(p, c->M) * sizeof(double));
@@ -464,11 +469,11 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         rc = alloc(c, (void**)&c->lxwF, lb);
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
     }
-    if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * sizeof(unsigned long long));
+    if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
     if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2)
         rc = alloc(c, (void**)&c->route_mask, (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1));
-    if (!rc) rc = launch_fill_u64(c->route_counts, 2, 0ull, c->st) ? FINESS_ERR_HIP : 0;
-    if (!rc) rc = launch_fill_u64(c->smax, 2 * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
+    if (!rc) rc = launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st) ? FINESS_ERR_HIP : 0;
+    if (!rc) rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (rc) {
         finess_destroy(c);
         return rc;
@@ -810,12 +815,17 @@ int finess_set_indicator(finess_ctx* c, const float* params, float threshold)
 int finess_route_stats(finess_ctx* c, long long* troubled, long long* faces)
 {
     if (!c) return FINESS_ERR_ARG;
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[2 * CFL_SPREAD];
     HIPCHK(c, hipMemcpyAsync(h, c->route_counts, sizeof h, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (launch_fill_u64(c->route_counts, 2, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "reset failed");
-    if (troubled) *troubled = (long long)h[0];
-    if (faces) *faces = (long long)h[1];
+    if (launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "reset failed");
+    unsigned long long nt = 0, nf = 0;
+    for (int k = 0; k < CFL_SPREAD; k++) {
+        nt += h[k];
+        nf += h[CFL_SPREAD + k];
+    }
+    if (troubled) *troubled = (long long)nt;
+    if (faces) *faces = (long long)nf;
     return FINESS_OK;
 }
 

@@ -21,7 +21,8 @@ __global__ __launch_bounds__(256) void k_faces1d(Stage1DArgs a)
     if (a.smax_slot) {
         if (alpha != alpha) alpha = __builtin_inf();
         // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
-        if (f < a.mx) atomicMax(a.smax_slot, (unsigned long long)__double_as_longlong(alpha));
+        if (f < a.mx)
+            atomicMax(a.smax_slot + blockIdx.x % CFL_SPREAD, (unsigned long long)__double_as_longlong(alpha));
     }
 }
 

@@ -26,7 +26,8 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 // Reduces two per-thread maxima over the workgroup and folds them into
-// slot[0..1] (non-negative doubles compare like their bit patterns).
+// slot[b] and slot[CFL_SPREAD + b], b = block index mod CFL_SPREAD
+// (non-negative doubles compare like their bit patterns).
 template <int NT>
 __device__ __forceinline__ void block_max2(double ax, double ay, unsigned long long* slot)
 {
@@ -42,8 +43,9 @@ __device__ __forceinline__ void block_max2(double ax, double ay, unsigned long l
             mx_ = red[0][w] > mx_ ? red[0][w] : mx_;
             my_ = red[1][w] > my_ ? red[1][w] : my_;
         }
-        atomicMax(&slot[0], (unsigned long long)__double_as_longlong(mx_));
-        atomicMax(&slot[1], (unsigned long long)__double_as_longlong(my_));
+        const int b = blockIdx.x % CFL_SPREAD;
+        atomicMax(&slot[b], (unsigned long long)__double_as_longlong(mx_));
+        atomicMax(&slot[CFL_SPREAD + b], (unsigned long long)__double_as_longlong(my_));
     }
 }
 
@@ -66,6 +68,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
     const int t = threadIdx.x;
+    double pa[P::MEQN], pb[P::MEQN];
+    phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
     phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
     __syncthreads();
     double ax = 0.0, ay = 0.0;
@@ -76,7 +80,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     __syncthreads();
     phase_faces<P, R, WK, LXW, TI, TJ>(t, 1, i0, j0, a, sm, ay);
     __syncthreads();
-    phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx);
+    phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
     if (a.smax) {
         // a NaN speed must not vanish in the max: report it as +inf
         if (ax != ax) ax = __builtin_inf();

@@ -50,6 +50,12 @@ __host__ __device__ __forceinline__ int bc_map(int bc_lo, int bc_hi, int g, int 
     return g;
 }
 
+// Workgroup maxima and counters are folded into CFL_SPREAD separate words
+// per quantity (chosen by block index): a single word hit by every
+// workgroup of a launch serialises thousands of atomics in one L2 channel.
+// The host reduces the CFL_SPREAD words when it reads them.
+constexpr int CFL_SPREAD = 64;
+
 struct StageArgs {
     // state planes, field-major: q[m][i][j] at m*plane + i*ld + j (ld = my)
     const double* src;     // stencil operand
@@ -73,14 +79,15 @@ struct StageArgs {
     double alpha_scaling;
     double eps, power;
     PhysParams pp;
-    // wave-speed maxima (bit patterns of non-negative doubles), [2] per slot
+    // wave-speed maxima (bit patterns of non-negative doubles),
+    // [2][CFL_SPREAD] per slot: x then y
     unsigned long long* smax;
     // troubled-interface routing (routed kernel only): network fragments,
     // or the exact weight-departure test when route_exact is set
     const struct MlpFrags* frags;
     int route_exact;
     double route_tau;
-    unsigned long long* route_counts;  // [0] troubled faces, [1] faces
+    unsigned long long* route_counts;  // [2][CFL_SPREAD]: troubled faces, faces
     // per-face routing flags, x faces [(mx+1) my] then y faces [mx (my+1)]:
     // written by the indicator when route_mask_read is 0, else read instead
     // of running the indicator (null: indicator only, nothing stored)
@@ -127,17 +134,49 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
                                                     StageSmem<P, R, TI, TJ>& sm)
 {
     using S = StageSmem<P, R, TI, TJ>;
-    for (int c = tid; c < S::NC; c += nthreads) {
-        const int ci = c / S::HJ, cj = c - ci * S::HJ;
-        const bool ri = ci < 3 || ci >= TI + 3, rj = cj < 3 || cj >= TJ + 3;
-        if (ri && rj) continue;                       // corners are never read
-        const int gi = i0 + ci - 3, gj = j0 + cj - 3;
-        if (gi > a.mx + 2 || gj > a.my + 2) continue; // beyond every face's stencil
-        double q[P::MEQN], rec[P::NV];
-        load_state<P>(a, gi, gj, q);
-        P::cell_record(q, a.pp, rec);
-        for (int v = 0; v < P::NV; v++) sm.cv[v][c] = R(rec[v]);
+    // two cells per thread and round: both cells' loads are issued before
+    // either record is formed, so their memory latencies overlap
+    constexpr int K = 2;
+    for (int c0 = tid; c0 < S::NC; c0 += K * nthreads) {
+        double q[K][P::MEQN];
+        bool ok[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int c = c0 + k * nthreads;
+            const int ci = c / S::HJ, cj = c - ci * S::HJ;
+            const bool ri = ci < 3 || ci >= TI + 3, rj = cj < 3 || cj >= TJ + 3;
+            const int gi = i0 + ci - 3, gj = j0 + cj - 3;
+            // corners are never read; nor is anything beyond every face's stencil
+            ok[k] = c < S::NC && !(ri && rj) && gi <= a.mx + 2 && gj <= a.my + 2;
+            if (ok[k]) load_state<P>(a, gi, gj, q[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!ok[k]) continue;
+            double rec[P::NV];
+            P::cell_record(q[k], a.pp, rec);
+            for (int v = 0; v < P::NV; v++) sm.cv[v][c0 + k * nthreads] = R(rec[v]);
+        }
     }
+}
+
+// The old-state operands of the thread's cell for the stage update, read at
+// the start of the kernel so that their latency hides behind the tile load
+// (those the FP64 tile already holds exactly are taken from LDS instead).
+template <class P, typename R, int TI, int TJ>
+__host__ __device__ __forceinline__ void phase_prefetch(int tid, int i0, int j0, const StageArgs& a,
+                                                        double qa[P::MEQN], double qb[P::MEQN])
+{
+    const int ci = tid / TJ, cj = tid - ci * TJ;
+    const int gi = i0 + ci, gj = j0 + cj;
+    for (int m = 0; m < P::MEQN; m++) { qa[m] = 0.0; qb[m] = 0.0; }
+    if (ci >= TI || gi >= a.mx || gj >= a.my || !a.out) return;
+    const long long idx = (long long)gi * a.my + gj;
+    const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
+    if (!(exact_lds && a.a_is_src))
+        for (int m = 0; m < P::MEQN; m++) qa[m] = a.qa[m * a.plane + idx];
+    if (a.use_b && !(exact_lds && a.b_is_src))
+        for (int m = 0; m < P::MEQN; m++) qb[m] = a.qb[m * a.plane + idx];
 }
 
 // One face: characteristic WENO-LLF flux (lib/2d/ConstructL.cpp:63-208 for
@@ -279,7 +318,8 @@ __host__ __device__ __forceinline__ void phase_lx(int tid, const StageArgs& a, c
 // L = Lx - (Ghat(j+1) - Ghat(j))/dy and the stage update in FP64.
 template <class P, typename R, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, const StageArgs& a,
-                                                      const StageSmem<P, R, TI, TJ>& sm, const double Lx[P::MEQN])
+                                                      const StageSmem<P, R, TI, TJ>& sm, const double Lx[P::MEQN],
+                                                      const double pa[P::MEQN], const double pb[P::MEQN])
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
@@ -294,10 +334,10 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
         const double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
         if (!a.out) continue;
-        const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : a.qa[m * a.plane + idx];
+        const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : pa[m];
         double v = a.a1 * qa;
         if (a.use_b) {
-            const double qb = (exact_lds && a.b_is_src) ? double(sm.cv[m][c]) : a.qb[m * a.plane + idx];
+            const double qb = (exact_lds && a.b_is_src) ? double(sm.cv[m][c]) : pb[m];
             v = v + a.a2 * qb;
         }
         a.out[m * a.plane + idx] = v + a.bdt * L;

@@ -84,8 +84,9 @@ __device__ __forceinline__ void routed_faces(int t, int dir, int i0, int j0, con
     __syncthreads();
     const int nt = rs.nt[dir], ns = rs.ns[dir];
     if (t == 0 && a.route_counts) {
-        atomicAdd(&a.route_counts[0], (unsigned long long)nt);
-        atomicAdd(&a.route_counts[1], (unsigned long long)(nt + ns));
+        const int b = blockIdx.x % CFL_SPREAD;
+        atomicAdd(&a.route_counts[b], (unsigned long long)nt);
+        atomicAdd(&a.route_counts[CFL_SPREAD + b], (unsigned long long)(nt + ns));
     }
     int face = -1;
     bool full = true;
@@ -121,6 +122,8 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
     const int t = threadIdx.x;
     if (!EXACT) mlp_stage_frags(a.frags, &rs.fr, t, NT);
     if (t < 2) { rs.nt[t] = 0; rs.ns[t] = 0; }  // ordered before the appends by the load barrier
+    double pa[P::MEQN], pb[P::MEQN];
+    phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
     phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
     __syncthreads();
     double ax = 0.0, ay = 0.0;
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
     __syncthreads();
     routed_faces<P, R, WK, TI, TJ, EXACT>(t, 1, i0, j0, a, sm, rs, ay);
     __syncthreads();
-    phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx);
+    phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
     if (a.smax) {
         if (ax != ax) ax = __builtin_inf();
         if (ay != ay) ay = __builtin_inf();
@@ -145,8 +148,9 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
                 mx_ = red[0][k] > mx_ ? red[0][k] : mx_;
                 my_ = red[1][k] > my_ ? red[1][k] : my_;
             }
-            atomicMax(&a.smax[0], (unsigned long long)__double_as_longlong(mx_));
-            atomicMax(&a.smax[1], (unsigned long long)__double_as_longlong(my_));
+            const int b = blockIdx.x % CFL_SPREAD;
+            atomicMax(&a.smax[b], (unsigned long long)__double_as_longlong(mx_));
+            atomicMax(&a.smax[CFL_SPREAD + b], (unsigned long long)__double_as_longlong(my_));
         }
     }
 }

@@ -1,0 +1,92 @@
+"""Training pipeline of the learned troubled-interface indicator (no
+counterpart in the reference).
+
+The network is trained on the GPU by the library's own kernels
+(``finess_mlp_*``): interfaces are harvested from solver states of the 2D
+Euler four-quadrant Riemann problem (snapshots along its run) and of a smooth
+2D Euler density wave (the smooth-advection case carried over to the Euler
+system), each labelled by whether the JS nonlinear weights depart from the
+linear weights by more than ``tau``.  The logit threshold is then chosen on a
+held-out snapshot (another resolution, another time) for a target recall of
+the troubled class, so that missing a troubled interface is rare: the full
+characteristic WENO remains the fallback for everything the network flags.
+"""
+import time
+
+import numpy as np
+
+from . import Mlp, Solver, problems
+
+# snapshot times of the four-quadrant problem used for training (t_final 0.3)
+TRAIN_TIMES = (0.0, 0.02, 0.05, 0.08, 0.12, 0.16, 0.2, 0.25, 0.3)
+
+
+class Indicator:
+    """A trained network: the FP32 parameter vector, the logit threshold and
+    what training measured."""
+
+    def __init__(self, params, threshold, info):
+        self.params = params
+        self.threshold = threshold
+        self.info = info
+
+    def install(self, solver):
+        solver.set_indicator(self.params, self.threshold)
+
+
+def _harvest_run(net, p, q, times, stride, tau):
+    s = Solver(p)
+    try:
+        s.set_state(q)
+        dtv = np.array([0.0, p.initial_dt, p.max_dt])
+        for t in times:
+            if t > 0:
+                s.advance(t, dtv)
+            net.harvest(s, stride=stride, tau=tau)
+    finally:
+        s.close()
+
+
+def train_indicator(mx=1024, tau=0.05, stride=2, epochs=6, batch=4096, lr=2e-3, recall=0.99, seed=3,
+                    times=TRAIN_TIMES, holdout=(768, 0.12)):
+    """Harvests about 1e7 labelled interfaces at ``mx``^2, trains on the GPU
+    and picks the threshold for ``recall`` on the held-out snapshot
+    ``holdout`` = (resolution, time).  Returns an :class:`Indicator`."""
+    t0 = time.perf_counter()
+    net = Mlp(seed=seed, capacity=1 << 24)
+    try:
+        p, q, _ = problems.riemann4(mx=mx)
+        _harvest_run(net, p, q, times, stride, tau)
+        pw, qw, _ = problems.density_wave(mx=max(64, mx // 2), amp=0.3)
+        _harvest_run(net, pw, qw, (0.0,), 1, tau)
+        n = net.n
+        _, y, _ = net.get_data()
+        pos = float(y.mean())
+        t1 = time.perf_counter()
+        loss = net.train(epochs=epochs, batch=batch, lr=lr)
+        t2 = time.perf_counter()
+        P = net.params
+        # held-out snapshot: threshold for the target recall, measured with
+        # the in-kernel inference routine
+        hp, hq, _ = problems.riemann4(mx=holdout[0])
+        hold = Mlp(seed=seed + 1, capacity=1 << 22)
+        try:
+            _harvest_run(hold, hp, hq, (holdout[1],), 1, tau)
+            Xh, yh, _ = hold.get_data()
+        finally:
+            hold.close()
+        zh = net.logits(Xh, path=1)
+    finally:
+        net.close()
+    zt = zh[yh > 0.5]
+    thr = float(np.quantile(zt, 1.0 - recall)) if len(zt) else 0.0
+    flagged = zh > thr
+    info = {
+        "interfaces": int(n), "troubled_label_fraction": round(pos, 4), "tau": tau,
+        "loss_per_epoch": [round(float(v), 5) for v in loss],
+        "harvest_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+        "holdout": {"mx": holdout[0], "t": holdout[1], "recall": round(float(np.mean(flagged[yh > 0.5])), 4),
+                    "flagged_fraction": round(float(flagged.mean()), 4),
+                    "label_fraction": round(float(yh.mean()), 4)},
+    }
+    return Indicator(P, thr, info)

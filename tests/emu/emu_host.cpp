@@ -44,7 +44,11 @@ void run_tile(const StageArgs& a, int i0, int j0, StageSmem<P, R, TI, TJ>& sm)
     for (int t = 0; t < NT; t++) phase_faces<P, R, WK, LXW, TI, TJ>(t, 0, i0, j0, a, sm, ax);
     for (int t = 0; t < NT; t++) phase_lx<P, R, TI, TJ>(t, a, sm, &Lx[(size_t)t * P::MEQN]);
     for (int t = 0; t < NT; t++) phase_faces<P, R, WK, LXW, TI, TJ>(t, 1, i0, j0, a, sm, ay);
-    for (int t = 0; t < NT; t++) phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, &Lx[(size_t)t * P::MEQN]);
+    for (int t = 0; t < NT; t++) {
+        double pa[P::MEQN], pb[P::MEQN];
+        phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
+        phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, &Lx[(size_t)t * P::MEQN], pa, pb);
+    }
 }
 
 template <class P, typename R, int WK, bool LXW = false>

@@ -1,22 +1,39 @@
 #!/usr/bin/env python3
-"""Benchmark of the FINESS WENO path on one MI355X.
+"""Benchmark of the FINESS WENO path on one MI355X: learned+FP32 against
+plain FP64 at equal L1 density error.
 
-Workload (one GPU): 2D Euler four-quadrant Riemann problem, 1024 x 1024 cells,
-gamma 1.4, characteristic-wise WENO5-JS (eps 1e-6, p 2) with local
-Lax-Friedrichs splitting, SSP-RK3, a-posteriori CFL controller at CFL 0.4,
-transmissive boundaries, plain FP64.  A "step" is one accepted time step
-(three fused stage launches plus the controller's read-back of the wave-speed
-maxima).  Inputs are resident in HBM before the timed region starts.
+Workload (one GPU): 2D Euler four-quadrant Riemann problem on [0,1]^2,
+1024 x 1024 cells, gamma 1.4, characteristic-wise WENO5-JS (eps 1e-6, p 2)
+with local Lax-Friedrichs splitting, SSP-RK3, a-posteriori CFL controller at
+CFL 0.4, transmissive boundaries, t_final 0.3.  Two forms of the same solver:
 
-Prints one JSON line.  `value` is Mcell-updates/s = cells * steps / seconds.
-`roofline` refers to the dominant kernel (the fused stage kernel): achieved =
-algorithmic HBM bytes per launch / average launch duration, the duration
-measured with HIP events on the stream the kernel runs on.  `cpu_baseline` is
-the CPU restatement of the reference (oracle/) timed on this host's cores on a
-bounded sample of the same workload.
+  plain FP64    full characteristic WENO at every interface, FP64 throughout;
+  learned+FP32  the troubled-interface MLP (24-32-32-1, BF16 on the matrix
+                cores, FP32 accumulation; trained on the GPU before the timed
+                region on interfaces harvested from this problem's run and a
+                smooth density wave) routes smooth interfaces to the
+                component-wise linear FD5 flux and the rest to full
+                characteristic WENO; FP32 stencil arithmetic, FP64 state,
+                stage accumulation and pressure recovery.  The indicator runs
+                at the first stage of each step; the later stages reuse it.
+
+A "step" is one accepted time step (three fused stage launches and the
+controller's read-back of the wave-speed maxima).  Both forms are timed over
+K steps from the state at t = 0.15 (the middle of the run) after W warm-up
+steps; inputs are resident in HBM before the timed region starts.  The L1
+density error of each form at t_final is measured against a 2048 x 2048
+plain-FP64 run averaged onto the 1024 grid.
+
+Prints one JSON line.  `value` is the learned+FP32 Mcell-updates/s
+(cells * steps / seconds); the plain FP64 figures and the error ratio sit
+beside it.  `roofline` refers to the dominant kernel of the learned form (the
+routed stage kernel, all three stages of a step): achieved = algorithmic HBM
+bytes per launch / average launch duration, measured with HIP events on the
+stream the kernel runs on.  `cpu_baseline` is the CPU restatement of the
+reference (oracle/) timed on this host's cores on a bounded sample of the
+same workload.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -28,13 +45,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_UPDATE = 320.0   # SSP-RK3: 8 state passes x 5 fields x 8 B
+BYTES_PER_CELL_UPDATE = 320.0   # SSP-RK3: 8 state passes x 5 fields x 8 B (FP64 state in both forms)
 STAGES_PER_STEP = 3
+T_FINAL = 0.3
+T_TIMED = 0.15
 
 
 def cpu_baseline(mx, steps, threads):
     """Times the oracle's fixed-step loop (no controller) on `steps` steps of
-    the same workload at full size."""
+    the plain FP64 form of the workload at full size."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc  # the oracle loader lives with the tests
     from finess_amd import problems
@@ -50,8 +69,54 @@ def cpu_baseline(mx, steps, threads):
         "unit": "Mcell-updates/s",
         "cores": int(lib.orc_num_threads()),
         "kind": "port",
-        "sample": "%d SSP-RK3 step(s) of the %dx%d four-quadrant problem, FP64, fixed dt, %.1f s" % (steps, mx, mx, el),
+        "sample": "%d SSP-RK3 step(s) of the %dx%d four-quadrant problem, plain FP64, fixed dt, %.1f s"
+                  % (steps, mx, mx, el),
     }
+
+
+def make_solver(form, mx, indicator=None):
+    from finess_amd import PREC_FP32, ROUTE_LEARNED, Solver, problems
+    kw = {} if form == "plain" else dict(precision=PREC_FP32, routing=ROUTE_LEARNED)
+    p, q, _ = problems.riemann4(mx=mx, **kw)
+    s = Solver(p)
+    s.set_state(q)
+    if indicator is not None:
+        indicator.install(s)
+    return p, s
+
+
+def full_run(form, mx, indicator=None):
+    """Runs to t_final; returns the interior density, wall seconds and steps."""
+    p, s = make_solver(form, mx, indicator)
+    dtv = np.array([0.0, p.initial_dt, p.max_dt])
+    s.sync()
+    t0 = time.perf_counter()
+    st = s.advance(T_FINAL, dtv)
+    s.sync()
+    el = time.perf_counter() - t0
+    route = s.route_stats() if indicator is not None else None
+    q, _ = s.get_state()
+    s.close()
+    g = p.mbc
+    return q[g:g + mx, g:g + mx, 0].copy(), el, int(st.steps_accepted), route
+
+
+def timed_steps(form, mx, steps, warmup, indicator=None):
+    p, s = make_solver(form, mx, indicator)
+    dtv = np.array([0.0, p.initial_dt, p.max_dt])
+    s.advance(T_TIMED, dtv)
+    s.advance_steps(warmup, dtv)
+    s.sync()
+    t0 = time.perf_counter()
+    st = s.advance_steps(steps, dtv)
+    s.sync()
+    el = time.perf_counter() - t0
+    # dominant kernel: average stage-launch duration over back-to-back steps,
+    # HIP events on the context's stream
+    s.run_fixed_steps(float(dtv[1]), 30)
+    ms, launches = s.last_kernel_time()
+    s.close()
+    return el, int(st.steps_rejected), ms / 1e3 / launches
 
 
 def main():
@@ -62,55 +127,73 @@ def main():
     ap.add_argument("--mx", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the t_final runs and the L1 errors")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU slab decomposition is a design, not code)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    from finess_amd.indicator import train_indicator
 
-    from finess_amd import Solver, problems
-    p, q, _ = problems.riemann4(mx=args.mx)
-    cells = p.mx * p.my
-    s = Solver(p)
-    s.set_state(q)
-    dtv = np.array([0.0, p.initial_dt, p.max_dt])
-    s.advance_steps(args.warmup, dtv)
-    s.sync()
-    t0 = time.perf_counter()
-    st = s.advance_steps(args.steps, dtv)
-    s.sync()
-    el = time.perf_counter() - t0
-    value = cells * args.steps / el / 1e6
+    mx = args.mx
+    cells = mx * mx
+    ind = train_indicator(mx=mx)
 
-    # dominant kernel: average duration of the fused stage launches, timed
-    # with HIP events on the context's stream over back-to-back steps
-    dt_fixed = dtv[1]
-    s.run_fixed_steps(dt_fixed, 50)
-    ms, launches = s.last_kernel_time()
-    avg_launch_s = ms / 1e3 / launches
+    # throughput over K steps from the mid-run state
+    el_p, rej_p, launch_p = timed_steps("plain", mx, args.steps, args.warmup)
+    el_l, rej_l, launch_l = timed_steps("learned", mx, args.steps, args.warmup, ind)
+    v_plain = cells * args.steps / el_p / 1e6
+    v_learned = cells * args.steps / el_l / 1e6
+
+    accuracy = None
+    if not args.no_accuracy:
+        rho_ref, el_r, n_r, _ = full_run("plain", 2 * mx)
+        rho_ref = 0.25 * (rho_ref[0::2, 0::2] + rho_ref[1::2, 0::2] + rho_ref[0::2, 1::2] + rho_ref[1::2, 1::2])
+        rho_p, el_fp, n_p, _ = full_run("plain", mx)
+        rho_l, el_fl, n_l, route = full_run("learned", mx, ind)
+        l1_p = float(np.mean(np.abs(rho_p - rho_ref)))
+        l1_l = float(np.mean(np.abs(rho_l - rho_ref)))
+        accuracy = {
+            "reference": "plain FP64 at %dx%d to t=%.2f, 2x2-averaged" % (2 * mx, 2 * mx, T_FINAL),
+            "l1_density_plain_fp64": l1_p,
+            "l1_density_learned_fp32": l1_l,
+            "l1_ratio": round(l1_l / l1_p, 5),
+            "within_5_percent": bool(l1_l <= 1.05 * l1_p),
+            "l1_learned_vs_plain": float(np.mean(np.abs(rho_l - rho_p))),
+            "troubled_fraction": round(route[0] / max(route[1], 1), 4),
+            "full_run_mcups": {"plain_fp64": round(cells * n_p / el_fp / 1e6, 1),
+                               "learned_fp32": round(cells * n_l / el_fl / 1e6, 1),
+                               "steps": [n_p, n_l]},
+        }
+
     bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / STAGES_PER_STEP
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    s.close()
-
+    achieved = bytes_per_launch / launch_l / 1e9
     out = {
         "metric": "Mcell-updates/s, 2D Euler WENO5: learned+FP32 vs plain FP64 at equal L1 error",
-        "value": round(value, 3),
+        "value": round(v_learned, 3),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "ms_per_step": round(el_l / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": "2D Euler four-quadrant Riemann, %dx%d, char-WENO5-JS + LLF, SSP-RK3, CFL 0.4, "
-                        "transmissive, plain FP64" % (p.mx, p.my),
-            "mx": p.mx, "my": p.my, "meqn": 5, "form": "plain-fp64",
-            "steps_rejected": int(st.steps_rejected), "dt_last": float(dtv[1]),
+                        "transmissive, t_final %.1f; learned MLP routing + FP32 stencil vs plain FP64"
+                        % (mx, mx, T_FINAL),
+            "mx": mx, "my": mx, "meqn": 5, "timed_from_t": T_TIMED,
+            "steps_rejected": {"plain_fp64": rej_p, "learned_fp32": rej_l},
         },
+        "plain_fp64": {"value": round(v_plain, 3), "ms_per_step": round(el_p / args.steps * 1e3, 4),
+                       "avg_stage_launch_us": round(launch_p * 1e6, 2),
+                       "hbm_frac": round(bytes_per_launch / launch_p / 1e9 / HBM_PEAK_GBS, 4)},
+        "speedup_vs_plain_fp64": round(v_learned / v_plain, 4),
+        "indicator": ind.info,
+        "accuracy": accuracy,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -118,14 +201,14 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_stage2d<Euler2D,double,JS5>",
-            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "kernel": "k_stage2d_routed<Euler2D,float,JS5> (indicator stage and the two reusing stages)",
+            "avg_launch_us": round(launch_l * 1e6, 2),
             "bytes_per_launch": bytes_per_launch,
         },
     }
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(args.mx, args.cpu_steps, threads)
+        out["cpu_baseline"] = cpu_baseline(mx, args.cpu_steps, threads)
     print(json.dumps(out), flush=True)
 
 

@@ -119,6 +119,28 @@ def timed_steps(form, mx, steps, warmup, indicator=None):
     return el, int(st.steps_rejected), ms / 1e3 / launches
 
 
+def batched_shallow_water(n=512, steps=200):
+    """512 independent periodic shallow-water problems (grids drawn from
+    {64,...,384}^2), WENO5 SSP-RK3, FP32 stencil + FP64 state, per-problem
+    CFL controller, one persistent launch per stage in a HIP graph, tile
+    shape from the autotuner."""
+    from finess_amd import PREC_FP32, Batch, problems
+    draws = problems.sw_batch_members(n=n)
+    sizes = [(d[0], d[1]) for d in draws]
+    mem = [problems.sw_bump(nx=nx, ny=ny, A=A, cx=cx, cy=cy, s=s, precision=PREC_FP32)[:2]
+           for nx, ny, A, cx, cy, s in draws]
+    with Batch(mem[0][0], sizes) as b:
+        for k, (p, q) in enumerate(mem):
+            b.set_state(k, q, 0.0, p.initial_dt)
+        tile = b.autotune(5)
+        sec = b.run(steps)
+        rej = sum(b.get_state(k)[4] for k in range(n))
+    cells = sum(a * c for a, c in sizes)
+    return {"value": round(cells * steps / sec / 1e6, 1), "unit": "Mcell-updates/s", "problems": n,
+            "steps_per_problem": steps, "cells": cells, "seconds": round(sec, 4), "tile": list(tile),
+            "steps_rejected": int(rej)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +150,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the t_final runs and the L1 errors")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched shallow-water workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -206,6 +229,8 @@ def main():
             "bytes_per_launch": bytes_per_launch,
         },
     }
+    if not args.no_batch:
+        out["batched_shallow_water"] = batched_shallow_water()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(mx, args.cpu_steps, threads)

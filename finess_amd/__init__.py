@@ -150,6 +150,16 @@ SIGNATURES = {
     "finess_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _PF]),
     "finess_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, _PF, _PF, ctypes.c_int, _PF, _PF]),
     "finess_mlp_eval": (ctypes.c_int, [ctypes.c_void_p, _PF, ctypes.c_longlong, ctypes.c_int, _PF]),
+    "finess_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p)]),
+    "finess_batch_destroy": (None, [ctypes.c_void_p]),
+    "finess_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "finess_batch_set_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _PD, ctypes.c_double, ctypes.c_double]),
+    "finess_batch_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _PD, _PD, _PD,
+                                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "finess_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "finess_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _PD]),
 }
 
 
@@ -403,3 +413,74 @@ class Mlp:
         z = np.zeros(len(X), np.float32)
         self._check(self.lib.finess_mlp_eval(self.h, _fptr(X), len(X), int(path), _fptr(z)), "eval")
         return z
+
+
+class Batch:
+    """Many independent 2D problems (``finess_batch``): same physics, scheme
+    and controller settings as ``params``, each on the same domain with its
+    own grid; all advanced together by persistent launches captured in a HIP
+    graph, every problem under its own CFL controller."""
+
+    def __init__(self, params, sizes):
+        self.lib = load_library()
+        self.p = params.copy()
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        n = len(self.sizes)
+        mx = (ctypes.c_int * n)(*[a for a, _ in self.sizes])
+        my = (ctypes.c_int * n)(*[b for _, b in self.sizes])
+        h = ctypes.c_void_p()
+        rc = self.lib.finess_batch_create(ctypes.byref(self.p), n, mx, my, ctypes.byref(h))
+        if rc:
+            raise FinessError(rc, "finess_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.finess_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise FinessError(rc, "%s: %s" % (what, self.lib.finess_batch_last_error(self.h).decode()))
+
+    def shape_q(self, k):
+        g = self.p.mbc
+        mx, my = self.sizes[k]
+        return (mx + 2 * g, my + 2 * g, self.p.meqn)
+
+    def set_state(self, k, q, t=0.0, dt=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.shape == self.shape_q(k)
+        dt = self.p.initial_dt if dt is None else dt
+        self._check(self.lib.finess_batch_set_state(self.h, int(k), _ptr(q), float(t), float(dt)), "set_state")
+
+    def get_state(self, k):
+        q = np.zeros(self.shape_q(k))
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        acc, rej = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.finess_batch_get_state(self.h, int(k), _ptr(q), ctypes.byref(t), ctypes.byref(dt),
+                                                    ctypes.byref(acc), ctypes.byref(rej)), "get_state")
+        return q, t.value, dt.value, acc.value, rej.value
+
+    def autotune(self, trial_launches=5):
+        ti, tj = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.finess_batch_autotune(self.h, int(trial_launches), ctypes.byref(ti), ctypes.byref(tj)),
+                    "autotune")
+        return ti.value, tj.value
+
+    def run(self, steps):
+        sec = ctypes.c_double()
+        self._check(self.lib.finess_batch_run(self.h, int(steps), ctypes.byref(sec)), "run")
+        return sec.value

@@ -213,6 +213,30 @@ int  finess_mlp_gradient(finess_mlp* mlp, const float* X, const float* y, int n,
 /* Logits of n interfaces: layer GEMMs (path 0) or the in-kernel wave routine (path 1). */
 int  finess_mlp_eval(finess_mlp* mlp, const float* X, long long n, int path, float* logits);
 
+/* ---- batched mode (no counterpart in the reference) ------------------------
+ * Many independent 2D problems of uneven grid sizes (same physics, scheme and
+ * controller settings, each on the base domain with its own mx x my) advanced
+ * together: every stage of every problem is one persistent launch over the
+ * tiles of all problems, each problem keeps its own a-posteriori CFL
+ * controller on the device (accept/reject and dt from its own wave speeds),
+ * and the step sequence is captured once in a HIP graph and replayed.  The
+ * tile shape is chosen by finess_batch_autotune.  RK time stepping, 2D. */
+typedef struct finess_batch finess_batch;
+
+int  finess_batch_create(const finess_params* base, int nprob, const int* mx, const int* my, finess_batch** out);
+void finess_batch_destroy(finess_batch* b);
+const char* finess_batch_last_error(finess_batch* b);
+/* Problem k's state (reference layout of its own mx x my grid), time and first dt. */
+int  finess_batch_set_state(finess_batch* b, int k, const double* q, double t, double dt);
+int  finess_batch_get_state(finess_batch* b, int k, double* q, double* t, double* dt, long long* accepted,
+                            long long* rejected);
+/* Times `trial_launches` stage launches for each candidate tile shape and
+ * keeps the fastest (the state is not changed). */
+int  finess_batch_autotune(finess_batch* b, int trial_launches, int* tile_i, int* tile_j);
+/* Advances every problem by `steps` accepted steps (rejected ones are retried
+ * with the controller's smaller dt); *seconds = wall time of the run. */
+int  finess_batch_run(finess_batch* b, int steps, double* seconds);
+
 /* ---- library info --------------------------------------------------------- */
 const char* finess_version(void);
 int finess_device_count(void);

@@ -141,6 +141,19 @@ def batched_shallow_water(n=512, steps=200):
             "steps_rejected": int(rej)}
 
 
+def copy_calibration(mib=256, reps=5):
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +164,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the t_final runs and the L1 errors")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shallow-water workload")
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU slab decomposition is a design, not code)" % args.gpus, file=sys.stderr)
@@ -160,7 +180,7 @@ def main():
 
     mx = args.mx
     cells = mx * mx
-    ind = train_indicator(mx=mx)
+    ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride)
 
     # throughput over K steps from the mid-run state
     el_p, rej_p, launch_p = timed_steps("plain", mx, args.steps, args.warmup)

@@ -56,6 +56,9 @@ struct finess_ctx {
     unsigned char* route_mask = nullptr;          // per-face routing of the current step
     bool route_fresh = true;                      // next routed stage starts a step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> tev;  // per-launch timing marks (run_fixed_steps)
+    size_t tev_used = 0;
+    bool timing = false;
     double t = 0.0;
     int cur = 0;
     bool have_state = false;
@@ -145,6 +148,20 @@ Stage1DArgs base_stage1d_args(finess_ctx* c)
 // stepping: the stage belongs to a time step (the first stage of a step runs
 // the indicator and records its routing for the later ones when
 // route_per_step is set); otherwise a standalone ConstructL.
+// Records the next timing mark on the stream (stage launches are bracketed
+// by two marks while run_fixed_steps measures them).
+int tmark(finess_ctx* c)
+{
+    if (!c->timing) return 0;
+    if (c->tev_used == c->tev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return fail(c, FINESS_ERR_HIP, "event creation failed");
+        c->tev.push_back(e);
+    }
+    if (hipEventRecord(c->tev[c->tev_used++], c->st) != hipSuccess) return fail(c, FINESS_ERR_HIP, "event record failed");
+    return 0;
+}
+
 int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
 {
     const finess_params& p = c->p;
@@ -179,7 +196,9 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.a2 = a2;
         a.bdt = beta * dt;
         a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
+        if (tmark(c)) return FINESS_ERR_HIP;
         const int rc = launch_stage1d(weno_kind(p), a, c->st);
+        if (tmark(c)) return FINESS_ERR_HIP;
         return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D stage launch failed") : 0;
     }
     StageArgs a = base_stage_args(c);
@@ -199,8 +218,15 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.lxw_F = c->lxwF;
         a.lxw_G = c->lxwG;
     }
-    if (p.routing != FINESS_ROUTE_NONE) return stage_routed(c, a, true);
-    const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), lxw, a, c->st);
+    if (tmark(c)) return FINESS_ERR_HIP;
+    int rc;
+    if (p.routing != FINESS_ROUTE_NONE) {
+        rc = stage_routed(c, a, true);
+        if (!rc && tmark(c)) return FINESS_ERR_HIP;
+        return rc;
+    }
+    rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), lxw, a, c->st);
+    if (tmark(c)) return FINESS_ERR_HIP;
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
 }
 
@@ -500,6 +526,7 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->route_counts);
     hipFree(c->route_mask);
     if (c->smax_h) hipHostFree(c->smax_h);
+    for (hipEvent_t e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->st) hipStreamDestroy(c->st);
@@ -777,19 +804,25 @@ int finess_run_fixed_steps(finess_ctx* c, double dt, int nsteps, double* last_cf
     if (!c || nsteps < 0) return FINESS_ERR_ARG;
     if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
     int res = c->cur, slot = 0, rc;
-    const int nst = c->p.time_stepping_method == FINESS_TS_LXW ? 1 : c->p.time_order;
-    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    // every stage launch is bracketed by two events: the figure is kernel
+    // time only, without the launch gaps and the slot resets between stages
+    c->timing = true;
+    c->tev_used = 0;
     for (int n = 0; n < nsteps; n++) {
-        if ((rc = enqueue_step(c, dt, &res, &slot))) return rc;
+        if ((rc = enqueue_step(c, dt, &res, &slot))) { c->timing = false; return rc; }
         c->cur = res;
         c->t += dt;
     }
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timing = false;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double ms = 0.0;
+    for (size_t k = 0; k + 1 < c->tev_used; k += 2) {
+        float e = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&e, c->tev[k], c->tev[k + 1]));
+        ms += e;
+    }
     c->last_ms = ms;
-    c->last_launches = (long long)nsteps * nst;
+    c->last_launches = (long long)(c->tev_used / 2);
     if (last_cfl && nsteps > 0) {
         double cfl = 0.0;
         if ((rc = read_cfl(c, slot, dt, 1e300, &cfl))) return rc;

@@ -177,8 +177,9 @@ int finess_run_fixed_steps(finess_ctx* ctx, double dt, int nsteps, double* last_
 /* Device synchronisation of the context's stream. */
 int finess_sync(finess_ctx* ctx);
 
-/* Time of the last kernel-timed region (ms), measured with HIP events on the
- * context's stream, and the number of stage launches it covered. */
+/* Kernel time of the stage launches of the last finess_run_fixed_steps call
+ * (ms, each launch bracketed by HIP events on the context's stream) and the
+ * number of launches. */
 int finess_last_kernel_time(finess_ctx* ctx, double* ms, long long* launches);
 
 /* ---- troubled-interface indicator (no counterpart in the reference) ------ */

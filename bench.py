@@ -47,6 +47,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 320.0   # SSP-RK3: 8 state passes x 5 fields x 8 B (FP64 state in both forms)
 STAGES_PER_STEP = 3
+# HBM bytes per stage launch at 1024x1024 measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
+# counts in profiles/k_stage2d_routed_euler_fp32_riemann4_1024.md and
+# profiles/k_stage2d_euler_fp64_riemann4_1024.md)
+TRAFFIC_ROUTED_1024 = 116.6e6
+TRAFFIC_PLAIN_1024 = 114.2e6
 T_FINAL = 0.3
 T_TIMED = 0.15
 
@@ -233,7 +239,8 @@ def main():
         },
         "plain_fp64": {"value": round(v_plain, 3), "ms_per_step": round(el_p / args.steps * 1e3, 4),
                        "avg_stage_launch_us": round(launch_p * 1e6, 2),
-                       "hbm_frac": round(bytes_per_launch / launch_p / 1e9 / HBM_PEAK_GBS, 4)},
+                       "hbm_frac": round(bytes_per_launch / launch_p / 1e9 / HBM_PEAK_GBS, 4),
+                       "traffic": TRAFFIC_PLAIN_1024 if mx == 1024 else None},
         "speedup_vs_plain_fp64": round(v_learned / v_plain, 4),
         "indicator": ind.info,
         "accuracy": accuracy,
@@ -243,7 +250,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_ROUTED_1024 if mx == 1024 else None,
             "kernel": "k_stage2d_routed<Euler2D,float,JS5> (indicator stage and the two reusing stages)",
             "avg_launch_us": round(launch_l * 1e6, 2),
             "bytes_per_launch": bytes_per_launch,

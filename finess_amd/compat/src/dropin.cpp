@@ -301,6 +301,21 @@ finess_ctx* context()
     g_ctx = nullptr;
     rc = finess_create(&p, &g_ctx);
     if (rc) fail_like_reference(rc, "finess_create failed");
+    if (p.routing == FINESS_ROUTE_LEARNED) {
+        // [gpu] indicator: the trained network, FINESS_MLP_NPARAM float32
+        // parameters followed by the float32 logit threshold
+        const std::string path = global_ini_params.get_gpu("indicator", "");
+        std::vector<float> w(FINESS_MLP_NPARAM + 1);
+        FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "rb");
+        const size_t got = f ? std::fread(w.data(), sizeof(float), w.size(), f) : 0;
+        if (f) std::fclose(f);
+        if (got != w.size())
+            throw std::logic_error("finess-hip: gpu.routing = learned needs gpu.indicator = <file of " +
+                                   std::to_string(FINESS_MLP_NPARAM + 1) + " float32 values>; could not read '" +
+                                   path + "'");
+        rc = finess_set_indicator(g_ctx, w.data(), w[FINESS_MLP_NPARAM]);
+        if (rc) fail_like_reference(rc, finess_last_error(g_ctx));
+    }
     g_params = p;
     return g_ctx;
 }

@@ -33,6 +33,19 @@ class Indicator:
     def install(self, solver):
         solver.set_indicator(self.params, self.threshold)
 
+    def save(self, path):
+        """Writes the network for the drop-in layer's ``[gpu] indicator``:
+        the 1889 float32 parameters, then the float32 logit threshold."""
+        np.concatenate([np.asarray(self.params, np.float32),
+                        np.float32([self.threshold])]).astype("<f4").tofile(path)
+
+    @classmethod
+    def load(cls, path):
+        w = np.fromfile(path, dtype="<f4")
+        if w.size != 1890:
+            raise ValueError("%s: expected 1890 float32 values, found %d" % (path, w.size))
+        return cls(w[:1889].copy(), float(w[1889]), {"loaded_from": str(path)})
+
 
 def _harvest_run(net, p, q, times, stride, tau):
     s = Solver(p)

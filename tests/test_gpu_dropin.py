@@ -74,3 +74,28 @@ def test_kelvin_helmholtz_lxw_app(tmp_path):
     err = orc.rel_linf(q1, orc.interior(p, q0))
     print("LxW app vs oracle", err)
     assert np.all(err <= 1e-10), err
+
+
+def test_euler_quadrants_app_learned(tmp_path):
+    """The same app with `[gpu] routing = learned, precision = fp32` and a
+    network trained here and written with Indicator.save: it runs through the
+    drop-in layer and stays close to the plain result (the routing is decided
+    by the network, so the comparison is an error bound, not parity)."""
+    from finess_amd.indicator import train_indicator
+    ind = train_indicator(mx=128, epochs=2, stride=1, times=(0.0, 0.02, 0.05), holdout=(96, 0.03))
+    ind.save(str(tmp_path / "net.bin"))
+    ini = open(os.path.join(APPS, "euler_quadrants", "parameters.ini")).read()
+    ini = ini.replace("precision = fp64", "precision = fp32\nrouting   = learned\nindicator = net.bin")
+    (tmp_path / "parameters.ini").write_text(ini)
+    r = subprocess.run([os.path.join(APPS, "euler_quadrants", "finess.exe")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    p, q, aux = problems.riemann4(mx=64, initial_dt=1e-3, max_dt=1.0, nv=500000)
+    t0, q0 = oracle_frames(p, q, aux, 2, 0.02)
+    t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
+    assert abs(t1 - t0) < 1e-6
+    rho0 = orc.interior(p, q0)[..., 0]
+    d = np.abs(q1[..., 0] - rho0)
+    print("learned app vs plain oracle: L1", d.mean(), "Linf", d.max())
+    assert d.mean() < 2e-3 * rho0.mean() and d.max() < 0.1 * rho0.max()

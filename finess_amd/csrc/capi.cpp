@@ -53,7 +53,12 @@ struct finess_ctx {
     unsigned long long* smax_h = nullptr;  // pinned copy
     MlpFrags* frags = nullptr;                    // device copy of the installed network
     unsigned long long* route_counts = nullptr;   // device [2][CFL_SPREAD]
-    unsigned char* route_mask = nullptr;          // per-face routing of the current step
+    // routing record of the current step: per-face slot (-1 smooth), troubled
+    // list, its length, and the troubled faces' fluxes of the current stage
+    int* route_idx = nullptr;
+    int* route_list = nullptr;
+    unsigned long long* route_n = nullptr;
+    double* route_flux = nullptr;
     bool route_fresh = true;                      // next routed stage starts a step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;  // per-launch timing marks (run_fixed_steps)
@@ -172,9 +177,14 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
     a.route_exact = p.routing == FINESS_ROUTE_ORACLE;
     a.route_tau = p.route_tau;
     a.route_counts = c->route_counts;
-    if (stepping && p.route_per_step && c->route_mask) {
-        a.route_mask = c->route_mask;
+    if (stepping && p.route_per_step && c->route_idx) {
+        a.route_idx = c->route_idx;
+        a.route_list = c->route_list;
+        a.route_n = c->route_n;
+        a.route_flux = c->route_flux;
         a.route_mask_read = c->route_fresh ? 0 : 1;
+        if (c->route_fresh && launch_fill_u64(c->route_n, 1, 0ull, c->st))
+            return fail(c, FINESS_ERR_HIP, "routing record reset failed");
         c->route_fresh = false;
     }
     const int rc = launch_stage2d_routed(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), a, c->st);
@@ -496,8 +506,13 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
     }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
-    if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2)
-        rc = alloc(c, (void**)&c->route_mask, (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1));
+    if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2) {
+        const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
+        rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
+        if (!rc) rc = alloc(c, (void**)&c->route_list, nf * sizeof(int));
+        if (!rc) rc = alloc(c, (void**)&c->route_n, sizeof(unsigned long long));
+        if (!rc) rc = alloc(c, (void**)&c->route_flux, nf * 6 * sizeof(double));
+    }
     if (!rc) rc = launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (!rc) rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (rc) {
@@ -524,7 +539,10 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->smax);
     hipFree(c->frags);
     hipFree(c->route_counts);
-    hipFree(c->route_mask);
+    hipFree(c->route_idx);
+    hipFree(c->route_list);
+    hipFree(c->route_n);
+    hipFree(c->route_flux);
     if (c->smax_h) hipHostFree(c->smax_h);
     for (hipEvent_t e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);

@@ -164,32 +164,39 @@ __host__ __device__ __forceinline__ R face_departure(const R (*rec)[P::NV], int 
 // The FD5 weights (2, -13, 47, 27, -3)/60 on f + la q over cells 0..4 and on
 // f - la q over cells 5..1 fold into one symmetric stencil on f and one
 // antisymmetric stencil on q:
-//   fhat = [(f0+f5) - 8(f1+f4) + 37(f2+f3)]/60 + la [(q0-q5) - 5(q1-q4) + 10(q2-q3)]/60.
+//   fhat = [f0 - 8f1 + 37f2 + 37f3 - 8f4 + f5]/60 + la [q0 - 5q1 + 10q2 - 10q3 + 5q4 - q5]/60.
 template <class P, typename R, int NC>
 __host__ __device__ __forceinline__ R face_flux_linear(const R (*cv)[NC], int base, int stride, int dir,
                                                        const PhysParams& pp, R ascale, R fh[P::MEQN])
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    R q[6][M], f[6][M], cl[NV], cr[NV];
+    // accumulated cell by cell, so only the sums and one cell stay live
+    constexpr float A[6] = {1.f, -8.f, 37.f, 37.f, -8.f, 1.f};
+    constexpr float B[6] = {1.f, -5.f, 10.f, -10.f, 5.f, -1.f};
+    R fa[M], qb[M], cl[NV], cr[NV], ql[M], qr[M];
 #pragma unroll
     for (int s = 0; s < 6; s++) {
-        R rec[NV];
+        R rec[NV], q[M], f[M];
         for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
-        P::state(rec, dir, q[s]);
-        P::flux(q[s], rec, f[s]);
-        if (s == 2)
+        P::state(rec, dir, q);
+        P::flux(q, rec, f);
+        for (int k = 0; k < M; k++) {
+            fa[k] = s == 0 ? f[k] : fa[k] + R(A[s]) * f[k];
+            qb[k] = s == 0 ? q[k] : qb[k] + R(B[s]) * q[k];
+        }
+        if (s == 2) {
             for (int v = 0; v < NV; v++) cl[v] = rec[v];
-        if (s == 3)
+            for (int k = 0; k < M; k++) ql[k] = q[k];
+        }
+        if (s == 3) {
             for (int v = 0; v < NV; v++) cr[v] = rec[v];
+            for (int k = 0; k < M; k++) qr[k] = q[k];
+        }
     }
-    const R alpha = P::wave_speed(q[2], q[3], cl, cr, pp);
+    const R alpha = P::wave_speed(ql, qr, cl, cr, pp);
     const R la = ascale * alpha;
     const R k60 = R(1) / R(60);
-    for (int k = 0; k < M; k++) {
-        const R fa = (f[0][k] + f[5][k]) - R(8) * (f[1][k] + f[4][k]) + R(37) * (f[2][k] + f[3][k]);
-        const R qb = (q[0][k] - q[5][k]) - R(5) * (q[1][k] - q[4][k]) + R(10) * (q[2][k] - q[3][k]);
-        fh[P::rot_q(k, dir)] = (fa + la * qb) * k60;
-    }
+    for (int k = 0; k < M; k++) fh[P::rot_q(k, dir)] = (fa[k] + la * qb[k]) * k60;
     return alpha;
 }
 

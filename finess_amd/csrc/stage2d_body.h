@@ -88,10 +88,16 @@ struct StageArgs {
     int route_exact;
     double route_tau;
     unsigned long long* route_counts;  // [2][CFL_SPREAD]: troubled faces, faces
-    // per-face routing flags, x faces [(mx+1) my] then y faces [mx (my+1)]:
-    // written by the indicator when route_mask_read is 0, else read instead
-    // of running the indicator (null: indicator only, nothing stored)
-    unsigned char* route_mask;
+    // per-step routing record (null: indicator only, nothing stored).  Face
+    // ids: x faces i*my + j, then y faces (mx+1)*my + i*(my+1) + j.
+    //   route_idx[face]  -1 for a smooth face, else its slot in the troubled list
+    //   route_list[slot] face id; route_n: number of slots
+    //   route_flux[slot] 5 flux components + alpha of the troubled face
+    // route_mask_read: 0 = the indicator decides and records, 1 = read the record
+    int* route_idx;
+    int* route_list;
+    unsigned long long* route_n;
+    double* route_flux;
     int route_mask_read;
     // optional face-flux / L output for the drop-in ConstructL (may be null):
     double* Lout;          // L planes (mx*my)

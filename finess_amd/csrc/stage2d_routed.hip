@@ -1,18 +1,19 @@
-// stage2d_routed.hip -- the fused stage kernel with troubled-interface
-// routing (no counterpart in the reference).  Same tile, halo, stage update
-// and CFL reduction as k_stage2d; the face work of both directions is done
-// in one pass:
-//   1. every face of the tile (x faces then y faces, 487 for a 7 x 32 tile)
-//      gets a routing decision: the MLP on the matrix cores (each wave
-//      evaluates its 64 x faces, then its y faces), the exact weight-departure
-//      test, or the decision recorded by the first stage of the time step;
-//   2. the workgroup compacts the troubled faces of both directions to the
-//      front of one list and the smooth ones to the back, so the threads run
-//      full characteristic WENO5 on list entries 0..nt-1 and the
-//      component-wise linear flux on the rest (each thread takes two
-//      entries): at most one wave per tile runs both paths;
-//   3. the fluxes land in per-direction face buffers, and the update forms
-//      L from both.
+// stage2d_routed.hip -- the stage kernels of troubled-interface routing (no
+// counterpart in the reference).  A face is routed to full characteristic
+// WENO5 (troubled) or to the component-wise linear FD5 flux (smooth).
+//
+// The first stage of a step (k_stage2d_routed) decides: the MLP on the matrix
+// cores for every face of the tile (or the exact weight-departure test), then
+// a workgroup compaction puts the troubled faces of both directions at the
+// front of one list so that at most one wave per tile runs both paths; it
+// records the decisions as a compact list of troubled faces.
+//
+// The later stages reuse the record in two launches: k_route_full computes
+// the characteristic fluxes of the listed faces only (every lane busy, no
+// tile), and k_stage2d_lookup runs the tile with the linear flux at smooth
+// faces and the precomputed flux at troubled ones.  Without the
+// characteristic path the lookup kernel needs far fewer registers, so more
+// workgroups stay resident to hide the memory latency.
 #include <hip/hip_runtime.h>
 
 #include "indicator.h"
@@ -71,10 +72,30 @@ __device__ __forceinline__ void stencil_records(const StageSmem<P, R, TI, TJ>& s
         for (int v = 0; v < P::NV; v++) rec[k][v] = s.cv[v][g.base + k * g.stride];
 }
 
-// EXACT: route by the weight-departure label itself; EVAL: decide (and
-// record when a.route_mask is set), else read this step's record.
+// Owner of a face shared by two tiles: the tile for which it is not the
+// far edge (the domain's last face belongs to the only tile that has it).
+__device__ __forceinline__ bool face_owned(int dir, const FaceGeom& g, int i0, int j0, int TI, int TJ,
+                                           const StageArgs& a)
+{
+    return dir == 0 ? (g.fi < TI || i0 + g.fi == a.mx) : (g.fj < TJ || j0 + g.fj == a.my);
+}
+
+// Records a decision: troubled faces get a slot in the list.
+__device__ __forceinline__ void record_face(long long mi, bool troubled, const StageArgs& a)
+{
+    if (troubled) {
+        const int slot = (int)atomicAdd(a.route_n, 1ull);
+        a.route_list[slot] = (int)mi;
+        a.route_idx[mi] = slot;
+    } else {
+        a.route_idx[mi] = -1;
+    }
+}
+
+// EXACT: route by the weight-departure label itself, else by the MLP.  With
+// a.route_idx set, the decisions are recorded for the step's later stages.
 template <class P, typename R, int WK, int TI, int TJ, bool EXACT, bool EVAL>
-__global__ __launch_bounds__(256, (sizeof(R) == 4 && !EVAL) ? 4 : 1) void k_stage2d_routed(StageArgs a)
+__global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
 {
     constexpr int NT = 256, M = P::MEQN, NV = P::NV;
     using T = RouteTile<P, R, TI, TJ>;
@@ -88,11 +109,8 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && !EVAL) ? 4 : 1) void k_stag
 
     const FaceGeom gx = face_geom<TI, TJ>(t, 0, i0, j0, a);
     const FaceGeom gy = face_geom<TI, TJ>(t, 1, i0, j0, a);
+    static_assert(EVAL, "the reusing stages run k_route_full + k_stage2d_lookup");
     bool tx = false, ty = false;
-    if (!EVAL) {  // this step's routing, read before the tile so the latencies overlap
-        if (gx.valid) tx = a.route_mask[mask_index(0, gx, i0, j0, a)] != 0;
-        if (gy.valid) ty = a.route_mask[mask_index(1, gy, i0, j0, a)] != 0;
-    }
     double pa[M], pb[M];
     phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
     if (t == 0) { sm.nt = 0; sm.ns = 0; }
@@ -129,9 +147,9 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && !EVAL) ? 4 : 1) void k_stag
             }
         }
     }
-    if (EVAL && a.route_mask) {
-        if (gx.valid) a.route_mask[mask_index(0, gx, i0, j0, a)] = tx ? 1 : 0;
-        if (gy.valid) a.route_mask[mask_index(1, gy, i0, j0, a)] = ty ? 1 : 0;
+    if (a.route_idx) {
+        if (gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a)) record_face(mask_index(0, gx, i0, j0, a), tx, a);
+        if (gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a)) record_face(mask_index(1, gy, i0, j0, a), ty, a);
     }
     // compaction: troubled faces to the front of the list, smooth to the back
     if (gx.valid) {
@@ -222,6 +240,125 @@ __global__ __launch_bounds__(256, (sizeof(R) == 4 && !EVAL) ? 4 : 1) void k_stag
     }
 }
 
+// Characteristic WENO5 fluxes of the listed troubled faces of this stage's
+// source state (grid-stride over the list; the stencil is gathered through
+// the boundary map, the records formed as in the tile load).
+template <class P, typename R, int WK>
+__global__ __launch_bounds__(256) void k_route_full(StageArgs a)
+{
+    constexpr int M = P::MEQN, NV = P::NV;
+    const long long n = (long long)*a.route_n;
+    const long long nfx = (long long)(a.mx + 1) * a.my;
+    const WenoParams<R> wp{R(a.eps), R(a.power)};
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        const long long f = a.route_list[k];
+        int dir, gi, gj;
+        if (f < nfx) { dir = 0; gi = (int)(f / a.my); gj = (int)(f - (long long)gi * a.my); }
+        else { const long long h = f - nfx; dir = 1; gi = (int)(h / (a.my + 1)); gj = (int)(h - (long long)gi * (a.my + 1)); }
+        R cv[NV][6];
+        for (int s = 0; s < 6; s++) {
+            double q[M], rec[NV];
+            if (dir == 0) load_state<P>(a, gi - 3 + s, gj, q);
+            else load_state<P>(a, gi, gj - 3 + s, q);
+            P::cell_record(q, a.pp, rec);
+            for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
+        }
+        R fh[M];
+        const R alpha = face_flux<P, R, WK, false, 6>(cv, 0, 1, dir, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh);
+        double* out = a.route_flux + 6 * k;
+        for (int m = 0; m < M; m++) out[m] = double(fh[m]);
+        out[M] = double(alpha);
+    }
+}
+
+// One stage with the step's recorded routing: the linear flux at smooth
+// faces, the flux k_route_full computed at troubled ones.
+template <class P, typename R, int TI, int TJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 3))) void k_stage2d_lookup(StageArgs a)
+{
+    constexpr int NT = 256, M = P::MEQN;
+    using S = StageSmem<P, R, TI, TJ>;
+    static_assert((TI + 1) * TJ <= NT && TI * (TJ + 1) <= NT, "one face per thread and direction");
+    __shared__ S sm;
+    __shared__ R fy[M][TI * (TJ + 1)];
+    __shared__ double red[2][NT / 64];
+    const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
+    const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
+    const int i0 = ti * TI, j0 = tj * TJ;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const FaceGeom gx = face_geom<TI, TJ>(t, 0, i0, j0, a);
+    const FaceGeom gy = face_geom<TI, TJ>(t, 1, i0, j0, a);
+    // the step's routing and the update operands, read before the tile load
+    const int sx = gx.valid ? a.route_idx[mask_index(0, gx, i0, j0, a)] : -1;
+    const int sy = gy.valid ? a.route_idx[mask_index(1, gy, i0, j0, a)] : -1;
+    double pa[M], pb[M];
+    phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
+    phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
+    __syncthreads();
+    double ax = 0.0, ay = 0.0;
+    for (int dir = 0; dir < 2; dir++) {
+        const FaceGeom& h = dir ? gy : gx;
+        const int slot = dir ? sy : sx;
+        if (!h.valid) continue;
+        R fh[M];
+        R alpha;
+        if (slot >= 0) {
+            const double* in = a.route_flux + 6 * (long long)slot;
+            for (int m = 0; m < M; m++) fh[m] = R(in[m]);
+            alpha = R(in[M]);
+        } else {
+            alpha = face_flux_linear<P, R, S::NC>(sm.cv, h.base, h.stride, dir, a.pp, R(a.alpha_scaling), fh);
+        }
+        if (dir) {
+            for (int m = 0; m < M; m++) fy[m][t] = fh[m];
+            if (h.count) ay = rmax(ay, double(alpha));
+        } else {
+            for (int m = 0; m < M; m++) sm.face[m][t] = fh[m];
+            if (h.count) ax = rmax(ax, double(alpha));
+        }
+    }
+    __syncthreads();
+    {
+        const int ci = t / TJ, cj = t - ci * TJ;
+        const int gi = i0 + ci, gj = j0 + cj;
+        if (ci < TI && gi < a.mx && gj < a.my) {
+            const long long idx = (long long)gi * a.my + gj;
+            const int c = (ci + 3) * S::HJ + (cj + 3);
+            const bool exact_lds = sizeof(R) == sizeof(double);
+            const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+            for (int m = 0; m < M; m++) {
+                const double Lx = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
+                const double L = Lx - (double(fy[m][ci * (TJ + 1) + cj + 1]) - double(fy[m][ci * (TJ + 1) + cj])) * idy_;
+                const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : pa[m];
+                double v = a.a1 * qa;
+                if (a.use_b) {
+                    const double qb = (exact_lds && a.b_is_src) ? double(sm.cv[m][c]) : pb[m];
+                    v = v + a.a2 * qb;
+                }
+                a.out[m * a.plane + idx] = v + a.bdt * L;
+            }
+        }
+    }
+    if (a.smax) {
+        if (ax != ax) ax = __builtin_inf();
+        if (ay != ay) ay = __builtin_inf();
+        ax = wave_max_r(ax);
+        ay = wave_max_r(ay);
+        if (lane == 0) { red[0][w] = ax; red[1][w] = ay; }
+        __syncthreads();
+        if (t == 0) {
+            double mx_ = red[0][0], my_ = red[1][0];
+            for (int k = 1; k < NT / 64; k++) {
+                mx_ = red[0][k] > mx_ ? red[0][k] : mx_;
+                my_ = red[1][k] > my_ ? red[1][k] : my_;
+            }
+            const int b = blockIdx.x % CFL_SPREAD;
+            atomicMax(&a.smax[b], (unsigned long long)__double_as_longlong(mx_));
+            atomicMax(&a.smax[CFL_SPREAD + b], (unsigned long long)__double_as_longlong(my_));
+        }
+    }
+}
+
 template <typename R>
 static int launch_routed(const StageArgs& a0, hipStream_t st)
 {
@@ -230,9 +367,10 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
     const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
     a.ntj = ntj;
     const dim3 grid(nti * ntj), block(256);
-    if (a.route_mask_read)
-        hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, false, false>), grid, block, 0, st, a);
-    else if (a.route_exact)
+    if (a.route_mask_read) {
+        hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(1024), block, 0, st, a);
+        hipLaunchKernelGGL((k_stage2d_lookup<Euler2D, R, TI, TJ>), grid, block, 0, st, a);
+    } else if (a.route_exact)
         hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, true, true>), grid, block, 0, st, a);
     else
         hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, false, true>), grid, block, 0, st, a);
@@ -242,7 +380,7 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
 int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st)
 {
     if (physics != PHYS_EULER || wk != WENO_JS5_P2) return -1;
-    if (a.route_mask_read && !a.route_mask) return -1;
+    if (a.route_mask_read && !(a.route_idx && a.route_list && a.route_n && a.route_flux && a.out)) return -1;
     return fp32 ? launch_routed<float>(a, st) : launch_routed<double>(a, st);
 }
 

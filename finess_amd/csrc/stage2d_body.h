@@ -213,36 +213,37 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     // L is linear, so L(f + la q) and L(f - la q) are projected instead, and
     // the 1/2 is applied after the reconstruction: every WENO5 variant
     // satisfies W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.
-    // Each stencil point is projected as soon as it is read, so only the
-    // projected values stay live.
-    R gp[M][5], gm[M][5];
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-        R rec[NV], q[M], f[M];
-        for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
-        P::state(rec, dir, q);
-        if (LXW) {
-            for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
-        } else {
-            P::flux(q, rec, f);
-        }
-        R vp[M], vm[M], t[M];
-        for (int k = 0; k < M; k++) {
-            vp[k] = f[k] + la * q[k];
-            vm[k] = f[k] - la * q[k];
-        }
-        if (s < 5) {
-            P::left(e, vp, t);
-            for (int k = 0; k < M; k++) gp[k][s] = t[k];
-        }
-        if (s > 0) {
-            P::left(e, vm, t);
-            for (int k = 0; k < M; k++) gm[k][5 - s] = t[k];
-        }
-    }
+    // The two split parts are projected and reconstructed one after the
+    // other (cells 0..4, then cells 5..1): only one part's 25 projected
+    // values are live at a time, at the price of forming the flux of cells
+    // 1..4 twice.
     const WenoParams<R> wp4{R(4) * wp.eps, wp.power};
     R gh[M];
-    for (int k = 0; k < M; k++) gh[k] = R(0.5) * (weno5<WK>(gp[k], wp4) + weno5<WK>(gm[k], wp4));
+#pragma unroll 1
+    for (int part = 0; part < 2; part++) {
+        const R sg = part ? R(-1) : R(1);
+        R g[M][5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            const int s = part ? 5 - n : n;  // stencil point; g[.][n] is its position in the reconstruction
+            R rec[NV], q[M], f[M];
+            for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
+            P::state(rec, dir, q);
+            if (LXW) {
+                for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
+            } else {
+                P::flux(q, rec, f);
+            }
+            R vs[M], t[M];
+            for (int k = 0; k < M; k++) vs[k] = f[k] + (sg * la) * q[k];
+            P::left(e, vs, t);
+            for (int k = 0; k < M; k++) g[k][n] = t[k];
+        }
+        for (int k = 0; k < M; k++) {
+            const R w = weno5<WK>(g[k], wp4);
+            gh[k] = part ? R(0.5) * (gh[k] + w) : w;
+        }
+    }
     R fr[M];
     P::right(e, gh, fr);
     for (int k = 0; k < M; k++) fh[P::rot_q(k, dir)] = fr[k];

@@ -219,8 +219,10 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     // 1..4 twice.
     const WenoParams<R> wp4{R(4) * wp.eps, wp.power};
     R gh[M];
-#pragma unroll 1
-    for (int part = 0; part < 2; part++) {
+    // (5-field systems reading a tile: a rolled loop keeps the compiler from
+    // interleaving the two parts, which would make both live again; a
+    // register-resident 6-point stencil needs static indices instead)
+    auto part_flux = [&](int part) {
         const R sg = part ? R(-1) : R(1);
         R g[M][5];
 #pragma unroll
@@ -243,6 +245,13 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
             const R w = weno5<WK>(g[k], wp4);
             gh[k] = part ? R(0.5) * (gh[k] + w) : w;
         }
+    };
+    if constexpr (M >= 5 && NC > 6) {
+#pragma unroll 1
+        for (int part = 0; part < 2; part++) part_flux(part);
+    } else {
+        part_flux(0);
+        part_flux(1);
     }
     R fr[M];
     P::right(e, gh, fr);

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_route_full(StageArgs a)
 // One stage with the step's recorded routing: the linear flux at smooth
 // faces, the flux k_route_full computed at troubled ones.
 template <class P, typename R, int TI, int TJ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 3))) void k_stage2d_lookup(StageArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 3))) void k_stage2d_lookup(StageArgs a)
 {
     constexpr int NT = 256, M = P::MEQN;
     using S = StageSmem<P, R, TI, TJ>;

@@ -60,6 +60,7 @@ struct finess_ctx {
     unsigned long long* route_n = nullptr;
     double* route_flux = nullptr;
     bool route_fresh = true;                      // next routed stage starts a step
+    int rec_age = -1;                             // steps since the record was decided (-1: none)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;  // per-launch timing marks (run_fixed_steps)
     size_t tev_used = 0;
@@ -182,9 +183,12 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
         a.route_list = c->route_list;
         a.route_n = c->route_n;
         a.route_flux = c->route_flux;
-        a.route_mask_read = c->route_fresh ? 0 : 1;
-        if (c->route_fresh && launch_fill_u64(c->route_n, 1, 0ull, c->st))
-            return fail(c, FINESS_ERR_HIP, "routing record reset failed");
+        bool decide = c->route_fresh && (c->rec_age < 0 || c->rec_age >= p.route_per_step);
+        a.route_mask_read = decide ? 0 : 1;
+        if (decide) {
+            if (launch_fill_u64(c->route_n, 1, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
+            c->rec_age = 0;
+        }
         c->route_fresh = false;
     }
     const int rc = launch_stage2d_routed(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), a, c->st);
@@ -270,6 +274,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
     int rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st);
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
+    if (c->rec_age >= 0 && c->route_fresh == false) c->rec_age++;  // the previous step used the record
     c->route_fresh = true;
     if (p.time_stepping_method == FINESS_TS_LXW) {
         if ((rc = lxw_fluxes(c, A, dt))) return rc;
@@ -575,6 +580,7 @@ int finess_set_state(finess_ctx* c, const double* q, const double* aux, double t
         HIPCHK(c, hipStreamSynchronize(c->st));
     }
     c->t = t;
+    c->rec_age = -1;  // a new state has no routing record yet
     c->have_state = true;
     return FINESS_OK;
 }

@@ -101,9 +101,10 @@ typedef struct finess_params {
     int    routing;                     /* FINESS_ROUTE_* */
     double route_tau;                   /* label threshold on max_k |w_k - d_k| */
     int    tile_i, tile_j;              /* 0 = library default */
-    int    route_per_step;              /* 1: the indicator runs at the first stage of each
-                                           time step and its routing serves the step's later
-                                           stages; 0: it runs at every stage */
+    int    route_per_step;              /* k >= 1: the indicator runs at the first stage of
+                                           every k-th time step and its routing serves the
+                                           later stages of that step and the next k-1 steps;
+                                           0: it runs at every stage */
 } finess_params;
 
 /* Fills defaults exactly as the reference's IniParams defaults do. */

@@ -1166,7 +1166,9 @@ int orc_run_fixed_steps_routed(const finess_params* p, double* q, double* aux, d
         memset(smax, 0, smax_numel(p) * sizeof(double));
         unsigned char* m = per_step ? mask : NULL;
         orc_set_bnd(p, q, aux);
-        rc = construct_l_routed_mask(p, q, aux, 2, tau, m, 0, L, smax, NULL);
+        /* the first stage decides at every per_step-th step, else reuses */
+        const int reuse = per_step >= 1 && n % per_step != 0;
+        rc = construct_l_routed_mask(p, q, aux, 2, tau, m, reuse, L, smax, NULL);
         if (rc) break;
         update_soln(numel, RK_A1[k][0], RK_A2[k][0], RK_B[k][0], dt, q, L, qstar);
         orc_set_bnd(p, qstar, aux);

@@ -79,8 +79,9 @@ double orc_face_flux_linear(const finess_params* p, int ixy, const double* qs, d
  * where orc_face_departure > tau else linear.  *n_troubled counts faces. */
 int orc_construct_l_routed(const finess_params* p, const double* q, const double* aux, int mode, double tau,
                            double* Lstar, double* smax, long long* n_troubled);
-/* Fixed SSP-RK3 steps with mode-2 routing; per_step: the first stage of
- * each step decides the routing and the later stages reuse it. */
+/* Fixed SSP-RK3 steps with mode-2 routing.  per_step = k >= 1: the first
+ * stage of every k-th step decides the routing, which the later stages of
+ * that step and the next k-1 steps reuse; 0: decided at every stage. */
 int orc_run_fixed_steps_routed(const finess_params* p, double* q, double* aux, double dt, int nsteps, double tau,
                                int per_step, double* cfl);
 /* MLP 24-32-32-1 (parameter layout of include/finess_hip.h's finess_mlp),

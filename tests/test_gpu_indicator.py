@@ -127,10 +127,11 @@ def test_routed_exact_matches_oracle():
     assert np.all(err <= 1e-12)
 
 
-@pytest.mark.parametrize("per_step", [0, 1])
+@pytest.mark.parametrize("per_step", [0, 1, 2])
 def test_routed_stepping_matches_oracle(per_step):
-    """SSP-RK3 steps with exact-departure routing decided at every stage or
-    once per step (recorded at the first stage, reused by the later ones)."""
+    """SSP-RK3 steps with exact-departure routing decided at every stage, once
+    per step (recorded at the first stage, reused by the later ones), or once
+    every two steps."""
     p, q, _ = problems.riemann4(mx=64, my=72, routing=ROUTE_ORACLE, route_tau=0.05, route_per_step=per_step)
     dt = 0.3 / 72 / 2.5
     q_ref = q.copy()

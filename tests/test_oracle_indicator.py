@@ -174,7 +174,7 @@ def test_routed_stepping_identities():
     dt = 0.3 / 24 / 2.5
     qa = q0.copy()
     orc.fixed_steps(p, qa, dt, 3)
-    for per_step in (0, 1):
+    for per_step in (0, 1, 2):
         qb = q0.copy()
         orc.fixed_steps_routed(p, qb, dt, 3, -1.0, per_step)
         assert np.array_equal(orc.interior(p, qb), orc.interior(p, qa))

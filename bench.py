@@ -52,7 +52,7 @@ STAGES_PER_STEP = 3
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
 # counts in profiles/k_stage2d_routed_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 116.6e6
+TRAFFIC_ROUTED_1024 = 150.8e6   # mean over the 12 stages of a 4-step indicator interval
 TRAFFIC_PLAIN_1024 = 114.2e6
 T_FINAL = 0.3
 T_TIMED = 0.15
@@ -256,7 +256,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": TRAFFIC_ROUTED_1024 if mx == 1024 else None,
-            "kernel": "k_stage2d_routed<Euler2D,float,JS5> (indicator stage and the two reusing stages)",
+            "kernel": "learned+FP32 stage: k_stage2d_routed (indicator stage, every 4th step) or "
+                      "k_route_full + k_stage2d_lookup (all other stages); mean over a 4-step interval",
             "avg_launch_us": round(launch_l * 1e6, 2),
             "bytes_per_launch": bytes_per_launch,
         },

@@ -186,7 +186,7 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
         bool decide = c->route_fresh && (c->rec_age < 0 || c->rec_age >= p.route_per_step);
         a.route_mask_read = decide ? 0 : 1;
         if (decide) {
-            if (launch_fill_u64(c->route_n, 1, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
+            if (launch_fill_u64(c->route_n, 2, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
             c->rec_age = 0;
         }
         c->route_fresh = false;
@@ -458,6 +458,12 @@ int finess_params_finalize(finess_params* p)
         (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || weno_kind(*p) != WENO_JS5_P2 ||
          p->time_stepping_method != FINESS_TS_RK))
         return FINESS_ERR_CONFIG;
+    // the routing record packs a face's cell indices in 16 bits each and
+    // numbers the faces of both directions with 32-bit integers
+    if (p->routing != FINESS_ROUTE_NONE &&
+        (p->mx > 65535 || p->my > 65535 ||
+         (long long)(p->mx + 1) * p->my + (long long)p->mx * (p->my + 1) >= (1ll << 31)))
+        return FINESS_ERR_CONFIG;
     for (int k = 0; k < 4; k++)
         if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_REFLECT) return FINESS_ERR_CONFIG;
     return FINESS_OK;
@@ -515,7 +521,7 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
         rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
         if (!rc) rc = alloc(c, (void**)&c->route_list, nf * sizeof(int));
-        if (!rc) rc = alloc(c, (void**)&c->route_n, sizeof(unsigned long long));
+        if (!rc) rc = alloc(c, (void**)&c->route_n, 2 * sizeof(unsigned long long));
         if (!rc) rc = alloc(c, (void**)&c->route_flux, nf * 6 * sizeof(double));
     }
     if (!rc) rc = launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st) ? FINESS_ERR_HIP : 0;

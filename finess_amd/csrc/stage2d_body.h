@@ -91,7 +91,9 @@ struct StageArgs {
     // per-step routing record (null: indicator only, nothing stored).  Face
     // ids: x faces i*my + j, then y faces (mx+1)*my + i*(my+1) + j.
     //   route_idx[face]  -1 for a smooth face, else its slot in the troubled list
-    //   route_list[slot] face id; route_n: number of slots
+    //   route_list[slot] the face's (i, j) packed as i << 16 | j; troubled x
+    //                    faces take slots 0.., y faces (mx+1)*my..
+    //   route_n[2]       number of troubled x faces, y faces
     //   route_flux[slot] 5 flux components + alpha of the troubled face
     // route_mask_read: 0 = the indicator decides and records, 1 = read the record
     int* route_idx;
@@ -185,16 +187,15 @@ __host__ __device__ __forceinline__ void phase_prefetch(int tid, int i0, int j0,
         for (int m = 0; m < P::MEQN; m++) qb[m] = a.qb[m * a.plane + idx];
 }
 
-// One face: characteristic WENO-LLF flux (lib/2d/ConstructL.cpp:63-208 for
-// x, :215-358 for y).  base/stride locate the 6 stencil records i-3..i+2.
-template <class P, typename R, int WK, bool LXW, int NC>
-__host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int stride, int dir,
-                                                const double* lflux, long long lbase, long long lstride,
-                                                long long lplane, const PhysParams& pp,
-                                                const WenoParams<R>& wp, R ascale, R fh[P::MEQN])
+// Mean state of a face from the two cells beside it, its eigen-system and
+// its Lax-Friedrichs speed (lib/2d/ConstructL.cpp:72-77 and 149-174 for x,
+// :224-229 and 300-325 for y); returns alpha and sets la = ascale * alpha.
+template <class P, typename R, int NC>
+__host__ __device__ __forceinline__ R face_setup(const R (*cv)[NC], int base, int stride, int dir,
+                                                 const PhysParams& pp, R ascale,
+                                                 typename P::template Eig<R>& e, R& la)
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    // mean state and wave speed from the two cells beside the face
     R cl[NV], cr[NV], ql[M], qr[M];
     for (int v = 0; v < NV; v++) {
         cl[v] = cv[v][base + 2 * stride];
@@ -204,48 +205,72 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     P::state(cr, dir, qr);
     R qa[M];
     for (int k = 0; k < M; k++) qa[k] = R(0.5) * (qr[k] + ql[k]);
-    typename P::template Eig<R> e;
     P::eig(qa, pp, e);
     const R alpha = P::wave_speed(ql, qr, cl, cr, pp);
-    const R la = ascale * alpha;
-    // Lax-Friedrichs split projected in one go: the reference forms
-    // gp = (L f + la L q)/2 and gm = (L f - la L q)/2 (ConstructL.cpp:177-185);
-    // L is linear, so L(f + la q) and L(f - la q) are projected instead, and
-    // the 1/2 is applied after the reconstruction: every WENO5 variant
-    // satisfies W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.
-    // The two split parts are projected and reconstructed one after the
-    // other (cells 0..4, then cells 5..1): only one part's 25 projected
-    // values are live at a time, at the price of forming the flux of cells
-    // 1..4 twice.
+    la = ascale * alpha;
+    return alpha;
+}
+
+// One Lax-Friedrichs split part of a face flux, projected and reconstructed
+// (lib/2d/ConstructL.cpp:138-192 for x, :289-342 for y).  The reference forms gp = (L f + la L q)/2
+// and gm = (L f - la L q)/2; L is linear, so L(f + la q) (part 0, stencil
+// cells 0..4) and L(f - la q) (part 1, cells 5..1) are projected instead and
+// the 1/2 is applied after the reconstruction: every WENO5 variant satisfies
+// W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.  w[k] receives
+// twice the reconstructed characteristic flux of field k.
+template <class P, typename R, int WK, bool LXW, int NC>
+__host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, int stride, int dir,
+                                                   const double* lflux, long long lbase, long long lstride,
+                                                   long long lplane, const typename P::template Eig<R>& e,
+                                                   R la, const WenoParams<R>& wp4, int part, R w[P::MEQN])
+{
+    constexpr int M = P::MEQN, NV = P::NV;
+    const R sg = part ? R(-1) : R(1);
+    R g[M][5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        const int s = part ? 5 - n : n;  // stencil point; g[.][n] is its position in the reconstruction
+        R rec[NV], q[M], f[M];
+        for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
+        P::state(rec, dir, q);
+        if (LXW) {
+            for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
+        } else {
+            P::flux(q, rec, f);
+        }
+        R vs[M], t[M];
+        for (int k = 0; k < M; k++) vs[k] = f[k] + (sg * la) * q[k];
+        P::left(e, vs, t);
+        for (int k = 0; k < M; k++) g[k][n] = t[k];
+    }
+    for (int k = 0; k < M; k++) w[k] = weno5<WK>(g[k], wp4);
+}
+
+// One face: characteristic WENO-LLF flux (lib/2d/ConstructL.cpp:63-208 for
+// x, :215-358 for y).  base/stride locate the 6 stencil records i-3..i+2.
+template <class P, typename R, int WK, bool LXW, int NC>
+__host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int stride, int dir,
+                                                const double* lflux, long long lbase, long long lstride,
+                                                long long lplane, const PhysParams& pp,
+                                                const WenoParams<R>& wp, R ascale, R fh[P::MEQN])
+{
+    constexpr int M = P::MEQN;
+    typename P::template Eig<R> e;
+    R la;
+    const R alpha = face_setup<P, R, NC>(cv, base, stride, dir, pp, ascale, e, la);
     const WenoParams<R> wp4{R(4) * wp.eps, wp.power};
+    // The two split parts are reconstructed one after the other: only one
+    // part's 25 projected values are live at a time, at the price of forming
+    // the flux of cells 1..4 twice.
     R gh[M];
+    auto part_flux = [&](int part) {
+        R w[M];
+        face_part<P, R, WK, LXW, NC>(cv, base, stride, dir, lflux, lbase, lstride, lplane, e, la, wp4, part, w);
+        for (int k = 0; k < M; k++) gh[k] = part ? R(0.5) * (gh[k] + w[k]) : w[k];
+    };
     // (5-field systems reading a tile: a rolled loop keeps the compiler from
     // interleaving the two parts, which would make both live again; a
     // register-resident 6-point stencil needs static indices instead)
-    auto part_flux = [&](int part) {
-        const R sg = part ? R(-1) : R(1);
-        R g[M][5];
-#pragma unroll
-        for (int n = 0; n < 5; n++) {
-            const int s = part ? 5 - n : n;  // stencil point; g[.][n] is its position in the reconstruction
-            R rec[NV], q[M], f[M];
-            for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
-            P::state(rec, dir, q);
-            if (LXW) {
-                for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
-            } else {
-                P::flux(q, rec, f);
-            }
-            R vs[M], t[M];
-            for (int k = 0; k < M; k++) vs[k] = f[k] + (sg * la) * q[k];
-            P::left(e, vs, t);
-            for (int k = 0; k < M; k++) g[k][n] = t[k];
-        }
-        for (int k = 0; k < M; k++) {
-            const R w = weno5<WK>(g[k], wp4);
-            gh[k] = part ? R(0.5) * (gh[k] + w) : w;
-        }
-    };
     if constexpr (M >= 5 && NC > 6) {
 #pragma unroll 1
         for (int part = 0; part < 2; part++) part_flux(part);

@@ -47,6 +47,7 @@ struct RouteTile {
     R fy[P::MEQN][NFY];         // y-face fluxes
     int list[NF];               // face ids: x face f -> f, y face f -> NFX + f
     int nt, ns;                 // troubled / smooth faces of the tile
+    int rcnt[2][4], rbase[2];   // recorded troubled faces per direction and wave, the tile's first slots
     double red[2][4];
 };
 
@@ -80,22 +81,21 @@ __device__ __forceinline__ bool face_owned(int dir, const FaceGeom& g, int i0, i
     return dir == 0 ? (g.fi < TI || i0 + g.fi == a.mx) : (g.fj < TJ || j0 + g.fj == a.my);
 }
 
-// Records a decision: troubled faces get a slot in the list.
-__device__ __forceinline__ void record_face(long long mi, bool troubled, const StageArgs& a)
+// Stores a troubled face in list slot `slot` of the step's record: the list
+// entry is the face's cell-index pair (gi, gj) packed in 16 bits each (the
+// host limits routed grids to 65535 cells per side).
+__device__ __forceinline__ void record_troubled(long long mi, int gi, int gj, int slot, const StageArgs& a)
 {
-    if (troubled) {
-        const int slot = (int)atomicAdd(a.route_n, 1ull);
-        a.route_list[slot] = (int)mi;
-        a.route_idx[mi] = slot;
-    } else {
-        a.route_idx[mi] = -1;
-    }
+    a.route_list[slot] = (int)(((unsigned)gi << 16) | (unsigned)gj);
+    a.route_idx[mi] = slot;
 }
 
 // EXACT: route by the weight-departure label itself, else by the MLP.  With
 // a.route_idx set, the decisions are recorded for the step's later stages.
+// The learned FP32 kernel is held to 3 waves per SIMD (168 registers
+// including the MFMA accumulators; its LDS allows 3 workgroups per CU).
 template <class P, typename R, int WK, int TI, int TJ, bool EXACT, bool EVAL>
-__global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : (sizeof(R) == 4 ? 3 : 2)))) void k_stage2d_routed(StageArgs a)
 {
     constexpr int NT = 256, M = P::MEQN, NV = P::NV;
     using T = RouteTile<P, R, TI, TJ>;
@@ -147,9 +147,17 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
             }
         }
     }
+    // the record: smooth owned faces are marked at once; the troubled ones
+    // are counted per wave (ballots) and get their list slots from one
+    // global atomic per tile and direction, taken below (x faces fill the
+    // list from slot 0, y faces from slot (mx+1) my)
+    const bool ox = a.route_idx && gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a);
+    const bool oy = a.route_idx && gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a);
+    const unsigned long long bx = __ballot(ox && tx), by = __ballot(oy && ty);
     if (a.route_idx) {
-        if (gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a)) record_face(mask_index(0, gx, i0, j0, a), tx, a);
-        if (gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a)) record_face(mask_index(1, gy, i0, j0, a), ty, a);
+        if (ox && !tx) a.route_idx[mask_index(0, gx, i0, j0, a)] = -1;
+        if (oy && !ty) a.route_idx[mask_index(1, gy, i0, j0, a)] = -1;
+        if (lane == 0) { sm.rcnt[0][w] = __popcll(bx); sm.rcnt[1][w] = __popcll(by); }
     }
     // compaction: troubled faces to the front of the list, smooth to the back
     if (gx.valid) {
@@ -162,6 +170,11 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
     }
     __syncthreads();
     const int nt = sm.nt, n = nt + sm.ns;
+    if (t < 2 && a.route_idx) {  // thread d takes the slots of direction d
+        int c = 0;
+        for (int k = 0; k < NT / 64; k++) c += sm.rcnt[t][k];
+        sm.rbase[t] = c ? (int)atomicAdd(&a.route_n[t], (unsigned long long)c) : 0;
+    }
     if (t == 0 && a.route_counts) {
         const int b = blockIdx.x % CFL_SPREAD;
         atomicAdd(&a.route_counts[b], (unsigned long long)nt);
@@ -195,6 +208,19 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
         }
     }
     __syncthreads();
+    if (bx | by) {  // the wave's troubled owned faces (the lane's own bits of the ballots)
+        int bsx = sm.rbase[0], bsy = (a.mx + 1) * a.my + sm.rbase[1];
+        for (int k = 0; k < w; k++) { bsx += sm.rcnt[0][k]; bsy += sm.rcnt[1][k]; }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if ((bx >> lane) & 1ull) {
+            const FaceGeom h = face_geom<TI, TJ>(t, 0, i0, j0, a);
+            record_troubled(mask_index(0, h, i0, j0, a), i0 + h.fi, j0 + h.fj, bsx + __popcll(bx & below), a);
+        }
+        if ((by >> lane) & 1ull) {
+            const FaceGeom h = face_geom<TI, TJ>(t, 1, i0, j0, a);
+            record_troubled(mask_index(1, h, i0, j0, a), i0 + h.fi, j0 + h.fj, bsy + __popcll(by & below), a);
+        }
+    }
     // L = -(Fhat(i+1) - Fhat(i))/dx - (Ghat(j+1) - Ghat(j))/dy and the stage
     // update, in the operation order of the plain kernel
     {
@@ -240,34 +266,60 @@ __global__ __launch_bounds__(256) void k_stage2d_routed(StageArgs a)
     }
 }
 
-// Characteristic WENO5 fluxes of the listed troubled faces of this stage's
-// source state (grid-stride over the list; the stencil is gathered through
-// the boundary map, the records formed as in the tile load).
-template <class P, typename R, int WK>
-__global__ __launch_bounds__(256) void k_route_full(StageArgs a)
+// Characteristic WENO5 flux of troubled face k of direction DIR (list
+// slots from slot0; lanes with k >= n idle).  The stencil of a face whose
+// six cells all lie inside the grid is read directly, others through the
+// boundary map; the records are formed as in the tile load.
+template <class P, typename R, int WK, int DIR>
+__device__ __forceinline__ void route_face(const StageArgs& a, long long k, long long n, long long slot0)
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    const long long n = (long long)*a.route_n;
-    const long long nfx = (long long)(a.mx + 1) * a.my;
-    const WenoParams<R> wp{R(a.eps), R(a.power)};
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
-        const long long f = a.route_list[k];
-        int dir, gi, gj;
-        if (f < nfx) { dir = 0; gi = (int)(f / a.my); gj = (int)(f - (long long)gi * a.my); }
-        else { const long long h = f - nfx; dir = 1; gi = (int)(h / (a.my + 1)); gj = (int)(h - (long long)gi * (a.my + 1)); }
-        R cv[NV][6];
+    if (k >= n) return;
+    const long long slot = slot0 + k;
+    const unsigned id = (unsigned)a.route_list[slot];
+    const int gi = (int)(id >> 16), gj = (int)(id & 0xffffu);
+    R cv[NV][6];
+    const bool inner = !a.src_ghosts && (DIR == 0 ? (gi >= 3 && gi + 2 < a.mx) : (gj >= 3 && gj + 2 < a.my));
+    if (inner) {
+        const int stride = DIR == 0 ? a.my : 1;
+        const int c0 = DIR == 0 ? (gi - 3) * a.my + gj : gi * a.my + (gj - 3);
         for (int s = 0; s < 6; s++) {
             double q[M], rec[NV];
-            if (dir == 0) load_state<P>(a, gi - 3 + s, gj, q);
+            for (int m = 0; m < M; m++) q[m] = (a.src + m * a.plane)[c0 + s * stride];
+            P::cell_record(q, a.pp, rec);
+            for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
+        }
+    } else {
+        for (int s = 0; s < 6; s++) {
+            double q[M], rec[NV];
+            if (DIR == 0) load_state<P>(a, gi - 3 + s, gj, q);
             else load_state<P>(a, gi, gj - 3 + s, q);
             P::cell_record(q, a.pp, rec);
             for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
         }
-        R fh[M];
-        const R alpha = face_flux<P, R, WK, false, 6>(cv, 0, 1, dir, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh);
-        double* out = a.route_flux + 6 * k;
-        for (int m = 0; m < M; m++) out[m] = double(fh[m]);
-        out[M] = double(alpha);
+    }
+    const WenoParams<R> wp{R(a.eps), R(a.power)};
+    R fh[M];
+    const R alpha = face_flux<P, R, WK, false, 6>(cv, 0, 1, DIR, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh);
+    double* out = a.route_flux + 6 * slot;
+    for (int m = 0; m < M; m++) out[m] = double(fh[m]);
+    out[M] = double(alpha);
+}
+
+// Characteristic WENO5 fluxes of the listed troubled faces of this stage's
+// source state, one face per lane.  The x faces are laid out over whole
+// waves first, then the y faces, so that every wave runs one direction
+// (a compile-time constant in route_face) without divergence.
+template <class P, typename R, int WK>
+__global__ __launch_bounds__(256) void k_route_full(StageArgs a)
+{
+    const long long nx = (long long)a.route_n[0], ny = (long long)a.route_n[1];
+    const long long px = (nx + 63) & ~63ll;
+    const long long nfx = (long long)(a.mx + 1) * a.my;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; (g & ~63ll) < px + ny; g += step) {
+        if (g < px) route_face<P, R, WK, 0>(a, g, nx, 0);
+        else route_face<P, R, WK, 1>(a, g - px, ny, nfx);
     }
 }
 

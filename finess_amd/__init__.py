@@ -75,6 +75,7 @@ class Params(ctypes.Structure):
         ("route_tau", ctypes.c_double),
         ("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int),
         ("route_per_step", ctypes.c_int),
+        ("host_controller", ctypes.c_int),
     ]
 
     def copy(self):

@@ -31,6 +31,7 @@
 #include "mlp.h"
 #include "stage1d_body.h"
 #include "stage2d_body.h"
+#include "step_control.h"
 
 using namespace finess;
 
@@ -59,6 +60,8 @@ struct finess_ctx {
     int* route_list = nullptr;
     unsigned long long* route_n = nullptr;
     double* route_flux = nullptr;
+    StepCtl* ctl = nullptr;                       // device controller block (2D)
+    bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
     int rec_age = -1;                             // steps since the record was decided (-1: none)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -186,7 +189,9 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
         bool decide = c->route_fresh && (c->rec_age < 0 || c->rec_age >= p.route_per_step);
         a.route_mask_read = decide ? 0 : 1;
         if (decide) {
-            if (launch_fill_u64(c->route_n, 2, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
+            const int frc = c->devctl ? launch_fill_u64_live(c->route_n, 2, 0ull, &c->ctl->halt, c->st)
+                                      : launch_fill_u64(c->route_n, 2, 0ull, c->st);
+            if (frc) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
             c->rec_age = 0;
         }
         c->route_fresh = false;
@@ -227,6 +232,11 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
     a.a2 = a2;
     a.bdt = beta * dt;
     a.smax = c->smax + 2 * CFL_SPREAD * slot;
+    if (c->devctl) {
+        a.dtp = &c->ctl->dt;
+        a.beta = beta;
+        a.halt = &c->ctl->halt;
+    }
     const bool lxw = p.time_stepping_method == FINESS_TS_LXW;
     if (lxw) {
         a.lxw_F = c->lxwF;
@@ -272,7 +282,8 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
 {
     const finess_params& p = c->p;
     const int A = c->cur, B = (c->cur + 1) % 3, C = (c->cur + 2) % 3;
-    int rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st);
+    // (under the device controller the maxima are cleared by the controller)
+    int rc = c->devctl ? 0 : launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st);
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
     if (c->rec_age >= 0 && c->route_fresh == false) c->rec_age++;  // the previous step used the record
     c->route_fresh = true;
@@ -517,6 +528,7 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
     }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
+    if (!rc && p.ndims == 2) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
     if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2) {
         const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
         rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
@@ -553,6 +565,7 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->route_idx);
     hipFree(c->route_list);
     hipFree(c->route_n);
+    hipFree(c->ctl);
     hipFree(c->route_flux);
     if (c->smax_h) hipHostFree(c->smax_h);
     for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -740,8 +753,142 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     return FINESS_OK;
 }
 
+static const char* ctl_error_message(long long code)
+{
+    switch (code) {
+    case FINESS_ERR_DT: return "dt is out of bounds (dt > max_dt)";
+    case FINESS_ERR_NONFINITE: return "non-finite wave speed";
+    case FINESS_ERR_CFL: return "CFL number is out of bounds";
+    case FINESS_ERR_NV: return "exceeded allowed # of time steps";
+    default: return "device controller error";
+    }
+}
+
+// advance_impl's controller run on the device (k_step_control after every
+// attempt).  Steps are enqueued in chunks with the buffer rotation of
+// accepted steps and the control block is read back once per chunk.  A
+// rejected attempt halts the chunk's later kernels; the host then resumes
+// from that attempt's input, with the routing-record state it had after the
+// attempt, exactly as the host controller retries.  Decisions, dt sequence
+// and results are those of the host controller.
+static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
+{
+    const finess_params& p = c->p;
+    const bool by_steps = fixed_steps > 0;
+    StepCtl h;
+    memset(&h, 0, sizeof h);
+    h.t = c->t;
+    h.dt_next = dtv[1];
+    h.dt = (!by_steps && h.t + dtv[1] > tend) ? tend - h.t : dtv[1];
+    h.dtmin = h.dtmax = dtv[1];
+    h.n_step = 1;
+    StepCtlParams cp;
+    cp.tend = tend;
+    cp.max_dt = dtv[2];
+    cp.cfl_max = p.max_cfl;
+    cp.cfl_target = p.desired_cfl;
+    cp.dx = p.dx;
+    cp.dy = p.dy;
+    cp.nv = p.nv;
+    cp.target = by_steps ? fixed_steps : 0;
+    int rc = 0;
+    bool run = by_steps || c->t < tend;
+    if (run && !by_steps && h.n_step > p.nv) {
+        rc = fail(c, FINESS_ERR_NV, ctl_error_message(FINESS_ERR_NV));
+        run = false;
+    }
+    if (run) {
+        HIPCHK(c, hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->st));
+        if (launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "slot reset failed");
+        c->devctl = true;
+    }
+    struct Snap {
+        int in, out, rec_age;
+        bool fresh;
+    };
+    std::vector<Snap> snap;
+    long long seen = 0;   // attempts processed before the current chunk
+    long long kmax = 64;  // chunk length: 64, or shorter after a rejection
+    while (run) {
+        long long K;
+        if (by_steps) {
+            K = fixed_steps - h.acc;
+        } else {
+            const double est = (tend - h.t) / h.dt_next;
+            K = est < 63.0 ? (long long)ceil(est) + 1 : 64;
+        }
+        K = K < 1 ? 1 : (K > kmax ? kmax : K);
+        snap.clear();
+        for (long long k = 0; k < K && !rc; k++) {
+            Snap sn;
+            sn.in = c->cur;
+            int res = 0, slot = 0;
+            if ((rc = enqueue_step(c, 0.0, &res, &slot))) break;
+            if (launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * slot, cp, c->st)) {
+                rc = fail(c, FINESS_ERR_HIP, "controller launch failed");
+                break;
+            }
+            sn.out = res;
+            sn.rec_age = c->rec_age;
+            sn.fresh = c->route_fresh;
+            snap.push_back(sn);
+            c->cur = res;
+        }
+        if (!rc && hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->st) != hipSuccess) rc = FINESS_ERR_HIP;
+        if (!rc && hipStreamSynchronize(c->st) != hipSuccess) rc = FINESS_ERR_HIP;
+        if (rc) {
+            if (rc == FINESS_ERR_HIP && c->err.empty()) c->err = "device controller: stream failure";
+            break;
+        }
+        const long long m = h.done - seen - 1;  // the chunk's last processed attempt
+        seen = h.done;
+        if (h.halt == HALT_NONE) {
+            if (kmax < 64) kmax *= 2;
+            continue;
+        }
+        if (m < 0 || m >= (long long)snap.size()) {
+            rc = fail(c, FINESS_ERR_HIP, "device controller: inconsistent control block");
+            break;
+        }
+        const Snap& sn = snap[(size_t)m];
+        c->rec_age = sn.rec_age;
+        c->route_fresh = sn.fresh;
+        if (h.halt == HALT_DONE) {
+            c->cur = sn.out;
+            break;
+        }
+        if (h.halt == HALT_REJECT) {
+            c->cur = sn.in;
+            kmax = 4;
+            if (launch_fill_u64(&c->ctl->halt, 1, (unsigned long long)HALT_NONE, c->st)) {
+                rc = fail(c, FINESS_ERR_HIP, "controller restart failed");
+                break;
+            }
+            continue;
+        }
+        c->cur = h.err == FINESS_ERR_NV ? sn.out : sn.in;
+        rc = fail(c, (int)h.err, ctl_error_message(h.err));
+        break;
+    }
+    c->devctl = false;
+    if (!rc) dtv[1] = h.dt_next;
+    c->t = h.t;
+    if (stats) {
+        stats->steps_accepted = h.acc;
+        stats->steps_rejected = h.rej;
+        stats->dt_min = h.dtmin;
+        stats->dt_max = h.dtmax;
+        stats->last_cfl = h.cfl;
+        stats->t = h.t;
+    }
+    return rc;
+}
+
 static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
 {
+    if (c && dtv && c->have_state && c->ctl && c->p.ndims == 2 && c->p.time_stepping_method == FINESS_TS_RK &&
+        !c->p.verbosity && !c->p.host_controller)
+        return advance_dev(c, tend, dtv, fixed_steps, stats);
     if (!c || !dtv) return FINESS_ERR_ARG;
     if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
     const finess_params& p = c->p;

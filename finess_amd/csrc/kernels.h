@@ -31,6 +31,13 @@ int mlp_adam(float* p, float* g, float* m, float* v, float lr, int t, hipStream_
 int mlp_zero(float* p, long long n, hipStream_t st);
 
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
+
+// Device-side time-step controller (step_control.hip).
+struct StepCtl;
+struct StepCtlParams;
+int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParams& p, hipStream_t st);
+int launch_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt,
+                         hipStream_t st);
 int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st);
 int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],
                          int nx_mom, int ny_mom, hipStream_t st);

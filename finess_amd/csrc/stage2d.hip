@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     constexpr int NT = 256;
     static_assert((TI + 1) * TJ <= NT && TI * (TJ + 1) <= NT, "one face per thread and direction");
     __shared__ StageSmem<P, R, TI, TJ> sm;
+    if (stage_halted(a)) return;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;

@@ -75,6 +75,12 @@ struct StageArgs {
     int bc[4];             // left, right, bottom, top
     int a_is_src, b_is_src, use_b;
     double a1, a2, bdt;
+    // device-controlled stepping (null: off): the stage's dt is read on the
+    // device, bdt = beta * (*dtp), and every kernel of the run returns at
+    // once while *halt is non-zero
+    const double* dtp;
+    double beta;
+    const unsigned long long* halt;
     double dx, dy;
     double alpha_scaling;
     double eps, power;
@@ -106,6 +112,19 @@ struct StageArgs {
     double* smax_faces_x;  // (mx+1) x my alphas of x-faces, row-major i*(my)+j
     double* smax_faces_y;  // mx x (my+1)
 };
+
+// beta * dt of the stage update (dt from the device controller when set)
+__host__ __device__ __forceinline__ double stage_bdt(const StageArgs& a)
+{
+    return a.dtp ? a.beta * *a.dtp : a.bdt;
+}
+
+// The device controller stopped the run (a rejected step, the end, an error):
+// the kernels enqueued after it do nothing.
+__host__ __device__ __forceinline__ bool stage_halted(const StageArgs& a)
+{
+    return a.halt && *a.halt != 0ull;
+}
 
 template <class P, typename R, int TI, int TJ>
 struct StageSmem {
@@ -371,6 +390,7 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
     const int c = (ci + 3) * S::HJ + (cj + 3);
     const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
     const double idy_ = 1.0 / a.dy;
+    const double bdt = stage_bdt(a);
     for (int m = 0; m < M; m++) {
         const double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
@@ -381,7 +401,7 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
             const double qb = (exact_lds && a.b_is_src) ? double(sm.cv[m][c]) : pb[m];
             v = v + a.a2 * qb;
         }
-        a.out[m * a.plane + idx] = v + a.bdt * L;
+        a.out[m * a.plane + idx] = v + bdt * L;
     }
 }
 

@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     using S = StageSmem<P, R, TI, TJ>;
     static_assert(T::NFX <= NT && T::NFY <= NT, "one face per thread and direction");
     __shared__ T sm;
+    if (stage_halted(a)) return;
     const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
             const int c = (ci + 3) * S::HJ + (cj + 3);
             const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
             const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+            const double bdt = stage_bdt(a);
             for (int m = 0; m < M; m++) {
                 const double Lx = -(double(sm.s.face[m][(ci + 1) * TJ + cj]) - double(sm.s.face[m][ci * TJ + cj])) * idx_;
                 const double L = Lx - (double(sm.fy[m][ci * (TJ + 1) + cj + 1]) - double(sm.fy[m][ci * (TJ + 1) + cj])) * idy_;
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
                     const double qb = (exact_lds && a.b_is_src) ? double(sm.s.cv[m][c]) : pb[m];
                     v = v + a.a2 * qb;
                 }
-                a.out[m * a.plane + idx] = v + a.bdt * L;
+                a.out[m * a.plane + idx] = v + bdt * L;
             }
         }
     }
@@ -313,6 +315,7 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
 template <class P, typename R, int WK>
 __global__ __launch_bounds__(256) void k_route_full(StageArgs a)
 {
+    if (stage_halted(a)) return;
     const long long nx = (long long)a.route_n[0], ny = (long long)a.route_n[1];
     const long long px = (nx + 63) & ~63ll;
     const long long nfx = (long long)(a.mx + 1) * a.my;
@@ -334,6 +337,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     __shared__ S sm;
     __shared__ R fy[M][TI * (TJ + 1)];
     __shared__ double red[2][NT / 64];
+    if (stage_halted(a)) return;
     const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
@@ -378,6 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
             const int c = (ci + 3) * S::HJ + (cj + 3);
             const bool exact_lds = sizeof(R) == sizeof(double);
             const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+            const double bdt = stage_bdt(a);
             for (int m = 0; m < M; m++) {
                 const double Lx = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
                 const double L = Lx - (double(fy[m][ci * (TJ + 1) + cj + 1]) - double(fy[m][ci * (TJ + 1) + cj])) * idy_;
@@ -387,7 +392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
                     const double qb = (exact_lds && a.b_is_src) ? double(sm.cv[m][c]) : pb[m];
                     v = v + a.a2 * qb;
                 }
-                a.out[m * a.plane + idx] = v + a.bdt * L;
+                a.out[m * a.plane + idx] = v + bdt * L;
             }
         }
     }

@@ -1,0 +1,101 @@
+// step_control.hip -- device-side time-step controller of FinSolveRK
+// (lib/2d/FinSolveRK.cpp:77-106 and 318-350, with GetCFL from
+// lib/2d/GetCFL.cpp:31-74).  One 64-lane workgroup per step attempt: it
+// reduces the attempt's wave-speed maxima (and clears them for the next
+// attempt), forms the CFL number, accepts or rejects, sets the next dt and
+// stops the run (halt) at the end, at a rejection (the host re-plans from
+// the rejected attempt) or at an error.  The arithmetic is the host
+// controller's, operation for operation.
+#include <hip/hip_runtime.h>
+
+#include "../../include/finess_hip.h"
+#include "kernels.h"
+#include "stage2d_body.h"
+#include "step_control.h"
+
+namespace finess {
+
+__global__ __launch_bounds__(64) void k_step_control(StepCtl* c, unsigned long long* smax, StepCtlParams p)
+{
+    static_assert(CFL_SPREAD == 64, "one maxima word per lane and direction");
+    if (c->halt != HALT_NONE) return;
+    const int l = threadIdx.x;
+    unsigned long long bx = smax[l], by = smax[CFL_SPREAD + l];
+    smax[l] = 0ull;
+    smax[CFL_SPREAD + l] = 0ull;
+    // bit patterns of non-negative doubles (or +inf) order as the values
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long ox = (unsigned long long)__double_as_longlong(
+            __shfl_xor(__longlong_as_double((long long)bx), o, 64));
+        const unsigned long long oy = (unsigned long long)__double_as_longlong(
+            __shfl_xor(__longlong_as_double((long long)by), o, 64));
+        bx = ox > bx ? ox : bx;
+        by = oy > by ? oy : by;
+    }
+    if (l != 0) return;
+    const double sx = __longlong_as_double((long long)bx), sy = __longlong_as_double((long long)by);
+    const double dt = c->dt, told = c->t;
+    c->done = c->done + 1;
+    auto error = [&](long long code, double t) {
+        c->err = code;
+        c->t = t;
+        c->halt = HALT_ERROR;
+    };
+    // GetCFL's checks, in its order
+    if (dt > p.max_dt) { error(FINESS_ERR_DT, told + dt); return; }
+    if (!isfinite(sx) || !isfinite(sy)) { error(FINESS_ERR_NONFINITE, told + dt); return; }
+    double cfl = -100.0;
+    cfl = rmax(dt * sx / p.dx, cfl);
+    cfl = rmax(dt * sy / p.dy, cfl);
+    c->cfl = cfl;
+    if (cfl > 1.0e8) { error(FINESS_ERR_CFL, told + dt); return; }
+    double dtn;
+    if (cfl > 0.0) {
+        dtn = rmin(p.max_dt, dt * p.cfl_target / cfl);
+        c->dtmin = rmin(dtn, c->dtmin);
+        c->dtmax = rmax(dtn, c->dtmax);
+    } else {
+        dtn = p.max_dt;
+    }
+    double t = told;
+    if (cfl <= p.cfl_max) {
+        t = told + dt;
+        c->acc = c->acc + 1;
+        if (p.target ? c->acc >= p.target : !(t < p.tend)) {
+            c->halt = HALT_DONE;
+        } else if (!p.target) {
+            c->n_step = c->n_step + 1;
+            if (c->n_step > p.nv) { c->dt_next = dtn; error(FINESS_ERR_NV, t); return; }
+        }
+    } else {
+        c->rej = c->rej + 1;
+        c->halt = HALT_REJECT;
+    }
+    c->t = t;
+    c->dt_next = dtn;
+    c->dt = (!p.target && t + dtn > p.tend) ? p.tend - t : dtn;
+}
+
+int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParams& p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_control, dim3(1), dim3(64), 0, st, c, smax, p);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// Fill that a halted run skips (the routing record reset of a step that a
+// rejection or the end has cancelled must not touch the record).
+__global__ void k_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt)
+{
+    if (*halt != HALT_NONE) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u64_live, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v, halt);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+} // namespace finess

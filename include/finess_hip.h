@@ -105,6 +105,10 @@ typedef struct finess_params {
                                            every k-th time step and its routing serves the
                                            later stages of that step and the next k-1 steps;
                                            0: it runs at every stage */
+    int    host_controller;             /* 1: the host runs the time-step controller after
+                                           every step; 0 (default): 2D Runge-Kutta runs
+                                           without verbosity are controlled on the device
+                                           (same decisions, no host round trip per step) */
 } finess_params;
 
 /* Fills defaults exactly as the reference's IniParams defaults do. */

@@ -64,6 +64,7 @@ struct finess_ctx {
     bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
     int rec_age = -1;                             // steps since the record was decided (-1: none)
+    int rec_par = 0;                              // counter pair of the current record
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;  // per-launch timing marks (run_fixed_steps)
     size_t tev_used = 0;
@@ -184,15 +185,18 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
     if (stepping && p.route_per_step && c->route_idx) {
         a.route_idx = c->route_idx;
         a.route_list = c->route_list;
-        a.route_n = c->route_n;
         a.route_flux = c->route_flux;
         bool decide = c->route_fresh && (c->rec_age < 0 || c->rec_age >= p.route_per_step);
         a.route_mask_read = decide ? 0 : 1;
+        // two counter pairs: a new record counts into the pair the previous
+        // indicator stage cleared and clears the other one
         if (decide) {
-            const int frc = c->devctl ? launch_fill_u64_live(c->route_n, 2, 0ull, &c->ctl->halt, c->st)
-                                      : launch_fill_u64(c->route_n, 2, 0ull, c->st);
-            if (frc) return fail(c, FINESS_ERR_HIP, "routing record reset failed");
+            a.route_n = c->route_n + 2 * (1 - c->rec_par);
+            a.route_n_clear = c->route_n + 2 * c->rec_par;
+            c->rec_par = 1 - c->rec_par;
             c->rec_age = 0;
+        } else {
+            a.route_n = c->route_n + 2 * c->rec_par;
         }
         c->route_fresh = false;
     }
@@ -533,7 +537,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
         rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
         if (!rc) rc = alloc(c, (void**)&c->route_list, nf * sizeof(int));
-        if (!rc) rc = alloc(c, (void**)&c->route_n, 2 * sizeof(unsigned long long));
+        if (!rc) rc = alloc(c, (void**)&c->route_n, 4 * sizeof(unsigned long long));
+        if (!rc) rc = launch_fill_u64(c->route_n, 4, 0ull, c->st) ? FINESS_ERR_HIP : 0;
         if (!rc) rc = alloc(c, (void**)&c->route_flux, nf * 6 * sizeof(double));
     }
     if (!rc) rc = launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st) ? FINESS_ERR_HIP : 0;
@@ -803,7 +808,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         c->devctl = true;
     }
     struct Snap {
-        int in, out, rec_age;
+        int in, out, rec_age, rec_par;
         bool fresh;
     };
     std::vector<Snap> snap;
@@ -830,6 +835,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
             }
             sn.out = res;
             sn.rec_age = c->rec_age;
+            sn.rec_par = c->rec_par;
             sn.fresh = c->route_fresh;
             snap.push_back(sn);
             c->cur = res;
@@ -852,6 +858,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         }
         const Snap& sn = snap[(size_t)m];
         c->rec_age = sn.rec_age;
+        c->rec_par = sn.rec_par;
         c->route_fresh = sn.fresh;
         if (h.halt == HALT_DONE) {
             c->cur = sn.out;

@@ -36,8 +36,6 @@ int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStrea
 struct StepCtl;
 struct StepCtlParams;
 int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParams& p, hipStream_t st);
-int launch_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt,
-                         hipStream_t st);
 int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st);
 int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],
                          int nx_mom, int ny_mom, hipStream_t st);

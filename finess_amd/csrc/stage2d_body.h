@@ -99,12 +99,16 @@ struct StageArgs {
     //   route_idx[face]  -1 for a smooth face, else its slot in the troubled list
     //   route_list[slot] the face's (i, j) packed as i << 16 | j; troubled x
     //                    faces take slots 0.., y faces (mx+1)*my..
-    //   route_n[2]       number of troubled x faces, y faces
+    //   route_n[2]       number of troubled x faces, y faces (zero when the
+    //                    indicator starts a record)
+    //   route_n_clear[2] set to zero by the indicator stage: the counter pair
+    //                    of the previous record, which the next record uses
     //   route_flux[slot] 5 flux components + alpha of the troubled face
     // route_mask_read: 0 = the indicator decides and records, 1 = read the record
     int* route_idx;
     int* route_list;
     unsigned long long* route_n;
+    unsigned long long* route_n_clear;
     double* route_flux;
     int route_mask_read;
     // optional face-flux / L output for the drop-in ConstructL (may be null):

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     }
     __syncthreads();
     const int nt = sm.nt, n = nt + sm.ns;
+    if (t < 2 && a.route_n_clear && blockIdx.x == 0) a.route_n_clear[t] = 0ull;
     if (t < 2 && a.route_idx) {  // thread d takes the slots of direction d
         int c = 0;
         for (int k = 0; k < NT / 64; k++) c += sm.rcnt[t][k];

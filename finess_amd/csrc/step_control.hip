@@ -82,20 +82,4 @@ int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParam
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-// Fill that a halted run skips (the routing record reset of a step that a
-// rejection or the end has cancelled must not touch the record).
-__global__ void k_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt)
-{
-    if (*halt != HALT_NONE) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-int launch_fill_u64_live(unsigned long long* p, int n, unsigned long long v, const unsigned long long* halt,
-                         hipStream_t st)
-{
-    hipLaunchKernelGGL(k_fill_u64_live, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v, halt);
-    return hipGetLastError() == hipSuccess ? 0 : -4;
-}
-
 } // namespace finess

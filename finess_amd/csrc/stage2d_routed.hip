@@ -269,6 +269,100 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     }
 }
 
+// The indicator alone, for the first stage of a new routing record: every
+// face of the tile is classified (by the network, or by the exact
+// weight-departure label with EXACT) and the decisions are recorded; the
+// stage itself then runs from the record as k_route_full + k_stage2d_lookup.
+// Without the fluxes and the update the kernel keeps only the tile records
+// and the network, so more workgroups stay resident than in the combined
+// kernel.
+template <class P, typename R, int TI, int TJ, bool EXACT>
+__global__ __launch_bounds__(256) void k_classify(StageArgs a)
+{
+    constexpr int NT = 256, NV = P::NV;
+    using S = StageSmem<P, R, TI, TJ>;
+    __shared__ S ss;
+    __shared__ int cnt[4][4];  // per wave: recorded troubled x, y; troubled faces, faces
+    __shared__ int rbase[2];
+    if (stage_halted(a)) return;
+    const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
+    const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
+    const int i0 = ti * TI, j0 = tj * TJ;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const FaceGeom gx = face_geom<TI, TJ>(t, 0, i0, j0, a);
+    const FaceGeom gy = face_geom<TI, TJ>(t, 1, i0, j0, a);
+    bool tx = false, ty = false;
+    if constexpr (!EXACT) {
+        __shared__ MlpTile ml;
+        mlp_stage_frags(a.frags, &ml.fr, t, NT);
+        phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, ss);
+        __syncthreads();
+        for (int dir = 0; dir < 2; dir++) {
+            const FaceGeom& g = dir ? gy : gx;
+            float f[MLP_IN];
+            for (int k = 0; k < MLP_IN; k++) f[k] = 0.f;
+            if (g.valid) {
+                R rec[6][NV];
+                stencil_records<P, R, TI, TJ>(ss, g, rec);
+                face_features<P, R>(rec, dir, a.pp, f);
+            }
+            const bool tr = mlp_wave_logit(f, ml.fr, ml.feat[w], ml.lo[w], lane) > ml.fr.threshold;
+            if (dir) ty = tr; else tx = tr;
+        }
+    } else {
+        phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, ss);
+        __syncthreads();
+        if (gx.valid) {
+            R rec[6][NV];
+            stencil_records<P, R, TI, TJ>(ss, gx, rec);
+            tx = face_departure<P, R>(rec, 0, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
+        }
+        if (gy.valid) {
+            R rec[6][NV];
+            stencil_records<P, R, TI, TJ>(ss, gy, rec);
+            ty = face_departure<P, R>(rec, 1, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
+        }
+    }
+    tx = tx && gx.valid;
+    ty = ty && gy.valid;
+    // the record, as in k_stage2d_routed: smooth owned faces marked at once,
+    // troubled owned faces counted per wave and given slots from one atomic
+    // per tile and direction
+    const bool ox = gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a);
+    const bool oy = gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a);
+    const unsigned long long bx = __ballot(ox && tx), by = __ballot(oy && ty);
+    if (ox && !tx) a.route_idx[mask_index(0, gx, i0, j0, a)] = -1;
+    if (oy && !ty) a.route_idx[mask_index(1, gy, i0, j0, a)] = -1;
+    const int nt = __popcll(__ballot(tx)) + __popcll(__ballot(ty));
+    const int nf = __popcll(__ballot(gx.valid)) + __popcll(__ballot(gy.valid));
+    if (lane == 0) {
+        cnt[0][w] = __popcll(bx);
+        cnt[1][w] = __popcll(by);
+        cnt[2][w] = nt;
+        cnt[3][w] = nf;
+    }
+    __syncthreads();
+    if (t < 2) {  // thread d takes the slots of direction d
+        if (a.route_n_clear && blockIdx.x == 0) a.route_n_clear[t] = 0ull;
+        int c = 0;
+        for (int k = 0; k < NT / 64; k++) c += cnt[t][k];
+        rbase[t] = c ? (int)atomicAdd(&a.route_n[t], (unsigned long long)c) : 0;
+    } else if (t < 4 && a.route_counts) {
+        int c = 0;
+        for (int k = 0; k < NT / 64; k++) c += cnt[t][k];
+        atomicAdd(&a.route_counts[(t - 2) * CFL_SPREAD + blockIdx.x % CFL_SPREAD], (unsigned long long)c);
+    }
+    __syncthreads();
+    if (bx | by) {
+        int bsx = rbase[0], bsy = (a.mx + 1) * a.my + rbase[1];
+        for (int k = 0; k < w; k++) { bsx += cnt[0][k]; bsy += cnt[1][k]; }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (ox && tx) record_troubled(mask_index(0, gx, i0, j0, a), i0 + gx.fi, j0 + gx.fj, bsx + __popcll(bx & below), a);
+        if (oy && ty)
+            record_troubled(mask_index(1, gy, i0, j0, a), i0 + gy.fi, j0 + gy.fj, bsy + __popcll(by & below), a);
+    }
+}
+
 // Characteristic WENO5 flux of troubled face k of direction DIR (list
 // slots from slot0; lanes with k >= n idle).  The stencil of a face whose
 // six cells all lie inside the grid is read directly, others through the
@@ -425,7 +519,14 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
     const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
     a.ntj = ntj;
     const dim3 grid(nti * ntj), block(256);
-    if (a.route_mask_read) {
+    if (a.route_mask_read || a.route_idx) {
+        // a new record: the indicator first, then the stage from the record
+        if (!a.route_mask_read) {
+            if (a.route_exact)
+                hipLaunchKernelGGL((k_classify<Euler2D, R, TI, TJ, true>), grid, block, 0, st, a);
+            else
+                hipLaunchKernelGGL((k_classify<Euler2D, R, TI, TJ, false>), grid, block, 0, st, a);
+        }
         hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(1024), block, 0, st, a);
         hipLaunchKernelGGL((k_stage2d_lookup<Euler2D, R, TI, TJ>), grid, block, 0, st, a);
     } else if (a.route_exact)
@@ -438,7 +539,8 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
 int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st)
 {
     if (physics != PHYS_EULER || wk != WENO_JS5_P2) return -1;
-    if (a.route_mask_read && !(a.route_idx && a.route_list && a.route_n && a.route_flux && a.out)) return -1;
+    if ((a.route_mask_read || a.route_idx) && !(a.route_idx && a.route_list && a.route_n && a.route_flux && a.out))
+        return -1;
     return fp32 ? launch_routed<float>(a, st) : launch_routed<double>(a, st);
 }
 

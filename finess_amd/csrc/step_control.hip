@@ -18,9 +18,15 @@ namespace finess {
 __global__ __launch_bounds__(64) void k_step_control(StepCtl* c, unsigned long long* smax, StepCtlParams p)
 {
     static_assert(CFL_SPREAD == 64, "one maxima word per lane and direction");
-    if (c->halt != HALT_NONE) return;
     const int l = threadIdx.x;
+    // every load is issued before anything waits on one: the maxima, and
+    // the whole control block in lane 0 (decided on a register copy and
+    // stored back in one piece)
     unsigned long long bx = smax[l], by = smax[CFL_SPREAD + l];
+    StepCtl h;
+    if (l == 0) h = *c;
+    const unsigned long long halt = __shfl(l == 0 ? h.halt : 0ull, 0, 64);
+    if (halt != HALT_NONE) return;
     smax[l] = 0ull;
     smax[CFL_SPREAD + l] = 0ull;
     // bit patterns of non-negative doubles (or +inf) order as the values
@@ -34,12 +40,14 @@ __global__ __launch_bounds__(64) void k_step_control(StepCtl* c, unsigned long l
     }
     if (l != 0) return;
     const double sx = __longlong_as_double((long long)bx), sy = __longlong_as_double((long long)by);
-    const double dt = c->dt, told = c->t;
-    c->done = c->done + 1;
+    const double dt = h.dt, told = h.t;
+    h.done = h.done + 1;
+    auto finish = [&]() { *c = h; };
     auto error = [&](long long code, double t) {
-        c->err = code;
-        c->t = t;
-        c->halt = HALT_ERROR;
+        h.err = code;
+        h.t = t;
+        h.halt = HALT_ERROR;
+        finish();
     };
     // GetCFL's checks, in its order
     if (dt > p.max_dt) { error(FINESS_ERR_DT, told + dt); return; }
@@ -47,33 +55,34 @@ __global__ __launch_bounds__(64) void k_step_control(StepCtl* c, unsigned long l
     double cfl = -100.0;
     cfl = rmax(dt * sx / p.dx, cfl);
     cfl = rmax(dt * sy / p.dy, cfl);
-    c->cfl = cfl;
+    h.cfl = cfl;
     if (cfl > 1.0e8) { error(FINESS_ERR_CFL, told + dt); return; }
     double dtn;
     if (cfl > 0.0) {
         dtn = rmin(p.max_dt, dt * p.cfl_target / cfl);
-        c->dtmin = rmin(dtn, c->dtmin);
-        c->dtmax = rmax(dtn, c->dtmax);
+        h.dtmin = rmin(dtn, h.dtmin);
+        h.dtmax = rmax(dtn, h.dtmax);
     } else {
         dtn = p.max_dt;
     }
     double t = told;
     if (cfl <= p.cfl_max) {
         t = told + dt;
-        c->acc = c->acc + 1;
-        if (p.target ? c->acc >= p.target : !(t < p.tend)) {
-            c->halt = HALT_DONE;
+        h.acc = h.acc + 1;
+        if (p.target ? h.acc >= p.target : !(t < p.tend)) {
+            h.halt = HALT_DONE;
         } else if (!p.target) {
-            c->n_step = c->n_step + 1;
-            if (c->n_step > p.nv) { c->dt_next = dtn; error(FINESS_ERR_NV, t); return; }
+            h.n_step = h.n_step + 1;
+            if (h.n_step > p.nv) { h.dt_next = dtn; error(FINESS_ERR_NV, t); return; }
         }
     } else {
-        c->rej = c->rej + 1;
-        c->halt = HALT_REJECT;
+        h.rej = h.rej + 1;
+        h.halt = HALT_REJECT;
     }
-    c->t = t;
-    c->dt_next = dtn;
-    c->dt = (!p.target && t + dtn > p.tend) ? p.tend - t : dtn;
+    h.t = t;
+    h.dt_next = dtn;
+    h.dt = (!p.target && t + dtn > p.tend) ? p.tend - t : dtn;
+    finish();
 }
 
 int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParams& p, hipStream_t st)

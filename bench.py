@@ -18,8 +18,10 @@ CFL 0.4, transmissive boundaries, t_final 0.3.  Two forms of the same solver:
                 at the first stage of every fourth step; the stages in between
                 reuse its routing record.
 
-A "step" is one accepted time step (three fused stage launches and the
-controller's read-back of the wave-speed maxima).  Both forms are timed over
+A "step" is one accepted time step: three stages (plain: one fused kernel
+each; learned: the troubled-face pass and the lookup stage kernel, with the
+indicator kernel in front at every fourth step) and the time-step
+controller, which runs on the device after each step.  Both forms are timed over
 K steps from the state at t = 0.15 (the middle of the run) after W warm-up
 steps; inputs are resident in HBM before the timed region starts.  The L1
 density error of each form at t_final is measured against a 2048 x 2048
@@ -27,12 +29,18 @@ plain-FP64 run averaged onto the 1024 grid.
 
 Prints one JSON line.  `value` is the learned+FP32 Mcell-updates/s
 (cells * steps / seconds); the plain FP64 figures and the error ratio sit
-beside it.  `roofline` refers to the dominant kernel of the learned form (the
-routed stage kernel, all three stages of a step): achieved = algorithmic HBM
-bytes per launch / average launch duration, measured with HIP events on the
-stream the kernel runs on.  `cpu_baseline` is the CPU restatement of the
+beside it.  `roofline` refers to the stage of the learned form (the kernels
+of one stage together, averaged over the stages of a 4-step indicator
+interval): achieved = algorithmic HBM bytes per stage / average stage
+duration, measured with HIP events on the stream the kernels run on.  `cpu_baseline` is the CPU restatement of the
 reference (oracle/) timed on this host's cores on a bounded sample of the
 same workload.
+
+Two further workloads of the same solver are reported beside the headline
+(skip them with --no-extra): the third-order Lax-Wendroff form on the
+periodic Kelvin-Helmholtz problem at 2048 x 2048 (FP64 against the FP32
+stencil form), and the Mach-3 shock-bubble at 4096 x 1024 (learned+FP32 with
+the same indicator, which never saw this problem, against plain FP64).
 """
 import argparse
 import json
@@ -50,10 +58,10 @@ BYTES_PER_CELL_UPDATE = 320.0   # SSP-RK3: 8 state passes x 5 fields x 8 B (FP64
 STAGES_PER_STEP = 3
 # HBM bytes per stage launch at 1024x1024 measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
-# counts in profiles/k_stage2d_routed_euler_fp32_riemann4_1024.md and
+# counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 150.8e6   # mean over the 12 stages of a 4-step indicator interval
-TRAFFIC_PLAIN_1024 = 114.2e6
+TRAFFIC_ROUTED_1024 = 153.7e6   # mean over the 12 stages of a 4-step indicator interval
+TRAFFIC_PLAIN_1024 = 113.8e6
 T_FINAL = 0.3
 T_TIMED = 0.15
 # the indicator runs at the first stage of every ROUTE_INTERVAL-th step; the
@@ -152,6 +160,77 @@ def batched_shallow_water(n=512, steps=200):
             "steps_rejected": int(rej)}
 
 
+def _interior_density(p, q):
+    g = p.mbc
+    return q[g:g + p.mx, g:g + p.my, 0].copy()
+
+
+def kelvin_helmholtz_lxw(mx=2048, steps=20, warmup=3):
+    """Third-order Lax-Wendroff, periodic Kelvin-Helmholtz at mx^2: the FP64
+    and the FP32-stencil forms (FP64 state and accumulation) timed over
+    `steps` accepted steps after `warmup`, from t = 0, and the relative L1
+    density difference between the two after those steps."""
+    from finess_amd import PREC_FP32, Solver, problems
+    out, rho, tt = {}, {}, {}
+    for name, kw in (("fp64", {}), ("fp32", dict(precision=PREC_FP32))):
+        p, q, _ = problems.kelvin_helmholtz(mx=mx, **kw)
+        with Solver(p) as s:
+            s.set_state(q)
+            dtv = np.array([0.0, p.initial_dt, p.max_dt])
+            s.advance_steps(warmup, dtv)
+            s.sync()
+            t0 = time.perf_counter()
+            st = s.advance_steps(steps, dtv)
+            s.sync()
+            el = time.perf_counter() - t0
+            qf, tt[name] = s.get_state()
+        rho[name] = _interior_density(p, qf)
+        out[name] = {"value": round(mx * mx * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 4),
+                     "steps_rejected": int(st.steps_rejected)}
+    out["speedup_fp32"] = round(out["fp32"]["value"] / out["fp64"]["value"], 4)
+    out["l1_rel_density_fp32_vs_fp64"] = float(np.mean(np.abs(rho["fp32"] - rho["fp64"])) / np.mean(np.abs(rho["fp64"])))
+    out["t"] = {k: round(v, 6) for k, v in tt.items()}
+    out["workload"] = "2D Euler Kelvin-Helmholtz %dx%d periodic, LxW order 3, WENO5-JS, CFL 0.4, %d steps after %d" % (
+        mx, mx, steps, warmup)
+    return out
+
+
+def shock_bubble_learned(ind, mx=4096, my=1024, steps=40, warmup=5):
+    """Mach-3 shock-bubble at mx x my: learned+FP32 (routing record every
+    ROUTE_INTERVAL steps) against plain FP64, timed over `steps` accepted
+    steps after `warmup`, from t = 0; the relative L1 density difference
+    between the forms after those steps and the routed fraction."""
+    from finess_amd import PREC_FP32, ROUTE_LEARNED, Solver, problems
+    out, rho = {}, {}
+    for name, kw in (("plain_fp64", {}),
+                     ("learned_fp32", dict(precision=PREC_FP32, routing=ROUTE_LEARNED, route_per_step=ROUTE_INTERVAL))):
+        p, q, _ = problems.shock_bubble(mx=mx, my=my, **kw)
+        with Solver(p) as s:
+            s.set_state(q)
+            if name == "learned_fp32":
+                ind.install(s)
+            dtv = np.array([0.0, p.initial_dt, p.max_dt])
+            s.advance_steps(warmup, dtv)
+            s.sync()
+            t0 = time.perf_counter()
+            st = s.advance_steps(steps, dtv)
+            s.sync()
+            el = time.perf_counter() - t0
+            route = s.route_stats() if name == "learned_fp32" else None
+            qf, t = s.get_state()
+        rho[name] = _interior_density(p, qf)
+        out[name] = {"value": round(mx * my * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 4),
+                     "steps_rejected": int(st.steps_rejected), "t": round(t, 6)}
+        if route is not None:
+            out[name]["troubled_fraction"] = round(route[0] / max(route[1], 1), 4)
+    out["speedup_vs_plain_fp64"] = round(out["learned_fp32"]["value"] / out["plain_fp64"]["value"], 4)
+    out["l1_rel_density_learned_vs_plain"] = float(
+        np.mean(np.abs(rho["learned_fp32"] - rho["plain_fp64"])) / np.mean(np.abs(rho["plain_fp64"])))
+    out["workload"] = ("2D Euler Mach-3 shock-bubble %dx%d, WENO5-JS + LLF, SSP-RK3, CFL 0.4, %d steps after %d; "
+                       "indicator trained on the four-quadrant problem and a density wave" % (mx, my, steps, warmup))
+    return out
+
+
 def copy_calibration(mib=256, reps=5):
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters of a profiled run to bytes."""
@@ -175,6 +254,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the t_final runs and the L1 errors")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shallow-water workload")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the Lax-Wendroff Kelvin-Helmholtz and the shock-bubble workloads")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -256,14 +337,17 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": TRAFFIC_ROUTED_1024 if mx == 1024 else None,
-            "kernel": "learned+FP32 stage: k_stage2d_routed (indicator stage, every 4th step) or "
-                      "k_route_full + k_stage2d_lookup (all other stages); mean over a 4-step interval",
+            "kernel": "learned+FP32 stage: k_route_full + k_stage2d_lookup, with k_classify in front at the "
+                      "first stage of every 4th step; mean over the 12 stages of a 4-step interval",
             "avg_launch_us": round(launch_l * 1e6, 2),
             "bytes_per_launch": bytes_per_launch,
         },
     }
     if not args.no_batch:
         out["batched_shallow_water"] = batched_shallow_water()
+    if not args.no_extra:
+        out["kelvin_helmholtz_lxw_2048"] = kelvin_helmholtz_lxw()
+        out["shock_bubble_4096x1024"] = shock_bubble_learned(ind)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(mx, args.cpu_steps, threads)

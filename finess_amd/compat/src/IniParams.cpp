@@ -230,4 +230,6 @@ void IniParams::to_finess_params(finess_params* p) const
     p->routing = rt == "learned" ? FINESS_ROUTE_LEARNED : rt == "oracle" ? FINESS_ROUTE_ORACLE : FINESS_ROUTE_NONE;
     p->route_tau = as_double(raw("gpu", "route_tau", "0.05", false), "gpu.route_tau");
     p->sw_compat_hl = as_bool(raw("gpu", "sw_compat_hl", "true", false), "gpu.sw_compat_hl");
+    p->route_per_step = as_int(raw("gpu", "route_per_step", "1", false), "gpu.route_per_step");
+    p->host_controller = as_bool(raw("gpu", "host_controller", "false", false), "gpu.host_controller") ? 1 : 0;
 }

@@ -76,6 +76,7 @@ class Params(ctypes.Structure):
         ("tile_i", ctypes.c_int), ("tile_j", ctypes.c_int),
         ("route_per_step", ctypes.c_int),
         ("host_controller", ctypes.c_int),
+        ("mpp_limiter", ctypes.c_int),
     ]
 
     def copy(self):

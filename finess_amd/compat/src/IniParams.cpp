@@ -217,6 +217,7 @@ void IniParams::to_finess_params(finess_params* p) const
     p->nv = nv;
     p->global_alpha = global_alpha;
     p->source_term = source_term;
+    p->mpp_limiter = mpp_limiter ? 1 : 0;
     p->verbosity = verbosity;
     p->weno_version = weno_version == WenoVersion::FD ? FINESS_WENO_FD
                       : weno_version == WenoVersion::Z ? FINESS_WENO_Z : FINESS_WENO_JS;

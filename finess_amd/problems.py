@@ -126,6 +126,45 @@ def kelvin_helmholtz(mx=2048, my=None, seed=20240611, cfl=0.4, lxw=True, **kw):
     return p, np.ascontiguousarray(euler_conserved(rho, u, v, np.full_like(X, 2.5))), np.zeros(p.shape_aux())
 
 
+# 2D Euler near-vacuum test for the positivity-preserving limiter: the four
+# quadrants move apart diagonally (rho = 1, p = 0.4, |u| = |v| = u0), the 2D
+# counterpart of the 1D double rarefaction; Lax-Wendroff (mbc = 5)
+# ---------------------------------------------------------------------------
+def double_rarefaction(mx=64, my=None, u0=2.0, cfl=0.3, **kw):
+    my = mx if my is None else my
+    args = dict(ndims=2, mx=mx, my=my, mbc=5, meqn=5, maux=1, physics=PHYS_EULER, gamma=GAMMA,
+                bc=[BC_EXTRAP] * 4, time_order=3, time_stepping_method=TS_LXW, desired_cfl=cfl,
+                max_cfl=max(cfl, 0.45), initial_dt=cfl / max(mx, my) / (u0 + 1.0) / 2.0, max_dt=1.0, nv=10 ** 7)
+    args.update(kw)
+    p = default_params(**args)
+    X, Y = cell_centres(p)
+    u = np.where(X < 0.5, -u0, u0)
+    v = np.where(Y < 0.5, -u0, u0)
+    return p, np.ascontiguousarray(euler_conserved(np.ones_like(X), u, v, np.full_like(X, 0.4))), np.zeros(p.shape_aux())
+
+
+# 2D Euler Sedov blast as apps/2d/euler/sedov_blast sets it up
+# (parameters.ini, QinitFunc.cpp, AfterQinit.cpp): [0, 1.1]^2, rho = 1, zero
+# velocity, energy 1e-12, and 0.979264 / (dx dy) in the corner cell (1, 1);
+# reflecting left and bottom, extrapolation right and top; Lax-Wendroff with
+# the positivity-preserving limiter
+# ---------------------------------------------------------------------------
+def sedov_blast(mx=160, my=None, cfl=0.4, mpp_limiter=1, **kw):
+    my = mx if my is None else my
+    args = dict(ndims=2, mx=mx, my=my, mbc=5, meqn=5, maux=1, physics=PHYS_EULER, gamma=GAMMA,
+                xlow=0.0, xhigh=1.1, ylow=0.0, yhigh=1.1, bc=[BC_REFLECT, BC_EXTRAP, BC_REFLECT, BC_EXTRAP],
+                time_order=3, time_stepping_method=TS_LXW, desired_cfl=cfl, max_cfl=0.5, initial_dt=1e-5,
+                max_dt=1.0e10, nv=100000, mpp_limiter=mpp_limiter)
+    args.update(kw)
+    p = default_params(**args)
+    q = np.zeros(p.shape_q())
+    q[..., 0] = 1.0
+    q[..., 4] = 1.0e-12
+    g = p.mbc
+    q[g, g, 4] = 0.979264 / (p.dx * p.dy)
+    return p, q, np.zeros(p.shape_aux())
+
+
 # ---------------------------------------------------------------------------
 # workload 4 member: 2D shallow water Gaussian bump, periodic (g = 1)
 # ---------------------------------------------------------------------------

@@ -109,6 +109,10 @@ typedef struct finess_params {
                                            every step; 0 (default): 2D Runge-Kutta runs
                                            without verbosity are controlled on the device
                                            (same decisions, no host round trip per step) */
+    int    mpp_limiter;                 /* Lax-Wendroff, 2D Euler: limit the fluxes toward the
+                                           Lax-Friedrichs fluxes so that density and pressure stay
+                                           positive (lib/2d/FinSolveLxW.cpp:126-157); Runge-Kutta
+                                           ignores it, as the reference does */
 } finess_params;
 
 /* Fills defaults exactly as the reference's IniParams defaults do. */

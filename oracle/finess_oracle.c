@@ -539,8 +539,22 @@ static double interface_flux(const finess_params* p, int ixy, const double qs[6]
 /* 2D ConstructL, lib/2d/ConstructL.cpp:13-408.  If Fsrc/Gsrc are non-NULL the
  * stencil fluxes are read from them (ConstructLxWL, lib/2d/ConstructLxWL.cpp:
  * 106-236) instead of FluxFunc. */
+static int construct_l_2d_faces(const finess_params* p, const double* q, const double* aux,
+                                const double* Fsrc, const double* Gsrc, double* Lstar, double* smax,
+                                double* Fhat_out, double* Ghat_out);
+
 static int construct_l_2d(const finess_params* p, const double* q, const double* aux,
                           const double* Fsrc, const double* Gsrc, double* Lstar, double* smax)
+{
+    return construct_l_2d_faces(p, q, aux, Fsrc, Gsrc, Lstar, smax, NULL, NULL);
+}
+
+/* As construct_l_2d; Fhat_out / Ghat_out (if non-NULL, laid out as the
+ * reference's Fhat (mx+1, my) and Ghat (mx, my+1) with mbc ghosts) receive the
+ * interface fluxes (the pFhat / pGhat outputs of lib/2d/ConstructLxWL.cpp:30-33). */
+static int construct_l_2d_faces(const finess_params* p, const double* q, const double* aux,
+                                const double* Fsrc, const double* Gsrc, double* Lstar, double* smax,
+                                double* Fhat_out, double* Ghat_out)
 {
     const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
     const double dx = p->dx, dy = p->dy;
@@ -597,6 +611,8 @@ static int construct_l_2d(const finess_params* p, const double* q, const double*
                 tmp = tmp - (Ghat[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Ghat[ix3(i, j, m, my + 1, meqn, mbc)]) / dy;
                 Lstar[ix3(i, j, m, my, meqn, mbc)] = tmp;
             }
+    if (Fhat_out) memcpy(Fhat_out, Fhat, (size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn * sizeof(double));
+    if (Ghat_out) memcpy(Ghat_out, Ghat, (size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn * sizeof(double));
     free(Fhat);
     free(Ghat);
     return FINESS_OK;
@@ -722,7 +738,225 @@ static int rk_step(const finess_params* p, double* qnew, double* qstar, double* 
     return FINESS_OK;
 }
 
-/* One LxW step, lib/2d/FinSolveLxW.cpp:122-171 (no MPP limiter). */
+/* ---- positivity-preserving (MPP) flux limiter of the LxW path ---------- */
+
+/* GlobalWaveSpd, lib/2d/GlobalWaveSpd.cpp:20-80: the largest |s1|, |s2| of
+ * SetWaveSpd over the interior cells, each cell against itself. */
+void orc_global_wave_spd(const finess_params* p, const double* q, const double* aux, double* alpha1,
+                         double* alpha2)
+{
+    const int na = naux(p);
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = 1; i <= p->mx; i++)
+        for (int j = 1; j <= p->my; j++) {
+            const double* Q = &q[ix3(i, j, 1, p->my, p->meqn, p->mbc)];
+            const double* A = &aux[ix3(i, j, 1, p->my, na, p->mbc)];
+            double s1, s2;
+            orc_wave_speed(p, 1, Q, Q, A, A, &s1, &s2);
+            a1 = Max(a1, Max(fabs(s1), fabs(s2)));
+            orc_wave_speed(p, 2, Q, Q, A, A, &s1, &s2);
+            a2 = Max(a2, Max(fabs(s1), fabs(s2)));
+        }
+    *alpha1 = a1;
+    *alpha2 = a2;
+}
+
+/* ConstructLFL, lib/2d/ConstructLFL.cpp:25-159: Lax-Friedrichs interface
+ * fluxes with the global wave speeds (:112-128); smax(1,1,1) and smax(1,1,2)
+ * are set to those speeds (:62-64).  The Lstar it also forms (:140-158) is
+ * not used by the limited update and is not restated.  q carries its
+ * boundary values. */
+static void construct_lfl(const finess_params* p, const double* q, const double* aux, double* fLF,
+                          double* gLF, double* smax)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    double ax, ay;
+    orc_global_wave_spd(p, q, aux, &ax, &ay);
+    smax[ix3(1, 1, 1, my + 1, 2, mbc)] = ax;
+    smax[ix3(1, 1, 2, my + 1, 2, mbc)] = ay;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx + 1; i++)
+        for (int j = 1; j <= my; j++) {
+            double fa[MAXEQ], fb[MAXEQ], gd[MAXEQ];
+            orc_flux(p, &q[ix3(i - 1, j, 1, my, meqn, mbc)], &aux[ix3(i - 1, j, 1, my, na, mbc)], fa, gd);
+            orc_flux(p, &q[ix3(i, j, 1, my, meqn, mbc)], &aux[ix3(i, j, 1, my, na, mbc)], fb, gd);
+            for (int m = 1; m <= meqn; m++)
+                fLF[ix3(i, j, m, my, meqn, mbc)] = 0.5 * (fa[m - 1] + fb[m - 1]) +
+                    0.5 * ax * (q[ix3(i - 1, j, m, my, meqn, mbc)] - q[ix3(i, j, m, my, meqn, mbc)]);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my + 1; j++) {
+            double ga[MAXEQ], gb[MAXEQ], fd[MAXEQ];
+            orc_flux(p, &q[ix3(i, j - 1, 1, my, meqn, mbc)], &aux[ix3(i, j - 1, 1, my, na, mbc)], fd, ga);
+            orc_flux(p, &q[ix3(i, j, 1, my, meqn, mbc)], &aux[ix3(i, j, 1, my, na, mbc)], fd, gb);
+            for (int m = 1; m <= meqn; m++)
+                gLF[ix3(i, j, m, my + 1, meqn, mbc)] = 0.5 * (ga[m - 1] + gb[m - 1]) +
+                    0.5 * ay * (q[ix3(i, j - 1, m, my, meqn, mbc)] - q[ix3(i, j, m, my, meqn, mbc)]);
+        }
+}
+
+/* Pressure of a 2D Euler state as the limiter forms it. */
+static double mpp_pressure(double gm1, const double* Q)
+{
+    const double rho = Q[0], u1 = Q[1] / rho, u2 = Q[2] / rho, u3 = Q[3] / rho, energy = Q[4];
+    return gm1 * (energy - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+}
+
+/* ApplyMPPLimiter2D for 2D Euler, apps/2d/euler/lib/ApplyMPPLimiter2D.cpp:
+ * 27-286.  Per cell, the largest fractions amin[k] of the four
+ * (high - Lax-Friedrichs) flux differences that keep the density above
+ * 1e-12 (:56-95) and then the pressure non-negative for every combination
+ * of the four faces (16 combinations, 9 bisection steps each, :96-232); each
+ * face takes the smaller fraction of its two cells (:236-262) and its flux
+ * becomes theta (fHat - fLF) + fLF (:264-284).  fHat, gHat are limited in
+ * place. */
+static void apply_mpp_limiter_2d(const finess_params* p, double dt, const double* q, const double* fLF,
+                                 const double* gLF, double* fHat, double* gHat)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc;
+    const double eps = 1.0e-12, rho_min = eps;
+    const double dtx = dt / p->dx, dty = dt / p->dy;
+    const double gm1 = p->gamma - 1.0;
+    double* amin = (double*)malloc((size_t)mx * my * 4 * sizeof(double));
+#define FX(a, b, m) fHat[ix3(a, b, m, my, meqn, mbc)]
+#define FL(a, b, m) fLF[ix3(a, b, m, my, meqn, mbc)]
+#define GX(a, b, m) gHat[ix3(a, b, m, my + 1, meqn, mbc)]
+#define GL(a, b, m) gLF[ix3(a, b, m, my + 1, meqn, mbc)]
+#define AM(a, b, k) amin[((size_t)((a) - 1) * my + ((b) - 1)) * 4 + (k)]
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my; j++) {
+            /* density: the Lax-Friedrichs update's margin above rho_min */
+            const double gmin = rho_min - (q[ix3(i, j, 1, my, meqn, mbc)] + dtx * (FL(i, j, 1) - FL(i + 1, j, 1)) +
+                                           dty * (GL(i, j, 1) - GL(i, j + 1, 1)));
+            double ff[4];
+            ff[0] = dtx * (FX(i, j, 1) - FL(i, j, 1));
+            ff[1] = -dtx * (FX(i + 1, j, 1) - FL(i + 1, j, 1));
+            ff[2] = dty * (GX(i, j, 1) - GL(i, j, 1));
+            ff[3] = -dty * (GX(i, j + 1, 1) - GL(i, j + 1, 1));
+            int neg[4];
+            for (int k = 0; k < 4; k++) neg[k] = ff[k] < 0.0;
+            double ffsum = (double)neg[0] * ff[0];
+            for (int k = 1; k < 4; k++) ffsum = ffsum + (double)neg[k] * ff[k];
+            double a[4];
+            for (int k = 0; k < 4; k++) a[k] = neg[k] ? Min(gmin / (ffsum - eps), 1.e0) : 1.e0;
+
+            /* pressure: every on/off combination of the four faces */
+            double qlf[MAXEQ], fh[4][MAXEQ], fl[4][MAXEQ], q0[MAXEQ];
+            for (int m = 1; m <= meqn; m++) {
+                qlf[m - 1] = q[ix3(i, j, m, my, meqn, mbc)] + dtx * (FL(i, j, m) - FL(i + 1, j, m)) +
+                             dty * (GL(i, j, m) - GL(i, j + 1, m));
+                fh[0][m - 1] = FX(i, j, m);
+                fh[1][m - 1] = FX(i + 1, j, m);
+                fh[2][m - 1] = GX(i, j, m);
+                fh[3][m - 1] = GX(i, j + 1, m);
+                fl[0][m - 1] = FL(i, j, m);
+                fl[1][m - 1] = FL(i + 1, j, m);
+                fl[2][m - 1] = GL(i, j, m);
+                fl[3][m - 1] = GL(i, j + 1, m);
+                q0[m - 1] = q[ix3(i, j, m, my, meqn, mbc)];
+            }
+            double rescale[16];
+            for (int c = 0; c < 16; c++) {
+                /* c = 8 i1 + 4 i2 + 2 i3 + i4: face k is on when bit (3 - k) is set */
+                double aa[4];
+                for (int k = 0; k < 4; k++) aa[k] = (double)((c >> (3 - k)) & 1) * a[k];
+                double qh[MAXEQ];
+                for (int m = 0; m < meqn; m++) {
+                    double f4[4];
+                    for (int k = 0; k < 4; k++) f4[k] = aa[k] * fh[k][m] + (1.0 - aa[k]) * fl[k][m];
+                    qh[m] = q0[m] + dtx * (f4[0] - f4[1]) + dty * (f4[2] - f4[3]);
+                }
+                double phigh = mpp_pressure(gm1, qh);
+                rescale[c] = 1.0;
+                if (phigh < 0.0) {
+                    double root1 = 0.0, root2 = 1.0, rate = 0.0;
+                    for (int nn = 1; nn < 10; nn++) {
+                        rate = (root1 + root2) / 2.0;
+                        double qt[MAXEQ];
+                        for (int m = 0; m < meqn; m++) qt[m] = rate * qh[m] + (1.0 - rate) * qlf[m];
+                        phigh = mpp_pressure(gm1, qt);
+                        if (phigh < 0.0) root2 = rate;
+                        else root1 = rate;
+                    }
+                    if (!(phigh > 0.0)) rate = root1;
+                    rescale[c] = rate;
+                }
+            }
+            /* face k: the smallest rescale over the combinations with it on */
+            for (int k = 0; k < 4; k++) {
+                double r = 1.0;
+                for (int c = 0; c < 16; c++)
+                    if ((c >> (3 - k)) & 1) r = Min(r, rescale[c]);
+                AM(i, j, k) = r * a[k];
+            }
+        }
+    /* theta per face from its two cells, then the limited fluxes */
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx + 1; i++)
+        for (int j = 1; j <= my; j++) {
+            const double th = i == 1 ? AM(1, j, 0) : i == mx + 1 ? AM(mx, j, 1) : Min(AM(i, j, 0), AM(i - 1, j, 1));
+            for (int m = 1; m <= meqn; m++) FX(i, j, m) = th * (FX(i, j, m) - FL(i, j, m)) + FL(i, j, m);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my + 1; j++) {
+            const double th = j == 1 ? AM(i, 1, 2) : j == my + 1 ? AM(i, my, 3) : Min(AM(i, j, 2), AM(i, j - 1, 3));
+            for (int m = 1; m <= meqn; m++) GX(i, j, m) = th * (GX(i, j, m) - GL(i, j, m)) + GL(i, j, m);
+        }
+#undef FX
+#undef FL
+#undef GX
+#undef GL
+#undef AM
+    free(amin);
+}
+
+static size_t xface_numel(const finess_params* p)
+{
+    return (size_t)(p->mx + 1 + 2 * p->mbc) * (p->my + 2 * p->mbc) * p->meqn;
+}
+static size_t yface_numel(const finess_params* p)
+{
+    return (size_t)(p->mx + 2 * p->mbc) * (p->my + 1 + 2 * p->mbc) * p->meqn;
+}
+
+/* The interface fluxes of one limited LxW step (q carries its boundary
+ * values): high-order Fh, Gh (ConstructLxWL), Lax-Friedrichs Fl, Gl
+ * (ConstructLFL) and the limited Fm, Gm (ApplyMPPLimiter2D); smax as the
+ * reference leaves it.  Arrays as construct_l_2d_faces. */
+static int lxw_mpp_faces(const finess_params* p, double dt, const double* q, const double* aux, double* F,
+                         double* G, double* L, double* smax, double* Fh, double* Gh, double* Fl, double* Gl,
+                         double* Fm, double* Gm)
+{
+    int rc;
+    if ((rc = orc_construct_integrated_r(p, dt, q, aux, F, G))) return rc;
+    if ((rc = construct_l_2d_faces(p, q, aux, F, G, L, smax, Fm, Gm))) return rc;
+    if (Fh) memcpy(Fh, Fm, xface_numel(p) * sizeof(double));
+    if (Gh) memcpy(Gh, Gm, yface_numel(p) * sizeof(double));
+    construct_lfl(p, q, aux, Fl, Gl, smax);
+    apply_mpp_limiter_2d(p, dt, q, Fl, Gl, Fm, Gm);
+    return FINESS_OK;
+}
+
+int orc_lxw_mpp_faces(const finess_params* p, double dt, const double* q, const double* aux, double* Fh,
+                      double* Gh, double* Fl, double* Gl, double* Fm, double* Gm)
+{
+    int rc = check_config(p);
+    if (rc) return rc;
+    const size_t numel = state_numel(p, p->meqn);
+    double* F = (double*)calloc(numel, sizeof(double));
+    double* G = (double*)calloc(numel, sizeof(double));
+    double* L = (double*)calloc(numel, sizeof(double));
+    double* smax = (double*)calloc(smax_numel(p), sizeof(double));
+    rc = lxw_mpp_faces(p, dt, q, aux, F, G, L, smax, Fh, Gh, Fl, Gl, Fm, Gm);
+    free(F); free(G); free(L); free(smax);
+    return rc;
+}
+
+/* One LxW step, lib/2d/FinSolveLxW.cpp:122-171; with mpp_limiter the
+ * limited fluxes update the interior cells, x then y (:126-157). */
 static int lxw_step(const finess_params* p, double* qnew, double* aux, double* F, double* G,
                     double* L, double* smax, double dt)
 {
@@ -730,6 +964,29 @@ static int lxw_step(const finess_params* p, double* qnew, double* aux, double* F
     int rc;
     orc_set_bnd(p, qnew, aux);
     if ((rc = orc_construct_integrated_r(p, dt, qnew, aux, F, G))) return rc;
+    if (p->mpp_limiter) {
+        const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc;
+        double* Fl = (double*)calloc(xface_numel(p), sizeof(double));
+        double* Gl = (double*)calloc(yface_numel(p), sizeof(double));
+        double* Fm = (double*)calloc(xface_numel(p), sizeof(double));
+        double* Gm = (double*)calloc(yface_numel(p), sizeof(double));
+        rc = lxw_mpp_faces(p, dt, qnew, aux, F, G, L, smax, NULL, NULL, Fl, Gl, Fm, Gm);
+        if (!rc) {
+            const double dtdx = dt / p->dx, dtdy = dt / p->dy;
+#pragma omp parallel for schedule(static)
+            for (int i = 1; i <= mx; i++)
+                for (int j = 1; j <= my; j++)
+                    for (int m = 1; m <= meqn; m++) {
+                        double* Q = &qnew[ix3(i, j, m, my, meqn, mbc)];
+                        double tmp = Fm[ix3(i + 1, j, m, my, meqn, mbc)] - Fm[ix3(i, j, m, my, meqn, mbc)];
+                        *Q = *Q - dtdx * tmp;
+                        tmp = Gm[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Gm[ix3(i, j, m, my + 1, meqn, mbc)];
+                        *Q = *Q - dtdy * tmp;
+                    }
+        }
+        free(Fl); free(Gl); free(Fm); free(Gm);
+        return rc;
+    }
     if ((rc = orc_construct_lxw_l(p, qnew, aux, F, G, L, smax))) return rc;
 #pragma omp parallel for schedule(static)
     for (long long k = 0; k < (long long)numel; k++) qnew[k] = qnew[k] + dt * L[k];

@@ -67,6 +67,19 @@ int orc_construct_integrated_r(const finess_params* p, double dt, const double* 
 int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux,
                         const double* F, const double* G, double* Lstar, double* smax);
 
+/* Positivity-preserving limiter of the LxW path (2D Euler): GlobalWaveSpd
+ * (lib/2d/GlobalWaveSpd.cpp:20-80), and the interface fluxes of one limited
+ * step of q (boundary values set): high-order Fh, Gh, Lax-Friedrichs Fl, Gl
+ * (lib/2d/ConstructLFL.cpp:25-159), limited Fm, Gm
+ * (apps/2d/euler/lib/ApplyMPPLimiter2D.cpp:27-286).  x-face arrays
+ * (mx+1+2mbc)(my+2mbc)meqn, y-face arrays (mx+2mbc)(my+1+2mbc)meqn, index
+ * (i, j, m) as the state's with my+1 in place of my for y faces.  Any output
+ * may be NULL except Fm, Gm. */
+void orc_global_wave_spd(const finess_params* p, const double* q, const double* aux, double* alpha1,
+                         double* alpha2);
+int orc_lxw_mpp_faces(const finess_params* p, double dt, const double* q, const double* aux, double* Fh,
+                      double* Gh, double* Fl, double* Gl, double* Fm, double* Gm);
+
 /* ---- troubled-interface indicator (no counterpart in the reference) ----
  * qs: the 6 stencil states q(i-3..i+2) across a face, [6][meqn]. */
 void   orc_face_features(const finess_params* p, int ixy, const double* qs, float* feat /* [24] */);

@@ -60,6 +60,8 @@ def oracle():
             "orc_run_fixed_steps": (ctypes.c_int, [PP, PD, PD, ctypes.c_double, ctypes.c_int, PD]),
             "orc_construct_integrated_r": (ctypes.c_int, [PP, ctypes.c_double, PD, PD, PD, PD]),
             "orc_construct_lxw_l": (ctypes.c_int, [PP, PD, PD, PD, PD, PD, PD]),
+            "orc_global_wave_spd": (None, [PP, PD, PD, PD, PD]),
+            "orc_lxw_mpp_faces": (ctypes.c_int, [PP, ctypes.c_double, PD, PD, PD, PD, PD, PD, PD, PD]),
             "orc_num_threads": (ctypes.c_int, []),
             "orc_face_features": (None, [PP, ctypes.c_int, PD, PF]),
             "orc_face_departure": (ctypes.c_double, [PP, ctypes.c_int, PD]),
@@ -145,6 +147,31 @@ def integrated_r(p, dt, q, aux=None):
     rc = oracle().orc_construct_integrated_r(ctypes.byref(p), float(dt), ptr(q), ptr(aux), ptr(F), ptr(G))
     assert rc == 0, rc
     return F, G
+
+
+def global_wave_spd(p, q, aux=None):
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    a1, a2 = ctypes.c_double(), ctypes.c_double()
+    oracle().orc_global_wave_spd(ctypes.byref(p), ptr(q), ptr(aux), ctypes.byref(a1), ctypes.byref(a2))
+    return a1.value, a2.value
+
+
+def face_shapes(p):
+    """Shapes of the x-face and y-face flux arrays (ghosts included)."""
+    g = p.mbc
+    return (p.mx + 1 + 2 * g, p.my + 2 * g, p.meqn), (p.mx + 2 * g, p.my + 1 + 2 * g, p.meqn)
+
+
+def lxw_mpp_faces(p, dt, q, aux=None):
+    """High-order, Lax-Friedrichs and limited interface fluxes of one limited
+    LxW step of q (boundary values set): dict of x-face / y-face arrays."""
+    aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    sx, sy = face_shapes(p)
+    out = {k: np.zeros(sx if k[0] == "F" else sy) for k in ("Fh", "Gh", "Fl", "Gl", "Fm", "Gm")}
+    rc = oracle().orc_lxw_mpp_faces(ctypes.byref(p), float(dt), ptr(q), ptr(aux), ptr(out["Fh"]), ptr(out["Gh"]),
+                                    ptr(out["Fl"]), ptr(out["Gl"]), ptr(out["Fm"]), ptr(out["Gm"]))
+    assert rc == 0, rc
+    return out
 
 
 def emu_construct_l(p, q):

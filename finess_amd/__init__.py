@@ -130,6 +130,7 @@ SIGNATURES = {
     "finess_construct_l": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, _PD]),
     "finess_fin_solve_rk": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_fin_solve_lxw": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
+    "finess_apply_mpp_limiter_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD, _PD]),
     "finess_construct_integrated_r": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD]),
     "finess_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, ctypes.POINTER(Stats)]),
     "finess_advance_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _PD, ctypes.POINTER(Stats)]),
@@ -281,6 +282,16 @@ class Solver:
         self._check(self.lib.finess_construct_integrated_r(self.h, float(dt), _ptr(q), None, _ptr(F), _ptr(G)),
                     "construct_integrated_r")
         return F, G
+
+    def apply_mpp_limiter_2d(self, dt, q, fLF, gLF, fHat, gHat):
+        """ApplyMPPLimiter2D: limits fHat, gHat (reference layouts with
+        ghosts) in place toward fLF, gLF for the update of q by dt."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, fLF, gLF)]
+        for a in (fHat, gHat):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        self._check(self.lib.finess_apply_mpp_limiter_2d(self.h, float(dt), _ptr(arrs[0]), _ptr(arrs[1]),
+                                                         _ptr(arrs[2]), _ptr(fHat), _ptr(gHat)),
+                    "apply_mpp_limiter_2d")
 
     def fin_solve_rk(self, q, t, tend, dtv, aux=None):
         """FinSolveRK: advances q in place from t to tend; returns t."""

@@ -60,6 +60,10 @@ struct finess_ctx {
     int* route_list = nullptr;
     unsigned long long* route_n = nullptr;
     double* route_flux = nullptr;
+    // limited Lax-Wendroff: interface fluxes (high-order, then limited in
+    // place; Lax-Friedrichs), per-cell fractions, global wave-speed words
+    double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
+    unsigned long long* mpp_gmax = nullptr;
     StepCtl* ctl = nullptr;                       // device controller block (2D)
     bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
@@ -280,6 +284,49 @@ int lxw_fluxes(finess_ctx* c, int src, double dt)
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "LxW flux launch failed") : 0;
 }
 
+// The limited Lax-Wendroff step (lib/2d/FinSolveLxW.cpp:126-157): the stage
+// kernel writes the high-order interface fluxes (no update), then the
+// limiter (mpp2d.hip) forms the Lax-Friedrichs fluxes, limits and updates
+// buf[src] into buf[dst].
+int mpp_step(finess_ctx* c, int src, int dst, double dt)
+{
+    const finess_params& p = c->p;
+    StageArgs a = base_stage_args(c);
+    a.src = c->buf[src];
+    a.qa = c->buf[src];
+    a.a_is_src = 1;
+    a.a1 = 1.0;
+    a.bdt = dt;
+    a.smax = c->smax;
+    a.lxw_F = c->lxwF;
+    a.lxw_G = c->lxwG;
+    a.face_x = c->mpp_fh;
+    a.face_y = c->mpp_gh;
+    if (tmark(c)) return FINESS_ERR_HIP;
+    int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 1, a, c->st);
+    if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "LxW face-flux launch failed");
+    MppLaunch m;
+    m.q = c->buf[src];
+    m.out = c->buf[dst];
+    m.mx = p.mx;
+    m.my = p.my;
+    for (int k = 0; k < 4; k++) m.bc[k] = p.bc[k];
+    m.dx = p.dx;
+    m.dy = p.dy;
+    m.dt = dt;
+    m.gamma = p.gamma;
+    m.fh = c->mpp_fh;
+    m.gh = c->mpp_gh;
+    m.fl = c->mpp_fl;
+    m.gl = c->mpp_gl;
+    m.amin = c->mpp_amin;
+    m.gmax = c->mpp_gmax;
+    m.smax = c->smax;
+    if (launch_mpp(m, c->st)) return fail(c, FINESS_ERR_HIP, "limiter launch failed");
+    if (tmark(c)) return FINESS_ERR_HIP;
+    return 0;
+}
+
 // Enqueues one full time step from buf[cur]; returns the index holding the
 // result in *res and the slot holding the CFL speeds in *slot.
 int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
@@ -293,7 +340,11 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     c->route_fresh = true;
     if (p.time_stepping_method == FINESS_TS_LXW) {
         if ((rc = lxw_fluxes(c, A, dt))) return rc;
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        if (p.mpp_limiter) {
+            if ((rc = mpp_step(c, A, B, dt))) return rc;
+        } else if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) {
+            return rc;
+        }
         *res = B;
         *slot = 0;
         return 0;
@@ -530,6 +581,15 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6) * sizeof(double);
         rc = alloc(c, (void**)&c->lxwF, lb);
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
+        if (!rc && p.mpp_limiter) {
+            const size_t nfx = (size_t)(p.mx + 1) * p.my, nfy = (size_t)p.mx * (p.my + 1);
+            rc = alloc(c, (void**)&c->mpp_fh, nfx * c->M * sizeof(double));
+            if (!rc) rc = alloc(c, (void**)&c->mpp_fl, nfx * c->M * sizeof(double));
+            if (!rc) rc = alloc(c, (void**)&c->mpp_gh, nfy * c->M * sizeof(double));
+            if (!rc) rc = alloc(c, (void**)&c->mpp_gl, nfy * c->M * sizeof(double));
+            if (!rc) rc = alloc(c, (void**)&c->mpp_amin, (size_t)4 * p.mx * p.my * sizeof(double));
+            if (!rc) rc = alloc(c, (void**)&c->mpp_gmax, 2 * CFL_SPREAD * sizeof(unsigned long long));
+        }
     }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
     if (!rc && p.ndims == 2) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
@@ -571,6 +631,12 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->route_list);
     hipFree(c->route_n);
     hipFree(c->ctl);
+    hipFree(c->mpp_fh);
+    hipFree(c->mpp_gh);
+    hipFree(c->mpp_fl);
+    hipFree(c->mpp_gl);
+    hipFree(c->mpp_amin);
+    hipFree(c->mpp_gmax);
     hipFree(c->route_flux);
     if (c->smax_h) hipHostFree(c->smax_h);
     for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -581,6 +647,66 @@ void finess_destroy(finess_ctx* c)
 }
 
 const char* finess_last_error(const finess_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int finess_apply_mpp_limiter_2d(finess_ctx* c, double dt, const double* q, const double* fLF, const double* gLF,
+                                double* fHat, double* gHat)
+{
+    if (!c || !q || !fLF || !gLF || !fHat || !gHat) return FINESS_ERR_ARG;
+    const finess_params& p = c->p;
+    if (!c->mpp_fh) return fail(c, FINESS_ERR_CONFIG, "no limiter on this context (2D Euler, Lax-Wendroff, mpp_limiter)");
+    const int mx = p.mx, my = p.my, g = p.mbc, M = c->M;
+    const size_t nfx = (size_t)(mx + 1) * my, nfy = (size_t)mx * (my + 1);
+    // reference layout (i, j, m), 0-based interior indices, leading dimension ld
+    auto ref = [&](int i, int j, int m, int ld) { return ((size_t)(i + g) * (ld + 2 * g) + (j + g)) * M + m; };
+    std::vector<double> hx(nfx * M), lx(nfx * M), hy(nfy * M), ly(nfy * M);
+    for (int m = 0; m < M; m++)
+        for (int i = 0; i <= mx; i++)
+            for (int j = 0; j < my; j++) {
+                hx[m * nfx + (size_t)i * my + j] = fHat[ref(i, j, m, my)];
+                lx[m * nfx + (size_t)i * my + j] = fLF[ref(i, j, m, my)];
+            }
+    for (int m = 0; m < M; m++)
+        for (int i = 0; i < mx; i++)
+            for (int j = 0; j <= my; j++) {
+                hy[m * nfy + (size_t)i * (my + 1) + j] = gHat[ref(i, j, m, my + 1)];
+                ly[m * nfy + (size_t)i * (my + 1) + j] = gLF[ref(i, j, m, my + 1)];
+            }
+    double* qd = c->buf[(c->cur + 2) % 3];  // not the held state
+    HIPCHK(c, hipMemcpyAsync(c->staging, q, ref_numel(p, M) * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (launch_ref_to_planes(c->staging, qd, mx, my, M, g, 0, c->st)) return fail(c, FINESS_ERR_HIP, "layout conversion failed");
+    HIPCHK(c, hipMemcpyAsync(c->mpp_fh, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->mpp_fl, lx.data(), lx.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->mpp_gh, hy.data(), hy.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->mpp_gl, ly.data(), ly.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    MppLaunch m;
+    m.q = qd;
+    m.out = nullptr;
+    m.mx = mx;
+    m.my = my;
+    for (int k = 0; k < 4; k++) m.bc[k] = p.bc[k];
+    m.dx = p.dx;
+    m.dy = p.dy;
+    m.dt = dt;
+    m.gamma = p.gamma;
+    m.fh = c->mpp_fh;
+    m.gh = c->mpp_gh;
+    m.fl = c->mpp_fl;
+    m.gl = c->mpp_gl;
+    m.amin = c->mpp_amin;
+    m.gmax = c->mpp_gmax;
+    m.smax = c->smax;
+    if (launch_mpp_limit(m, c->st)) return fail(c, FINESS_ERR_HIP, "limiter launch failed");
+    HIPCHK(c, hipMemcpyAsync(hx.data(), c->mpp_fh, hx.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(hy.data(), c->mpp_gh, hy.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int mm = 0; mm < M; mm++)
+        for (int i = 0; i <= mx; i++)
+            for (int j = 0; j < my; j++) fHat[ref(i, j, mm, my)] = hx[mm * nfx + (size_t)i * my + j];
+    for (int mm = 0; mm < M; mm++)
+        for (int i = 0; i < mx; i++)
+            for (int j = 0; j <= my; j++) gHat[ref(i, j, mm, my + 1)] = hy[mm * nfy + (size_t)i * (my + 1) + j];
+    return FINESS_OK;
+}
 
 int finess_set_state(finess_ctx* c, const double* q, const double* aux, double t)
 {

@@ -32,6 +32,22 @@ int mlp_zero(float* p, long long n, hipStream_t st);
 
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 
+// Positivity-preserving limiter of the 2D Lax-Wendroff step, Euler
+// (mpp2d.hip): from the high-order interface fluxes fh, gh (x faces
+// [m][(mx+1) my], y faces [m][mx (my+1)]) to the limited update q -> out.
+struct MppLaunch {
+    const double* q;
+    double* out;
+    int mx, my;
+    int bc[4];
+    double dx, dy, dt, gamma;
+    double *fh, *gh, *fl, *gl, *amin;
+    unsigned long long *gmax, *smax;
+};
+int launch_mpp(const MppLaunch& L, hipStream_t st);
+// the limiter alone: fh, gh limited in place from fl, gl (q, dt given)
+int launch_mpp_limit(const MppLaunch& L, hipStream_t st);
+
 // Device-side time-step controller (step_control.hip).
 struct StepCtl;
 struct StepCtlParams;

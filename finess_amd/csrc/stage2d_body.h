@@ -115,6 +115,12 @@ struct StageArgs {
     double* Lout;          // L planes (mx*my)
     double* smax_faces_x;  // (mx+1) x my alphas of x-faces, row-major i*(my)+j
     double* smax_faces_y;  // mx x (my+1)
+    // limited Lax-Wendroff (null: off): the interface fluxes are written
+    // (x faces [m][(mx+1) my], y faces [m][mx (my+1)]) for the limiter, and
+    // the faces at (0, 0) stay out of the CFL maxima, whose value there the
+    // limiter sets to the global wave speeds
+    double* face_x;
+    double* face_y;
 };
 
 // beta * dt of the stage update (dt from the device controller when set)
@@ -354,13 +360,27 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
     if (!g.valid) return;
     const int fi = g.fi, fj = g.fj, base = g.base, stride = g.stride;
     const long long lbase = g.lbase, lstride = g.lstride;
-    const bool count = g.count;
+    bool count = g.count;
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
     const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F, lbase,
                                                     lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh);
     (void)sizeof(S);
     for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
+    if constexpr (LXW) {
+        if (a.face_x) {
+            const int gi = i0 + fi, gj = j0 + fj;
+            if (gi == 0 && gj == 0) count = false;
+            // a face shared by two tiles is written by the tile for which it is not the far edge
+            const bool own = dir ? (fj < TJ || gj == a.my) : (fi < TI || gi == a.mx);
+            if (own) {
+                const long long nf = dir ? (long long)a.mx * (a.my + 1) : (long long)(a.mx + 1) * a.my;
+                const long long f = dir ? (long long)gi * (a.my + 1) + gj : (long long)gi * a.my + gj;
+                double* dst = dir ? a.face_y : a.face_x;
+                for (int m = 0; m < M; m++) dst[m * nf + f] = double(fh[m]);
+            }
+        }
+    }
     if (count) amax = rmax(amax, double(alpha));
     if (dir == 0 && a.smax_faces_x) a.smax_faces_x[(long long)(i0 + fi) * a.my + (j0 + fj)] = double(alpha);
     if (dir == 1 && a.smax_faces_y) a.smax_faces_y[(long long)(i0 + fi) * (a.my + 1) + (j0 + fj)] = double(alpha);

@@ -161,6 +161,17 @@ int finess_fin_solve_lxw(finess_ctx* ctx, double* q, const double* aux,
 int finess_construct_integrated_r(finess_ctx* ctx, double dt, const double* q,
                                   const double* aux, double* F, double* G);
 
+/* ApplyMPPLimiter2D for 2D Euler (apps/2d/euler/lib/ApplyMPPLimiter2D.cpp:27-286;
+ * an overridable object of the reference, lib/2d/Makefile.defs:109): limits the
+ * high-order interface fluxes fHat (dTensorBC3(mx+1,my,meqn,mbc)) and gHat
+ * (dTensorBC3(mx,my+1,meqn,mbc)) in place toward the Lax-Friedrichs fluxes fLF,
+ * gLF (same shapes) so that the update of q (dTensorBC3(mx,my,meqn,mbc)) by dt
+ * keeps density and pressure positive.  Interior faces only.  The context must
+ * be a 2D Euler Lax-Wendroff context with mpp_limiter set; its state is kept. */
+int finess_apply_mpp_limiter_2d(finess_ctx* ctx, double dt, const double* q,
+                                const double* fLF, const double* gLF,
+                                double* fHat, double* gHat);
+
 /* ---- device-resident stepping ------------------------------------------ */
 typedef struct finess_stats {
     long long steps_accepted;

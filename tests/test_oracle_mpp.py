@@ -156,6 +156,26 @@ def test_positivity_where_the_plain_step_fails(case):
         assert rho.min() > 0 and pr.min() >= 0
 
 
+def test_limited_step_conditioning():
+    """The limited step is discontinuous in its input (sign tests, bisection):
+    a 1e-15 relative perturbation of the state moves the Sedov blast by more
+    than 1e-9 within two steps, and the near-vacuum double rarefaction by
+    more than 1e-5 within 12 steps.  This bounds what any other
+    implementation can match bit for bit."""
+    rng = np.random.default_rng(1)
+    p, q, dt = _sedov_after(0)
+    qa, qb = q.copy(), q * (1 + 1e-15 * rng.standard_normal(q.shape))
+    orc.fixed_steps(p, qa, dt, 2)
+    orc.fixed_steps(p, qb, dt, 2)
+    assert orc.rel_linf(_interior(p, qa), _interior(p, qb)).max() > 1e-9
+    p, q, _ = problems.double_rarefaction(mx=40, my=32, u0=6.0, mpp_limiter=1)
+    dt = 0.3 / 40 / 7.4
+    qa, qb = q.copy(), q * (1 + 1e-15 * rng.standard_normal(q.shape))
+    orc.fixed_steps(p, qa, dt, 12)
+    orc.fixed_steps(p, qb, dt, 12)
+    assert orc.rel_linf(_interior(p, qa), _interior(p, qb)).max() > 1e-5
+
+
 def test_limited_update_is_conservative():
     p, q, dt = _sedov_after(2)
     f = orc.lxw_mpp_faces(p, dt, q)

@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("FINESS_HIP_LIB") or os.path.join(_HERE, "libfiness_hi
 # status codes (include/finess_hip.h)
 OK, ERR_CONFIG, ERR_NONFINITE, ERR_NV, ERR_HIP, ERR_DT, ERR_CFL, ERR_ARG, ERR_NODEVICE = 0, -1, -2, -3, -4, -5, -6, -7, -8
 WENO_JS, WENO_FD, WENO_Z = 0, 1, 2
-TS_RK, TS_LXW = 0, 1
+TS_RK, TS_LXW, TS_PIF = 0, 1, 2
 PHYS_ADVECTION, PHYS_EULER, PHYS_SHALLOW_WATER = 0, 1, 2
 BC_EXTRAP, BC_PERIODIC, BC_REFLECT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1

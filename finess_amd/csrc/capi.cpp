@@ -513,9 +513,14 @@ int finess_params_finalize(finess_params* p)
     if (p->mbc < 3) return FINESS_ERR_CONFIG;
     // the boundary map folds ghost indices g in [-n, 2n-1] only
     if (p->mbc > p->mx || (p->ndims == 2 && p->mbc > p->my)) return FINESS_ERR_CONFIG;
+    if (p->time_stepping_method < FINESS_TS_RK || p->time_stepping_method > FINESS_TS_PIF) return FINESS_ERR_CONFIG;
     if (p->time_stepping_method == FINESS_TS_LXW) {
         if (p->ndims != 2 || p->mbc < 5 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
         if (p->time_order < 2 || p->time_order > 3) return FINESS_ERR_CONFIG;
+    } else if (p->time_stepping_method == FINESS_TS_PIF) {
+        // RK4-PIF exists for time_order 4 only (lib/2d/FinSolveRK_PIF.cpp:151-206)
+        if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || p->time_order != 4) return FINESS_ERR_CONFIG;
+        if (p->precision != FINESS_PREC_FP64) return FINESS_ERR_CONFIG;
     } else if (p->time_order < 1 || p->time_order > 3) {
         return FINESS_ERR_CONFIG;
     }

@@ -41,6 +41,7 @@ extern "C" {
 
 #define FINESS_TS_RK   0
 #define FINESS_TS_LXW  1
+#define FINESS_TS_PIF  2  /* FinSolveRK_PIF (lib/2d/FinSolveRK_PIF.cpp:66-295): 2D, time_order 4 */
 
 /* physics compiled into the kernels (the app callback libraries):
  *   ADVECTION      apps/1d/advection/lib  f = aux(1)*q           (1D, meqn=1)

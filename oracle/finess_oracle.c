@@ -993,14 +993,171 @@ static int lxw_step(const finess_params* p, double* qnew, double* aux, double* F
     return FINESS_OK;
 }
 
-/* Shared controller of FinSolveRK / FinSolveLxW: lib/2d/FinSolveRK.cpp:77-367,
- * lib/2d/FinSolveLxW.cpp:100-229, lib/1d/FinSolveRK.cpp:88-363. */
-static int fin_solve(const finess_params* p, int lxw, double* qnew, double* aux, double* tio,
+/* ---- FinSolveRK_PIF (lib/2d/FinSolveRK_PIF.cpp) --------------------------- */
+
+/* ConstructL_NOC, lib/2d/FinSolveRK_PIF.cpp:723-977: component-wise WENO of
+ * the flux without characteristic projection and without wave-speed
+ * splitting (gp = f/2 over cells i-3..i+1, gm = f/2 over i+2..i-2); smax is
+ * not touched. */
+static int construct_l_noc(const finess_params* p, const double* q, const double* aux, double* Lstar)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p), ws = 5;
+    const double dx = p->dx, dy = p->dy;
+    double* Fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn, sizeof(double));
+    double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
+    if (!Fhat || !Ghat) { free(Fhat); free(Ghat); return FINESS_ERR_ARG; }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx + 1; i++)
+        for (int j = 1; j <= my; j++) {
+            double fs[6][MAXEQ], gd[MAXEQ];
+            for (int s = 1; s <= ws + 1; s++) {
+                const int is = i - 1 + s - 3;
+                orc_flux(p, &q[ix3(is, j, 1, my, meqn, mbc)], &aux[ix3(is, j, 1, my, na, mbc)], fs[s - 1], gd);
+            }
+            for (int m = 0; m < meqn; m++) {
+                double gp[5], gm[5];
+                for (int s = 1; s <= ws; s++) {
+                    gp[s - 1] = 0.5 * fs[s - 1][m];
+                    gm[s - 1] = 0.5 * fs[ws - s + 1][m];
+                }
+                Fhat[ix3(i, j, m + 1, my, meqn, mbc)] = orc_weno5(p->weno_version, p->epsilon, p->power_param, gp) +
+                                                        orc_weno5(p->weno_version, p->epsilon, p->power_param, gm);
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my + 1; j++) {
+            double gs[6][MAXEQ], fd[MAXEQ];
+            for (int s = 1; s <= ws + 1; s++) {
+                const int js = j - 1 + s - 3;
+                orc_flux(p, &q[ix3(i, js, 1, my, meqn, mbc)], &aux[ix3(i, js, 1, my, na, mbc)], fd, gs[s - 1]);
+            }
+            for (int m = 0; m < meqn; m++) {
+                double gp[5], gm[5];
+                for (int s = 1; s <= ws; s++) {
+                    gp[s - 1] = 0.5 * gs[s - 1][m];
+                    gm[s - 1] = 0.5 * gs[ws - s + 1][m];
+                }
+                Ghat[ix3(i, j, m + 1, my + 1, meqn, mbc)] = orc_weno5(p->weno_version, p->epsilon, p->power_param, gp) +
+                                                            orc_weno5(p->weno_version, p->epsilon, p->power_param, gm);
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my; j++)
+            for (int m = 1; m <= meqn; m++) {
+                double tmp = -(Fhat[ix3(i + 1, j, m, my, meqn, mbc)] - Fhat[ix3(i, j, m, my, meqn, mbc)]) / dx;
+                tmp = tmp - (Ghat[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Ghat[ix3(i, j, m, my + 1, meqn, mbc)]) / dy;
+                Lstar[ix3(i, j, m, my, meqn, mbc)] = tmp;
+            }
+    free(Fhat);
+    free(Ghat);
+    return FINESS_OK;
+}
+
+/* EulerStep, lib/2d/FinSolveRK_PIF.cpp:980-1004: qnew = qold + dt L on the
+ * interior. */
+static void euler_step(const finess_params* p, double dt, const double* qold, const double* L, double* qnew)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= mx; i++)
+        for (int j = 1; j <= my; j++)
+            for (int m = 1; m <= meqn; m++) {
+                const size_t k = ix3(i, j, m, my, meqn, mbc);
+                qnew[k] = qold[k] + dt * L[k];
+            }
+}
+
+/* Adds w times the flux of q (FluxFunc sampled over every cell, ghosts
+ * included, after the boundary values are set) to the running sums Fs, Gs;
+ * first: the sums start from 0 (lib/2d/FinSolveRK_PIF.cpp:155-186). */
+static void pif_flux_sum(const finess_params* p, const double* q, const double* aux, double w, int first,
+                         double* Fs, double* Gs)
+{
+    const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+#pragma omp parallel for schedule(static)
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int j = 1 - mbc; j <= my + mbc; j++) {
+            double f[MAXEQ], g[MAXEQ];
+            orc_flux(p, &q[ix3(i, j, 1, my, meqn, mbc)], &aux[ix3(i, j, 1, my, na, mbc)], f, g);
+            for (int m = 1; m <= meqn; m++) {
+                const size_t k = ix3(i, j, m, my, meqn, mbc);
+                if (first) { Fs[k] = 0.; Gs[k] = 0.; }
+                Fs[k] += w * f[m - 1];
+                Gs[k] += w * g[m - 1];
+            }
+        }
+}
+
+/* One RK4-PIF step, lib/2d/FinSolveRK_PIF.cpp:151-206 (time_order 4): three
+ * non-characteristic stages to q2, q3, q4, the time-averaged fluxes
+ * (R1 + 2 R2 + 2 R3 + R4)/6, then characteristic WENO on them (ConstructL
+ * with fstar, gstar, :308-715, the eigen-systems and wave speeds from q^n)
+ * and qnew = qold + dt L.  qnew is q^n on entry (its boundary values get
+ * set), qstar, Fs, Gs, L are scratch. */
+static int pif_step(const finess_params* p, double* qnew, double* qstar, double* aux, double* Fs, double* Gs,
+                    double* L, double* smax, double dt)
+{
+    const size_t numel = state_numel(p, p->meqn);
+    int rc;
+    double* qold = (double*)malloc(numel * sizeof(double));
+    if (!qold) return FINESS_ERR_ARG;
+    memcpy(qold, qnew, numel * sizeof(double));
+    orc_set_bnd(p, qnew, aux);
+    pif_flux_sum(p, qnew, aux, 1.0, 1, Fs, Gs);
+    if ((rc = construct_l_noc(p, qnew, aux, L))) goto out;
+    euler_step(p, 0.5 * dt, qold, L, qstar);
+    orc_set_bnd(p, qstar, aux);
+    pif_flux_sum(p, qstar, aux, 2.0, 0, Fs, Gs);
+    if ((rc = construct_l_noc(p, qstar, aux, L))) goto out;
+    euler_step(p, 0.5 * dt, qold, L, qstar);
+    orc_set_bnd(p, qstar, aux);
+    pif_flux_sum(p, qstar, aux, 2.0, 0, Fs, Gs);
+    if ((rc = construct_l_noc(p, qstar, aux, L))) goto out;
+    euler_step(p, dt, qold, L, qstar);
+    orc_set_bnd(p, qstar, aux);
+    pif_flux_sum(p, qstar, aux, 1.0, 0, Fs, Gs);
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)numel; k++) {
+        Fs[k] = Fs[k] / 6.0;
+        Gs[k] = Gs[k] / 6.0;
+    }
+    orc_set_bnd(p, qnew, aux);
+    if ((rc = construct_l_2d(p, qnew, aux, Fs, Gs, L, smax))) goto out;
+    euler_step(p, dt, qold, L, qnew);
+    orc_set_bnd(p, qnew, aux);
+out:
+    free(qold);
+    return rc;
+}
+
+/* Shared controller of FinSolveRK / FinSolveLxW / FinSolveRK_PIF:
+ * lib/2d/FinSolveRK.cpp:77-367, lib/2d/FinSolveLxW.cpp:100-229,
+ * lib/2d/FinSolveRK_PIF.cpp:122-293, lib/1d/FinSolveRK.cpp:88-363. */
+/* One step of the stepper `ts` (FINESS_TS_*). */
+static int any_step(const finess_params* p, int ts, double* q, double* qstar, double* aux, double* F, double* G,
+                    double* L, double* smax, double dt)
+{
+    if (ts == FINESS_TS_LXW) return lxw_step(p, q, aux, F, G, L, smax, dt);
+    if (ts == FINESS_TS_PIF) return pif_step(p, q, qstar, aux, F, G, L, smax, dt);
+    return rk_step(p, q, qstar, aux, L, smax, dt);
+}
+
+static int check_stepper(const finess_params* p, int ts)
+{
+    if (ts == FINESS_TS_LXW && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
+    if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4)) return FINESS_ERR_CONFIG;
+    return FINESS_OK;
+}
+
+static int fin_solve(const finess_params* p, int ts, double* qnew, double* aux, double* tio,
                      double tend, double* dtv, orc_stats* st)
 {
     int rc = check_config(p);
     if (rc) return rc;
-    if (lxw && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
+    if ((rc = check_stepper(p, ts))) return rc;
+    const int lxw = ts != FINESS_TS_RK;  /* flux arrays needed */
     const size_t numel = state_numel(p, p->meqn);
     double* qold = (double*)malloc(numel * sizeof(double));
     double* qstar = (double*)malloc(numel * sizeof(double));
@@ -1025,7 +1182,7 @@ static int fin_solve(const finess_params* p, int lxw, double* qnew, double* aux,
             if (told + dt > tend) dt = tend - told;
             t = told + dt;
             memset(smax, 0, smax_numel(p) * sizeof(double));
-            rc = lxw ? lxw_step(p, qnew, aux, F, G, L, smax, dt) : rk_step(p, qnew, qstar, aux, L, smax, dt);
+            rc = any_step(p, ts, qnew, qstar, aux, F, G, L, smax, dt);
             if (rc) goto done;
             rc = orc_get_cfl(p, dt, dtv[2], smax, &cfl);
             if (rc) goto done;
@@ -1065,14 +1222,22 @@ int orc_fin_solve_rk(const finess_params* p, double* q, double* aux, double* t, 
 int orc_fin_solve_lxw(const finess_params* p, double* q, double* aux, double* t, double tend,
                       double* dtv, orc_stats* st)
 {
-    return fin_solve(p, 1, q, aux, t, tend, dtv, st);
+    return fin_solve(p, FINESS_TS_LXW, q, aux, t, tend, dtv, st);
+}
+
+int orc_fin_solve_rk_pif(const finess_params* p, double* q, double* aux, double* t, double tend,
+                         double* dtv, orc_stats* st)
+{
+    return fin_solve(p, FINESS_TS_PIF, q, aux, t, tend, dtv, st);
 }
 
 int orc_run_fixed_steps(const finess_params* p, double* q, double* aux, double dt, int nsteps, double* cfl)
 {
     int rc = check_config(p);
     if (rc) return rc;
-    const int lxw = p->time_stepping_method == FINESS_TS_LXW;
+    const int ts = p->time_stepping_method;
+    if ((rc = check_stepper(p, ts))) return rc;
+    const int lxw = ts != FINESS_TS_RK;
     const size_t numel = state_numel(p, p->meqn);
     double* qstar = (double*)malloc(numel * sizeof(double));
     double* L = (double*)calloc(numel, sizeof(double));
@@ -1083,7 +1248,7 @@ int orc_run_fixed_steps(const finess_params* p, double* q, double* aux, double d
     double c = 0.0;
     for (int n = 0; n < nsteps && !rc; n++) {
         memset(smax, 0, smax_numel(p) * sizeof(double));
-        rc = lxw ? lxw_step(p, q, aux, F, G, L, smax, dt) : rk_step(p, q, qstar, aux, L, smax, dt);
+        rc = any_step(p, ts, q, qstar, aux, F, G, L, smax, dt);
         if (!rc) rc = orc_get_cfl(p, dt, 1e300, smax, &c);
     }
     if (cfl) *cfl = c;

@@ -55,6 +55,9 @@ int orc_fin_solve_rk (const finess_params* p, double* q, double* aux, double* t,
                       double tend, double* dtv, orc_stats* st);
 int orc_fin_solve_lxw(const finess_params* p, double* q, double* aux, double* t,
                       double tend, double* dtv, orc_stats* st);
+/* FinSolveRK_PIF (lib/2d/FinSolveRK_PIF.cpp:66-295; 2D, time_order 4). */
+int orc_fin_solve_rk_pif(const finess_params* p, double* q, double* aux, double* t,
+                         double tend, double* dtv, orc_stats* st);
 /* Fixed number of steps of size dt, no controller (for timing and parity at
  * a stated step count).  Returns the CFL of the last step in *cfl. */
 int orc_run_fixed_steps(const finess_params* p, double* q, double* aux, double dt,

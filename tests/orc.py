@@ -57,6 +57,7 @@ def oracle():
             "orc_get_cfl": (ctypes.c_int, [PP, ctypes.c_double, ctypes.c_double, PD, PD]),
             "orc_fin_solve_rk": (ctypes.c_int, [PP, PD, PD, PD, ctypes.c_double, PD, ctypes.POINTER(OrcStats)]),
             "orc_fin_solve_lxw": (ctypes.c_int, [PP, PD, PD, PD, ctypes.c_double, PD, ctypes.POINTER(OrcStats)]),
+            "orc_fin_solve_rk_pif": (ctypes.c_int, [PP, PD, PD, PD, ctypes.c_double, PD, ctypes.POINTER(OrcStats)]),
             "orc_run_fixed_steps": (ctypes.c_int, [PP, PD, PD, ctypes.c_double, ctypes.c_int, PD]),
             "orc_construct_integrated_r": (ctypes.c_int, [PP, ctypes.c_double, PD, PD, PD, PD]),
             "orc_construct_lxw_l": (ctypes.c_int, [PP, PD, PD, PD, PD, PD, PD]),
@@ -122,12 +123,12 @@ def set_bnd(p, q, aux=None):
     return q
 
 
-def fin_solve(p, q, t, tend, dtv, aux=None, lxw=False):
+def fin_solve(p, q, t, tend, dtv, aux=None, lxw=False, pif=False):
     lib = oracle()
     aux = np.ones(p.shape_aux()) if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
     tt = ctypes.c_double(t)
     st = OrcStats()
-    fn = lib.orc_fin_solve_lxw if lxw else lib.orc_fin_solve_rk
+    fn = lib.orc_fin_solve_rk_pif if pif else lib.orc_fin_solve_lxw if lxw else lib.orc_fin_solve_rk
     rc = fn(ctypes.byref(p), ptr(q), ptr(aux), ctypes.byref(tt), float(tend), ptr(dtv), ctypes.byref(st))
     return rc, tt.value, st
 

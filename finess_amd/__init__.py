@@ -130,6 +130,7 @@ SIGNATURES = {
     "finess_construct_l": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, _PD]),
     "finess_fin_solve_rk": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_fin_solve_lxw": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
+    "finess_fin_solve_rk_pif": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_apply_mpp_limiter_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD, _PD]),
     "finess_construct_integrated_r": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD]),
     "finess_advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, ctypes.POINTER(Stats)]),
@@ -299,6 +300,9 @@ class Solver:
 
     def fin_solve_lxw(self, q, t, tend, dtv, aux=None):
         return self._fin_solve(self.lib.finess_fin_solve_lxw, q, t, tend, dtv, aux)
+
+    def fin_solve_rk_pif(self, q, t, tend, dtv, aux=None):
+        return self._fin_solve(self.lib.finess_fin_solve_rk_pif, q, t, tend, dtv, aux)
 
     def _fin_solve(self, fn, q, t, tend, dtv, aux):
         assert q.dtype == np.float64 and q.flags["C_CONTIGUOUS"]

@@ -64,6 +64,7 @@ struct finess_ctx {
     // place; Lax-Friedrichs), per-cell fractions, global wave-speed words
     double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
     unsigned long long* mpp_gmax = nullptr;
+    double *pif_fx = nullptr, *pif_fy = nullptr;  // RK4-PIF interface fluxes of a non-characteristic stage
     StepCtl* ctl = nullptr;                       // device controller block (2D)
     bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
@@ -245,7 +246,9 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.beta = beta;
         a.halt = &c->ctl->halt;
     }
-    const bool lxw = p.time_stepping_method == FINESS_TS_LXW;
+    // Lax-Wendroff and the final RK4-PIF stage read their stencil fluxes from
+    // lxwF / lxwG (time-integrated, resp. time-averaged fluxes)
+    const bool lxw = p.time_stepping_method != FINESS_TS_RK;
     if (lxw) {
         a.lxw_F = c->lxwF;
         a.lxw_G = c->lxwG;
@@ -327,6 +330,48 @@ int mpp_step(finess_ctx* c, int src, int dst, double dt)
     return 0;
 }
 
+// One RK4-PIF step (lib/2d/FinSolveRK_PIF.cpp:151-206) from buf[A]: three
+// non-characteristic stages A -> B -> C -> B with the flux samples summed into
+// lxwF / lxwG, then the characteristic stage on the averaged fluxes into C.
+int pif_step(finess_ctx* c, int A, int B, int C, double dt)
+{
+    const finess_params& p = c->p;
+    PifLaunch L;
+    memset(&L, 0, sizeof L);
+    L.qold = c->buf[A];
+    L.mx = p.mx;
+    L.my = p.my;
+    for (int k = 0; k < 4; k++) L.bc[k] = p.bc[k];
+    L.dx = p.dx;
+    L.dy = p.dy;
+    L.eps = p.epsilon;
+    L.power = p.power_param;
+    L.gamma = p.gamma;
+    L.F = c->lxwF;
+    L.G = c->lxwG;
+    L.fx = c->pif_fx;
+    L.fy = c->pif_fy;
+    auto sum = [&](int src, double w, int first, int last) {
+        L.q = c->buf[src];
+        L.w = w;
+        L.first = first;
+        L.last = last;
+        return launch_pif_flux_sum(L, c->st);
+    };
+    auto noc = [&](int src, int dst, double cdt) {
+        L.q = c->buf[src];
+        L.out = c->buf[dst];
+        L.cdt = cdt;
+        return launch_pif_noc_stage(weno_kind(p), L, c->st);
+    };
+    if (tmark(c)) return FINESS_ERR_HIP;
+    if (sum(A, 1.0, 1, 0) || noc(A, B, 0.5 * dt) || sum(B, 2.0, 0, 0) || noc(B, C, 0.5 * dt) ||
+        sum(C, 2.0, 0, 0) || noc(C, B, dt) || sum(B, 1.0, 0, 1))
+        return fail(c, FINESS_ERR_HIP, "PIF stage launch failed");
+    if (tmark(c)) return FINESS_ERR_HIP;
+    return stage(c, A, A, -1, C, 1.0, 0.0, 1.0, dt, 0);
+}
+
 // Enqueues one full time step from buf[cur]; returns the index holding the
 // result in *res and the slot holding the CFL speeds in *slot.
 int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
@@ -338,6 +383,12 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
     if (c->rec_age >= 0 && c->route_fresh == false) c->rec_age++;  // the previous step used the record
     c->route_fresh = true;
+    if (p.time_stepping_method == FINESS_TS_PIF) {
+        if ((rc = pif_step(c, A, B, C, dt))) return rc;
+        *res = C;
+        *slot = 0;
+        return 0;
+    }
     if (p.time_stepping_method == FINESS_TS_LXW) {
         if ((rc = lxw_fluxes(c, A, dt))) return rc;
         if (p.mpp_limiter) {
@@ -582,7 +633,11 @@ int finess_create(const finess_params* pin, finess_ctx** out)
                 rc = FINESS_ERR_HIP;
         }
     }
-    if (!rc && p.time_stepping_method == FINESS_TS_LXW) {
+    if (!rc && p.time_stepping_method == FINESS_TS_PIF) {
+        rc = alloc(c, (void**)&c->pif_fx, (size_t)(p.mx + 1) * p.my * c->M * sizeof(double));
+        if (!rc) rc = alloc(c, (void**)&c->pif_fy, (size_t)p.mx * (p.my + 1) * c->M * sizeof(double));
+    }
+    if (!rc && p.time_stepping_method != FINESS_TS_RK) {
         const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6) * sizeof(double);
         rc = alloc(c, (void**)&c->lxwF, lb);
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
@@ -642,6 +697,8 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->mpp_gl);
     hipFree(c->mpp_amin);
     hipFree(c->mpp_gmax);
+    hipFree(c->pif_fx);
+    hipFree(c->pif_fy);
     hipFree(c->route_flux);
     if (c->smax_h) hipHostFree(c->smax_h);
     for (hipEvent_t e : c->tev) hipEventDestroy(e);
@@ -1096,6 +1153,17 @@ int finess_fin_solve_rk(finess_ctx* c, double* q, const double* aux, double* t, 
 {
     if (!c || !q || !t || !dtv) return FINESS_ERR_ARG;
     if (c->p.time_stepping_method != FINESS_TS_RK) return fail(c, FINESS_ERR_CONFIG, "context is not RK");
+    int rc = finess_set_state(c, q, aux, *t);
+    if (rc) return rc;
+    rc = finess_advance(c, tend, dtv, nullptr);
+    const int rc2 = finess_get_state(c, q, t);
+    return rc ? rc : rc2;
+}
+
+int finess_fin_solve_rk_pif(finess_ctx* c, double* q, const double* aux, double* t, double tend, double* dtv)
+{
+    if (!c || !q || !t || !dtv) return FINESS_ERR_ARG;
+    if (c->p.time_stepping_method != FINESS_TS_PIF) return fail(c, FINESS_ERR_CONFIG, "context is not RK4-PIF");
     int rc = finess_set_state(c, q, aux, *t);
     if (rc) return rc;
     rc = finess_advance(c, tend, dtv, nullptr);

@@ -48,6 +48,22 @@ int launch_mpp(const MppLaunch& L, hipStream_t st);
 // the limiter alone: fh, gh limited in place from fl, gl (q, dt given)
 int launch_mpp_limit(const MppLaunch& L, hipStream_t st);
 
+// RK4-PIF stages, 2D Euler (pif2d.hip): the weighted flux sums on the
+// extended grid, and one non-characteristic stage q -> out = qold + cdt L_NOC.
+struct PifLaunch {
+    const double* q;
+    const double* qold;
+    double* out;
+    int mx, my;
+    int bc[4];
+    double dx, dy, cdt, w;
+    int first, last;
+    double eps, power, gamma;
+    double *F, *G, *fx, *fy;
+};
+int launch_pif_flux_sum(const PifLaunch& L, hipStream_t st);
+int launch_pif_noc_stage(int wk, const PifLaunch& L, hipStream_t st);
+
 // Device-side time-step controller (step_control.hip).
 struct StepCtl;
 struct StepCtlParams;

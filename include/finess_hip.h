@@ -157,6 +157,11 @@ int finess_fin_solve_rk(finess_ctx* ctx, double* q, const double* aux,
 int finess_fin_solve_lxw(finess_ctx* ctx, double* q, const double* aux,
                          double* t, double tend, double* dtv);
 
+/* FinSolveRK_PIF (lib/2d/FinSolveRK_PIF.cpp:66-295; the reference links it as
+ * FinSolveUser, :31-35), same conventions; a FINESS_TS_PIF context. */
+int finess_fin_solve_rk_pif(finess_ctx* ctx, double* q, const double* aux,
+                            double* t, double tend, double* dtv);
+
 /* ConstructIntegratedR (lib/2d/ConstructIntegratedR.cpp:59-296): time-
  * integrated fluxes F, G (dTensorBC3(mx,my,meqn,mbc)) on cells [1-3, m+3]. */
 int finess_construct_integrated_r(finess_ctx* ctx, double dt, const double* q,

@@ -24,6 +24,9 @@ void ConstructL(StateVars& Q, dTensorBC3& Lstar, dTensorBC3& smax);
 // lib/2d/RunFinpack.h:18-21
 void FinSolveRK(StateVars& Qnew, double tend, double dtv[]);
 void FinSolveLxW(StateVars& Qnew, double tend, double dtv[]);
+// lib/2d/FinSolveRK_PIF.cpp:31-35, 66 (RK4-PIF; the reference links it as FinSolveUser)
+void FinSolveRK_PIF(StateVars& Qnew, double tend, double dtv[]);
+void FinSolveUser(StateVars& Qnew, double tend, double dtv[]);
 // app callbacks (lib/2d/ConstructL.h:7-34), consulted by the probes if linked
 void FluxFunc(const dTensor2& xpts, const dTensor2& Q, const dTensor2& Aux, dTensor3& flux);
 void ProjectLeftEig(int ixy, const dTensor1& Aux_ave, const dTensor1& Q_ave, const dTensor2& Qvals, dTensor2& Wvals);

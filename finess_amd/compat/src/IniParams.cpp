@@ -209,7 +209,13 @@ void IniParams::to_finess_params(finess_params* p) const
     p->maux = maux;
     p->space_order = space_order;
     p->time_order = time_order;
-    p->time_stepping_method = time_stepping_method == TimeSteppingMethod::LxW ? FINESS_TS_LXW : FINESS_TS_RK;
+    // User-Defined is the stepper the app links as FinSolveUser; the
+    // reference's apps link FinSolveRK_PIF (time_order 4), as this library's
+    // FinSolveUser does
+    p->time_stepping_method = time_stepping_method == TimeSteppingMethod::LxW ? FINESS_TS_LXW
+                              : (time_stepping_method == TimeSteppingMethod::USER_DEFINED && time_order == 4)
+                                  ? FINESS_TS_PIF
+                                  : FINESS_TS_RK;
     p->initial_dt = initial_dt;
     p->max_dt = max_dt;
     p->desired_cfl = desired_cfl;

@@ -333,13 +333,14 @@ void scatter(const std::vector<double>& v, T& t)
     for (int k = 0; k < t.numel(); k++) t.vset(k, v[k]);
 }
 
-void fin_solve(StateVars& Qnew, double tend, double dtv[], bool lxw)
+void fin_solve(StateVars& Qnew, double tend, double dtv[], int ts)
 {
     finess_ctx* c = context();
     std::vector<double> q = gather(Qnew.const_ref_q()), aux = gather(Qnew.const_ref_aux());
     double t = Qnew.get_t();
-    const int rc = lxw ? finess_fin_solve_lxw(c, q.data(), aux.data(), &t, tend, dtv)
-                       : finess_fin_solve_rk(c, q.data(), aux.data(), &t, tend, dtv);
+    const int rc = ts == FINESS_TS_LXW   ? finess_fin_solve_lxw(c, q.data(), aux.data(), &t, tend, dtv)
+                   : ts == FINESS_TS_PIF ? finess_fin_solve_rk_pif(c, q.data(), aux.data(), &t, tend, dtv)
+                                         : finess_fin_solve_rk(c, q.data(), aux.data(), &t, tend, dtv);
     if (rc) fail_like_reference(rc, finess_last_error(c));
     scatter(q, Qnew.ref_q());
     Qnew.set_t(t);
@@ -361,8 +362,11 @@ void ConstructL(StateVars& Q, dTensorBC3& Lstar, dTensorBC3& smax)
     scatter(sm, smax);
 }
 
-void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, false); }
-void FinSolveLxW(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, true); }
+void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, FINESS_TS_RK); }
+void FinSolveLxW(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, FINESS_TS_LXW); }
+// lib/2d/FinSolveRK_PIF.cpp:66, and FinSolveUser wrapping it as that file does (:31-35)
+void FinSolveRK_PIF(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, FINESS_TS_PIF); }
+void FinSolveUser(StateVars& Qnew, double tend, double dtv[]) { FinSolveRK_PIF(Qnew, tend, dtv); }
 #else
 void ConstructL(const StateVars& Q, dTensorBC2& Lstar, dTensorBC1& smax)
 {
@@ -375,5 +379,5 @@ void ConstructL(const StateVars& Q, dTensorBC2& Lstar, dTensorBC1& smax)
     scatter(sm, smax);
 }
 
-void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, false); }
+void FinSolveRK(StateVars& Qnew, double tend, double dtv[]) { fin_solve(Qnew, tend, dtv, FINESS_TS_RK); }
 #endif

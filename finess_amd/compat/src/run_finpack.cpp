@@ -116,6 +116,8 @@ int finess_run(int argc, char* argv[])
 #else
             if (ini.get_time_stepping_method() == IniParams::TimeSteppingMethod::LxW)
                 FinSolveLxW(Q, tend, dtv);
+            else if (ini.get_time_stepping_method() == IniParams::TimeSteppingMethod::USER_DEFINED)
+                FinSolveUser(Q, tend, dtv);  // lib/2d/RunFinpack.cpp:117-120
             else
                 FinSolveRK(Q, tend, dtv);
 #endif

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import orc
-from finess_amd import problems
+from finess_amd import TS_PIF, problems
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,11 +34,11 @@ def read_frame(path, shape2d=None, meqn=1):
     return t, np.transpose(q.reshape(meqn, my, mx), (2, 1, 0))
 
 
-def oracle_frames(p, q, aux, nout, tfinal, lxw=False):
+def oracle_frames(p, q, aux, nout, tfinal, lxw=False, pif=False):
     dtv = np.array([0.0, p.initial_dt, p.max_dt])
     t = 0.0
     for n in range(1, nout + 1):
-        rc, t, _ = orc.fin_solve(p, q, t, tfinal * n / nout, dtv, aux=aux, lxw=lxw)
+        rc, t, _ = orc.fin_solve(p, q, t, tfinal * n / nout, dtv, aux=aux, lxw=lxw, pif=pif)
         assert rc == 0
     return t, q
 
@@ -61,6 +61,28 @@ def test_advection_app(tmp_path):
     t1, q1 = read_frame(tmp_path / "output" / "q0001.dat", None, 1)
     assert abs(t1 - t0) < 1e-15
     err = orc.rel_linf(q1, orc.interior(p, q0))
+    assert np.all(err <= 1e-10), err
+
+
+def test_euler_quadrants_app_pif(tmp_path):
+    """The app with `time_stepping_method = User-Defined`, `time_order = 4`:
+    RunFinpack calls FinSolveUser, which this library links as RK4-PIF, as
+    lib/2d/FinSolveRK_PIF.cpp does for the reference's apps."""
+    ini = open(os.path.join(APPS, "euler_quadrants", "parameters.ini")).read()
+    ini = ini.replace("time_stepping_method = Runge-Kutta", "time_stepping_method = User-Defined")
+    ini = ini.replace("time_order  = 3", "time_order  = 4")
+    (tmp_path / "parameters.ini").write_text(ini)
+    r = subprocess.run([os.path.join(APPS, "euler_quadrants", "finess.exe")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    p, q, aux = problems.riemann4(mx=64, initial_dt=1e-3, max_dt=1.0, nv=500000, time_order=4,
+                                  time_stepping_method=TS_PIF)
+    t0, q0 = oracle_frames(p, q, aux, 2, 0.02, pif=True)
+    t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
+    assert abs(t1 - t0) < 1e-15
+    err = orc.rel_linf(q1, orc.interior(p, q0))
+    print("PIF app vs oracle", err)
     assert np.all(err <= 1e-10), err
 
 

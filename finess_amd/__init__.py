@@ -130,6 +130,8 @@ SIGNATURES = {
     "finess_construct_l": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, _PD]),
     "finess_fin_solve_rk": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_fin_solve_lxw": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
+    "finess_set_step_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "finess_read_step_log": (ctypes.c_int, [ctypes.c_void_p, _PD, ctypes.c_int]),
     "finess_fin_solve_rk_pif": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_apply_mpp_limiter_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD, _PD]),
     "finess_construct_integrated_r": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD]),
@@ -283,6 +285,18 @@ class Solver:
         self._check(self.lib.finess_construct_integrated_r(self.h, float(dt), _ptr(q), None, _ptr(F), _ptr(G)),
                     "construct_integrated_r")
         return F, G
+
+    def set_step_log(self, on=True):
+        self._check(self.lib.finess_set_step_log(self.h, 1 if on else 0), "set_step_log")
+
+    def read_step_log(self, max_records=1 << 20):
+        """The conservation records of the accepted steps since the last
+        read: array (n, meqn + 1) of (t, sum dx dy q per field)."""
+        buf = np.zeros((max_records, self.p.meqn + 1))
+        n = self.lib.finess_read_step_log(self.h, _ptr(buf), max_records)
+        if n < 0:
+            self._check(n, "read_step_log")
+        return buf[:n].copy()
 
     def apply_mpp_limiter_2d(self, dt, q, fLF, gLF, fHat, gHat):
         """ApplyMPPLimiter2D: limits fHat, gHat (reference layouts with

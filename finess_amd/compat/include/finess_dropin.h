@@ -51,6 +51,12 @@ struct ProbeReport {
 ProbeReport probe_app();
 // Releases the device context held by the operators.
 void release();
+// ConSoln (lib/2d/ConSoln.cpp:13-95, lib/1d/ConSoln.cpp): the sums of the
+// conserved fields of Q computed on the host in ConSoln's order, and the
+// line it writes to <output_dir>/conservation.dat (the file restarts at t = 0).
+// The operators append the lines of every accepted step themselves.
+void con_soln_sums(const StateVars& Q, double* sums);
+void con_soln(const StateVars& Q);
 }
 
 #endif

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -301,6 +303,12 @@ finess_ctx* context()
     g_ctx = nullptr;
     rc = finess_create(&p, &g_ctx);
     if (rc) fail_like_reference(rc, "finess_create failed");
+    // ConSoln after every accepted step, reduced on the device (without a
+    // capacity function; with one, con_soln_frame writes a line per call)
+    if (global_ini_params.get_mcapa() < 1) {
+        rc = finess_set_step_log(g_ctx, 1);
+        if (rc) fail_like_reference(rc, finess_last_error(g_ctx));
+    }
     if (p.routing == FINESS_ROUTE_LEARNED) {
         // [gpu] indicator: the trained network, FINESS_MLP_NPARAM float32
         // parameters followed by the float32 logit threshold
@@ -333,6 +341,71 @@ void scatter(const std::vector<double>& v, T& t)
     for (int k = 0; k < t.numel(); k++) t.vset(k, v[k]);
 }
 
+} // namespace
+// ConSoln's sums on the host in its order (lib/2d/ConSoln.cpp:44-73,
+// lib/1d/ConSoln.cpp:36-60): field by field, i then j, dx*dy*q (times the
+// capacity aux(mcapa) when mcapa >= 1).
+void con_soln_sums(const StateVars& Q, double* s)
+{
+    const IniParams& ini = global_ini_params;
+    const int meqn = ini.get_meqn(), mx = ini.get_mx(), mcapa = ini.get_mcapa();
+    const double dx = ini.get_dx();
+#if NDIMS == 1
+    for (int m = 1; m <= meqn; m++) {
+        double acc = 0.0;
+        for (int i = 1; i <= mx; i++) {
+            const double qtmp = Q.const_ref_q().get(i, m);
+            acc = mcapa < 1 ? acc + dx * qtmp : acc + Q.const_ref_aux().get(i, mcapa) * dx * qtmp;
+        }
+        s[m - 1] = acc;
+    }
+#else
+    const int my = ini.get_my();
+    const double dy = ini.get_dy();
+    for (int m = 1; m <= meqn; m++) {
+        double acc = 0.0;
+        for (int i = 1; i <= mx; i++)
+            for (int j = 1; j <= my; j++) {
+                const double qtmp = Q.const_ref_q().get(i, j, m);
+                acc = mcapa < 1 ? acc + dx * dy * qtmp : acc + Q.const_ref_aux().get(i, j, mcapa) * dx * dy * qtmp;
+            }
+        s[m - 1] = acc;
+    }
+#endif
+}
+namespace {
+
+// One line of conservation.dat in ConSoln's format (lib/2d/ConSoln.cpp:
+// 82-94): t and the sums, %24.16e, tiny sums printed as zero.
+void con_soln_line(std::ofstream& f, double t, const double* s, int meqn)
+{
+    f << std::setprecision(16);
+    f << std::setw(24) << std::scientific << t << " ";
+    for (int m = 0; m < meqn; m++) f << std::setw(24) << std::scientific << (std::fabs(s[m]) < 1.0e-99 ? 0.0 : s[m]) << " ";
+    f << std::endl;
+}
+
+// Appends the steps' conservation records of the last FinSolve call; with a
+// capacity function, one line for the state at its end, summed on the host
+// as ConSoln does.
+void con_soln_steps(finess_ctx* c, const StateVars& Q)
+{
+    const IniParams& ini = global_ini_params;
+    const int meqn = ini.get_meqn();
+    std::ofstream f((ini.get_output_dir() + "/conservation.dat").c_str(), std::ofstream::app);
+    if (ini.get_mcapa() < 1) {
+        std::vector<double> rec((size_t)(meqn + 1) * 256);
+        int n;
+        while ((n = finess_read_step_log(c, rec.data(), 256)) > 0)
+            for (int k = 0; k < n; k++) con_soln_line(f, rec[(size_t)k * (meqn + 1)], &rec[(size_t)k * (meqn + 1) + 1], meqn);
+        if (n < 0) fail_like_reference(n, finess_last_error(c));
+        return;
+    }
+    std::vector<double> s((size_t)meqn);
+    con_soln_sums(Q, s.data());
+    con_soln_line(f, Q.get_t(), s.data(), meqn);
+}
+
 void fin_solve(StateVars& Qnew, double tend, double dtv[], int ts)
 {
     finess_ctx* c = context();
@@ -344,8 +417,20 @@ void fin_solve(StateVars& Qnew, double tend, double dtv[], int ts)
     if (rc) fail_like_reference(rc, finess_last_error(c));
     scatter(q, Qnew.ref_q());
     Qnew.set_t(t);
+    con_soln_steps(c, Qnew);
 }
 } // namespace
+
+void con_soln(const StateVars& Q)
+{
+    const IniParams& ini = global_ini_params;
+    const int meqn = ini.get_meqn();
+    std::vector<double> s((size_t)meqn);
+    con_soln_sums(Q, s.data());
+    std::ofstream f((ini.get_output_dir() + "/conservation.dat").c_str(),
+                    Q.get_t() == 0 ? std::ofstream::out : std::ofstream::app);
+    con_soln_line(f, Q.get_t(), s.data(), meqn);
+}
 } // namespace finess_dropin
 
 using namespace finess_dropin;

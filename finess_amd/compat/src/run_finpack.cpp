@@ -29,9 +29,50 @@ void AuxFunc(const dTensor2& xpts, dTensor2& auxvals);
 
 namespace {
 
+// WriteQhelp (lib/2d/WriteQhelp.cpp:8-35, lib/1d/WriteQhelp.cpp)
+void write_qhelp()
+{
+    const IniParams& ini = global_ini_params;
+    FILE* f = std::fopen((ini.get_output_dir() + "/qhelp.dat").c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "%16d   : ndims   \n", ini.get_ndims());
+    std::fprintf(f, "%16d   : meqn    \n", ini.get_meqn());
+    std::fprintf(f, "%16d   : maux    \n", ini.get_maux());
+    std::fprintf(f, "%16d   : nout    \n", ini.get_nout());
+    std::fprintf(f, "%16d   : mx      \n", ini.get_mx());
+#if NDIMS == 2
+    std::fprintf(f, "%16d   : my      \n", ini.get_my());
+#endif
+    std::fprintf(f, "%16.8e : xlow    \n", ini.get_xlow());
+    std::fprintf(f, "%16.8e : xhigh   \n", ini.get_xhigh());
+#if NDIMS == 2
+    std::fprintf(f, "%16.8e : ylow    \n", ini.get_ylow());
+    std::fprintf(f, "%16.8e : yhigh   \n", ini.get_yhigh());
+#endif
+    std::fclose(f);
+}
+
+// Output (lib/2d/Output.cpp:13-62): q%04d.dat and a%04d.dat
 void write_frame(const StateVars& Q, int n)
 {
     const IniParams& ini = global_ini_params;
+    {
+        std::ostringstream fa;
+        fa << ini.get_output_dir() << "/a" << std::setfill('0') << std::setw(4) << n << ".dat";
+        std::ofstream a(fa.str().c_str());
+        a << std::setprecision(16);
+        a << std::setw(24) << std::scientific << Q.get_t() << "\n";
+        const int maux = ini.get_maux(), mxa = ini.get_mx();
+#if NDIMS == 1
+        for (int m = 1; m <= maux; m++)
+            for (int i = 1; i <= mxa; i++) a << std::setw(24) << std::scientific << Q.const_ref_aux().get(i, m) << "\n";
+#else
+        for (int m = 1; m <= maux; m++)
+            for (int j = 1; j <= ini.get_my(); j++)
+                for (int i = 1; i <= mxa; i++)
+                    a << std::setw(24) << std::scientific << Q.const_ref_aux().get(i, j, m) << "\n";
+#endif
+    }
     std::ostringstream fn;
     fn << ini.get_output_dir() << "/q" << std::setfill('0') << std::setw(4) << n << ".dat";
     std::ofstream f(fn.str().c_str());
@@ -103,7 +144,9 @@ int finess_run(int argc, char* argv[])
         StateVars Q(0.0, ini.get_mx(), ini.get_my(), ini.get_meqn(), ini.get_maux(), ini.get_mbc());
 #endif
         sample(Q);
+        write_qhelp();
         write_frame(Q, 0);
+        finess_dropin::con_soln(Q);  // lib/2d/RunFinpack.cpp:86-89
         double dtv[3] = {0.0, ini.get_initial_dt(), ini.get_max_dt()};
         const double dtout = ini.get_tfinal() / double(ini.get_nout());
         double tend = 0.0;

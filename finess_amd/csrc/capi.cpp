@@ -65,6 +65,11 @@ struct finess_ctx {
     double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
     unsigned long long* mpp_gmax = nullptr;
     double *pif_fx = nullptr, *pif_fy = nullptr;  // RK4-PIF interface fluxes of a non-characteristic stage
+    // per-step conservation log (ConSoln after every accepted step)
+    bool steplog = false;
+    double* slog = nullptr;      // device records (t, sums) of one chunk
+    double* spart = nullptr;     // partial sums
+    std::vector<double> hlog;    // records not yet read by the caller
     StepCtl* ctl = nullptr;                       // device controller block (2D)
     bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
@@ -698,6 +703,8 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->mpp_amin);
     hipFree(c->mpp_gmax);
     hipFree(c->pif_fx);
+    hipFree(c->slog);
+    hipFree(c->spart);
     hipFree(c->pif_fy);
     hipFree(c->route_flux);
     if (c->smax_h) hipHostFree(c->smax_h);
@@ -709,6 +716,31 @@ void finess_destroy(finess_ctx* c)
 }
 
 const char* finess_last_error(const finess_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int finess_set_step_log(finess_ctx* c, int on)
+{
+    if (!c) return FINESS_ERR_ARG;
+    if (on && !c->slog) {
+        int rc;
+        if ((rc = alloc(c, (void**)&c->slog, (size_t)64 * (c->M + 1) * sizeof(double)))) return rc;
+        if ((rc = alloc(c, (void**)&c->spart, (size_t)c->M * 240 * sizeof(double)))) return rc;
+    }
+    c->steplog = on != 0;
+    if (!on) c->hlog.clear();
+    return FINESS_OK;
+}
+
+int finess_read_step_log(finess_ctx* c, double* rec, int max_records)
+{
+    if (!c || (max_records > 0 && !rec) || max_records < 0) return FINESS_ERR_ARG;
+    const size_t w = (size_t)c->M + 1, avail = c->hlog.size() / w;
+    const size_t n = avail < (size_t)max_records ? avail : (size_t)max_records;
+    if (n) {
+        memcpy(rec, c->hlog.data(), n * w * sizeof(double));
+        c->hlog.erase(c->hlog.begin(), c->hlog.begin() + (long)(n * w));
+    }
+    return (int)n;
+}
 
 int finess_apply_mpp_limiter_2d(finess_ctx* c, double dt, const double* q, const double* fLF, const double* gLF,
                                 double* fHat, double* gHat)
@@ -946,6 +978,26 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     return FINESS_OK;
 }
 
+// Weight of a cell in ConSoln's sums: dx dy (2D), dx (1D).
+static double step_log_weight(const finess_params& p)
+{
+    return p.ndims == 1 ? p.dx : p.dx * p.dy;
+}
+
+// Appends the conservation record of the held state at time t (host
+// controller: after an accepted step).
+static int log_step(finess_ctx* c, double t)
+{
+    const finess_params& p = c->p;
+    if (launch_con_soln(c->buf[c->cur], c->plane, c->M, step_log_weight(p), c->spart, nullptr, 0, t, c->slog, c->st))
+        return fail(c, FINESS_ERR_HIP, "conservation sum launch failed");
+    const size_t at = c->hlog.size();
+    c->hlog.resize(at + c->M + 1);
+    HIPCHK(c, hipMemcpyAsync(c->hlog.data() + at, c->slog, (c->M + 1) * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
 static const char* ctl_error_message(long long code)
 {
     switch (code) {
@@ -1001,6 +1053,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
     };
     std::vector<Snap> snap;
     long long seen = 0;   // attempts processed before the current chunk
+    long long logged = 0; // conservation records read back
     long long kmax = 64;  // chunk length: 64, or shorter after a rejection
     while (run) {
         long long K;
@@ -1021,6 +1074,12 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
                 rc = fail(c, FINESS_ERR_HIP, "controller launch failed");
                 break;
             }
+            // the attempt's conservation record, written only if the controller just accepted it
+            if (c->steplog && launch_con_soln(c->buf[res], c->plane, c->M, step_log_weight(p), c->spart, c->ctl,
+                                              h.nrec, 0.0, c->slog, c->st)) {
+                rc = fail(c, FINESS_ERR_HIP, "conservation sum launch failed");
+                break;
+            }
             sn.out = res;
             sn.rec_age = c->rec_age;
             sn.rec_par = c->rec_par;
@@ -1036,6 +1095,15 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         }
         const long long m = h.done - seen - 1;  // the chunk's last processed attempt
         seen = h.done;
+        if (c->steplog && h.nrec > logged) {
+            const size_t n = (size_t)(h.nrec - logged) * (c->M + 1), at = c->hlog.size();
+            c->hlog.resize(at + n);
+            if (hipMemcpy(c->hlog.data() + at, c->slog, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = fail(c, FINESS_ERR_HIP, "conservation log read failed");
+                break;
+            }
+            logged = h.nrec;
+        }
         if (h.halt == HALT_NONE) {
             if (kmax < 64) kmax *= 2;
             continue;
@@ -1115,6 +1183,7 @@ static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed
                 m_accept = 1;
                 c->cur = res;
                 acc++;
+                if (c->steplog && (rc = log_step(c, t))) break;
             } else {
                 t = told;
                 rej++;

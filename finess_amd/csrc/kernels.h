@@ -32,6 +32,12 @@ int mlp_zero(float* p, long long n, hipStream_t st);
 
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
 
+// Conservation sums of a state (step_control.hip): record (t, sum of w*q per
+// field) into log; with ctl, only for an accepted attempt not yet recorded.
+struct StepCtl;
+int launch_con_soln(const double* q, long long plane, int M, double w, double* partial, StepCtl* ctl,
+                    long long base, double t, double* log, hipStream_t st);
+
 // Positivity-preserving limiter of the 2D Lax-Wendroff step, Euler
 // (mpp2d.hip): from the high-order interface fluxes fh, gh (x faces
 // [m][(mx+1) my], y faces [m][mx (my+1)]) to the limited update q -> out.

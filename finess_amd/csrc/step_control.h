@@ -19,6 +19,7 @@ struct StepCtl {
     long long done;     // attempts the controller has processed
     unsigned long long halt;
     long long err;      // FINESS_ERR_* when halt == HALT_ERROR
+    long long nrec;     // accepted steps whose conservation record is written
 };
 
 struct StepCtlParams {

@@ -167,6 +167,16 @@ int finess_fin_solve_rk_pif(finess_ctx* ctx, double* q, const double* aux,
 int finess_construct_integrated_r(finess_ctx* ctx, double dt, const double* q,
                                   const double* aux, double* F, double* G);
 
+/* ConSoln after every accepted step (lib/2d/ConSoln.cpp:13-95, called from
+ * lib/2d/FinSolveRK.cpp:361-362 and lib/2d/FinSolveLxW.cpp:228-229): with the
+ * log on, each accepted step of the context's steppers records (t, then for
+ * every field the sum over the interior of dx*dy*q (2D) or dx*q (1D)),
+ * reduced on the device.  finess_read_step_log copies up to max_records
+ * records (meqn+1 doubles each, oldest first) into rec, removes them from the
+ * log and returns how many it copied. */
+int finess_set_step_log(finess_ctx* ctx, int on);
+int finess_read_step_log(finess_ctx* ctx, double* rec, int max_records);
+
 /* ApplyMPPLimiter2D for 2D Euler (apps/2d/euler/lib/ApplyMPPLimiter2D.cpp:27-286;
  * an overridable object of the reference, lib/2d/Makefile.defs:109): limits the
  * high-order interface fluxes fHat (dTensorBC3(mx+1,my,meqn,mbc)) and gHat

@@ -46,12 +46,33 @@ def oracle_frames(p, q, aux, nout, tfinal, lxw=False, pif=False):
 def test_euler_quadrants_app(tmp_path):
     run_app("euler_quadrants", tmp_path)
     p, q, aux = problems.riemann4(mx=64, initial_dt=1e-3, max_dt=1.0, nv=500000)
-    t0, q0 = oracle_frames(p, q, aux, 2, 0.02)
+    dtv = np.array([0.0, p.initial_dt, p.max_dt])
+    q0 = q.copy()
+    steps = 0
+    for n in (1, 2):
+        rc, t0, st = orc.fin_solve(p, q0, 0.01 * (n - 1), 0.01 * n, dtv, aux=aux)
+        assert rc == 0
+        steps += st.steps_accepted
     t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
     assert abs(t1 - t0) < 1e-15
     err = orc.rel_linf(q1, orc.interior(p, q0))
     print("app vs oracle", err)
     assert np.all(err <= 1e-10)
+    # the other outputs: qhelp.dat, aux frames, conservation.dat with the t = 0
+    # line and one line per accepted step (ConSoln's format)
+    out = tmp_path / "output"
+    qhelp = (out / "qhelp.dat").read_text().split("\n")
+    assert qhelp[0].split()[0] == "2" and qhelp[4].split()[0] == "64" and qhelp[5].split()[0] == "64"
+    assert (out / "a0000.dat").exists() and (out / "a0002.dat").exists()
+    cons = np.loadtxt(out / "conservation.dat")
+    assert cons.shape == (1 + steps, 6)
+    g = p.mbc
+    sums0 = (q[g:g + 64, g:g + 64] * (p.dx * p.dy)).sum(axis=(0, 1))
+    assert cons[0, 0] == 0.0 and np.allclose(cons[0, 1:], sums0, rtol=1e-15, atol=1e-300)
+    assert abs(cons[-1, 0] - 0.02) < 1e-15
+    sums1 = (q1 * (p.dx * p.dy)).sum(axis=(0, 1))
+    print("conservation.dat last line", cons[-1], "frame sums", sums1)
+    assert np.allclose(cons[-1, 1:], sums1, rtol=1e-12, atol=1e-14)
 
 
 def test_advection_app(tmp_path):

@@ -41,7 +41,9 @@ struct finess_ctx {
     hipStream_t st = nullptr;
     int M = 0;
     long long plane = 0;        // cells of one field plane
-    double* buf[3] = {nullptr, nullptr, nullptr};
+    // plane sets: 0-2 the rotating SSP-RK / LxW states; 3, 4 the extra
+    // registers of the fourth- and fifth-order schemes (q2; L)
+    double* buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double* aux = nullptr;      // 1D advection speed (interior cells)
     double* lxwF = nullptr;
     double* lxwG = nullptr;
@@ -377,6 +379,37 @@ int pif_step(finess_ctx* c, int A, int B, int C, double dt)
     return stage(c, A, A, -1, C, 1.0, 0.0, 1.0, dt, 0);
 }
 
+// Coefficients of the 8-stage fifth-order scheme as published in
+// lib/2d/SetRKinfo.cpp:84-127 (delta, gamma rows 1-3, beta).
+const double RK5_DELTA[8] = {1.0e0, 1.528486658778845e00, 4.720094096662784e-02, 8.801244253465348e-01,
+                             1.019066090228480e+00, 1.049772291176110e+01, -4.254616508506826e+00, 0.0};
+const double RK5_GAMMA[3][8] = {
+    {0.0, -1.552288007713033e+01, 4.127286635722417e-01, -1.011819196331377e+00, -2.765748383780848e-01,
+     5.075770311217778e-02, 6.999810478513669e+00, -1.114908881433104e+01},
+    {1.0, 6.534691420958578e+00, 2.280056542904473e-01, 1.308684311397668e+00, 4.769419552531064e-01,
+     -6.368809762042849e-03, 9.339446057238532e-02, 9.556626047962331e-01},
+    {0.0, 0.0, 0.0, -2.510747784045939e+00, -8.576822794622042e-01, 1.044599944472272e+00, -7.000810861049136e+00,
+     1.906311811144179e+00}};
+const double RK5_BETA[8] = {8.653258038183180e-02, 9.544677980851571e-01, 2.651941386774408e-01,
+                            2.736914413910379e-01, 5.999778649323600e-01, 4.433177471748104e-03,
+                            5.309971130968292e-03, 5.830861806762871e-01};
+
+// L(buf[src]) into Lbuf, with the wave-speed maxima of the step (slot 0).
+int stage_l(finess_ctx* c, int src, double* Lbuf)
+{
+    const finess_params& p = c->p;
+    StageArgs a = base_stage_args(c);
+    a.src = c->buf[src];
+    a.qa = c->buf[src];
+    a.a_is_src = 1;
+    a.smax = c->smax;
+    a.Lout = Lbuf;
+    if (tmark(c)) return FINESS_ERR_HIP;
+    const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 0, a, c->st);
+    if (tmark(c)) return FINESS_ERR_HIP;
+    return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
+}
+
 // Enqueues one full time step from buf[cur]; returns the index holding the
 // result in *res and the slot holding the CFL speeds in *slot.
 int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
@@ -425,6 +458,39 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         *res = B;
         *slot = one_d ? 2 : 0;
         return 0;
+    case 4: {
+        // SSP(10,4) (lib/2d/FinSolveRK.cpp:213-291): stages ping-pong between
+        // B and C, q2 lives in buffer 3
+        int x = A, y = B;
+        for (int s = 1; s <= 9; s++) {
+            if ((rc = stage(c, x, x, -1, y, 1.0, 0.0, 1.0 / 6.0, dt, 0))) return rc;
+            x = y;
+            y = (y == B) ? C : B;
+            if (s == 5 && launch_rk4_combine(c->buf[A], c->buf[3], c->buf[x], (long long)c->M * c->plane, c->st))
+                return fail(c, FINESS_ERR_HIP, "RK4 combination launch failed");
+        }
+        if ((rc = stage(c, x, 3, x, y, 1.0, 3.0 / 5.0, 1.0 / 10.0, dt, 0))) return rc;
+        *res = y;
+        *slot = 0;
+        return 0;
+    }
+    case 5: {
+        // 8-stage fifth-order scheme (lib/2d/FinSolveRK.cpp:292-305): q2 in
+        // buffer 3 (zero at the start of the step), L in buffer 4
+        const long long n = (long long)c->M * c->plane;
+        int x = A, y = B;
+        for (int s = 0; s < 8; s++) {
+            if ((rc = stage_l(c, x, c->buf[4]))) return rc;
+            if (launch_rk5_update(c->buf[x], c->buf[3], c->buf[A], c->buf[4], c->buf[y], n, RK5_GAMMA[0][s],
+                                  RK5_GAMMA[1][s], RK5_GAMMA[2][s], RK5_DELTA[s], RK5_BETA[s] * dt, s == 0, c->st))
+                return fail(c, FINESS_ERR_HIP, "RK5 update launch failed");
+            x = y;
+            y = (y == B) ? C : B;
+        }
+        *res = x;
+        *slot = 0;
+        return 0;
+    }
     default:
         return fail(c, FINESS_ERR_CONFIG, "time_order not supported");
     }
@@ -577,9 +643,11 @@ int finess_params_finalize(finess_params* p)
         // RK4-PIF exists for time_order 4 only (lib/2d/FinSolveRK_PIF.cpp:151-206)
         if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || p->time_order != 4) return FINESS_ERR_CONFIG;
         if (p->precision != FINESS_PREC_FP64) return FINESS_ERR_CONFIG;
-    } else if (p->time_order < 1 || p->time_order > 3) {
+    } else if (p->time_order < 1 || p->time_order > (p->ndims == 2 ? 5 : 3)) {
         return FINESS_ERR_CONFIG;
     }
+    // the routed stages follow the three-stage schedule
+    if (p->routing != FINESS_ROUTE_NONE && p->time_order > 3) return FINESS_ERR_CONFIG;
     if (p->precision == FINESS_PREC_FP32 && weno_kind(*p) != WENO_JS5_P2) return FINESS_ERR_CONFIG;
     if (p->routing != FINESS_ROUTE_NONE &&
         (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || weno_kind(*p) != WENO_JS5_P2 ||
@@ -624,6 +692,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
     }
     const size_t sbytes = (size_t)c->M * c->plane * sizeof(double);
     for (int k = 0; k < 3 && !rc; k++) rc = alloc(c, (void**)&c->buf[k], sbytes);
+    if (p.time_order >= 4 && p.time_stepping_method == FINESS_TS_RK)
+        for (int k = 3; k < 5 && !rc; k++) rc = alloc(c, (void**)&c->buf[k], sbytes);
     if (!rc) rc = alloc(c, (void**)&c->smax, 2 * CFL_SPREAD * NSLOT * sizeof(unsigned long long));
     if (!rc && hipHostMalloc((void**)&c->smax_h, 2 * CFL_SPREAD * NSLOT * sizeof(unsigned long long), hipHostMallocDefault) !=
                    hipSuccess)
@@ -680,7 +750,7 @@ void finess_destroy(finess_ctx* c)
 {
     if (!c) return;
     if (c->st) hipStreamSynchronize(c->st);
-    for (int k = 0; k < 3; k++) hipFree(c->buf[k]);
+    for (int k = 0; k < 5; k++) hipFree(c->buf[k]);
     hipFree(c->aux);
     hipFree(c->lxwF);
     hipFree(c->lxwG);
@@ -1150,7 +1220,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
 static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
 {
     if (c && dtv && c->have_state && c->ctl && c->p.ndims == 2 && c->p.time_stepping_method == FINESS_TS_RK &&
-        !c->p.verbosity && !c->p.host_controller)
+        c->p.time_order <= 3 && !c->p.verbosity && !c->p.host_controller)
         return advance_dev(c, tend, dtv, fixed_steps, stats);
     if (!c || !dtv) return FINESS_ERR_ARG;
     if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");

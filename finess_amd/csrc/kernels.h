@@ -68,6 +68,12 @@ struct PifLaunch {
     double *F, *G, *fx, *fy;
 };
 int launch_pif_flux_sum(const PifLaunch& L, hipStream_t st);
+
+// Elementwise parts of the 10-stage fourth-order and 8-stage fifth-order
+// Runge-Kutta schemes (rk_low_storage.hip); n = meqn * cells.
+int launch_rk4_combine(const double* qn, double* q2, double* q1, long long n, hipStream_t st);
+int launch_rk5_update(const double* q1, double* q2, const double* qold, const double* L, double* out, long long n,
+                      double g1, double g2, double g3, double delta, double bdt, int first, hipStream_t st);
 int launch_pif_noc_stage(int wk, const PifLaunch& L, hipStream_t st);
 
 // Device-side time-step controller (step_control.hip).

@@ -703,8 +703,77 @@ static const double RK_A1[3][3] = {{1.0, 0, 0}, {1.0, 0.5, 0}, {1.0, 0.75, 2.0e0
 static const double RK_A2[3][3] = {{0.0, 0, 0}, {0.0, 0.5, 0}, {0.0, 0.25, 1.0e0 / 3.0e0}};
 static const double RK_B[3][3] = {{1.0, 0, 0}, {1.0, 0.5, 0}, {1.0, 0.25, 2.0e0 / 3.0e0}};
 
+/* The 10-stage fourth-order and the 8-stage fifth-order schemes,
+ * lib/2d/SetRKinfo.cpp:66-129 (coefficients as published there). */
+static const double RK5_DELTA[8] = {1.0e0, 1.528486658778845e00, 4.720094096662784e-02, 8.801244253465348e-01,
+                                    1.019066090228480e+00, 1.049772291176110e+01, -4.254616508506826e+00, 0.0};
+static const double RK5_GAMMA[3][8] = {
+    {0.0, -1.552288007713033e+01, 4.127286635722417e-01, -1.011819196331377e+00, -2.765748383780848e-01,
+     5.075770311217778e-02, 6.999810478513669e+00, -1.114908881433104e+01},
+    {1.0, 6.534691420958578e+00, 2.280056542904473e-01, 1.308684311397668e+00, 4.769419552531064e-01,
+     -6.368809762042849e-03, 9.339446057238532e-02, 9.556626047962331e-01},
+    {0.0, 0.0, 0.0, -2.510747784045939e+00, -8.576822794622042e-01, 1.044599944472272e+00, -7.000810861049136e+00,
+     1.906311811144179e+00}};
+static const double RK5_BETA[8] = {8.653258038183180e-02, 9.544677980851571e-01, 2.651941386774408e-01,
+                                   2.736914413910379e-01, 5.999778649323600e-01, 4.433177471748104e-03,
+                                   5.309971130968292e-03, 5.830861806762871e-01};
+
+/* lib/2d/FinSolveRK.cpp:213-291: SSP(10,4).  q1, q2 start as q^n; stages 1-5
+ * and 6-9 are q1 = q1 + dt/6 L(q1), between them q2 = (q2 + 9 q1)/25 and
+ * q1 = 15 q2 - 5 q1, and the last stage is q1 = q2 + 3/5 q1 + dt/10 L(q1). */
+static int rk4_step(const finess_params* p, double* qnew, double* aux, double* L, double* smax, double dt,
+                    double* q2)
+{
+    const size_t numel = state_numel(p, p->meqn);
+    int rc;
+    memcpy(q2, qnew, numel * sizeof(double));
+    for (int s = 1; s <= 9; s++) {
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+        update_soln(numel, 1.0, 0.0, 1.0 / 6.0, dt, qnew, L, qnew);
+        if (s == 5) {
+#pragma omp parallel for schedule(static)
+            for (long long k = 0; k < (long long)numel; k++) {
+                const double tmp = (q2[k] + 9.0 * qnew[k]) / 25.0;
+                q2[k] = tmp;
+                qnew[k] = 15.0 * tmp - 5.0 * qnew[k];
+            }
+        }
+    }
+    orc_set_bnd(p, qnew, aux);
+    if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+    update_soln(numel, 1.0, 3.0 / 5.0, 1.0 / 10.0, dt, q2, L, qnew);
+    return FINESS_OK;
+}
+
+/* lib/2d/FinSolveRK.cpp:292-305 with the two-register update of
+ * lib/2d/UpdateSoln.cpp:40-80: q2 = 0; for s = 1..8, with q1 = qnew:
+ * q2 = q2 + delta_s q1, q1 = g1 q1 + g2 q2 + g3 q^n + beta_s dt L(q1). */
+static int rk5_step(const finess_params* p, double* qnew, double* aux, double* L, double* smax, double dt,
+                    double* q2, const double* qold)
+{
+    const size_t numel = state_numel(p, p->meqn);
+    int rc;
+    memset(q2, 0, numel * sizeof(double));
+    for (int s = 0; s < 8; s++) {
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = orc_construct_l(p, qnew, aux, L, smax))) return rc;
+        const double g1 = RK5_GAMMA[0][s], g2 = RK5_GAMMA[1][s], g3 = RK5_GAMMA[2][s], delta = RK5_DELTA[s];
+        const double beta = RK5_BETA[s];
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < (long long)numel; k++) {
+            const double s1 = qnew[k];
+            const double s2 = q2[k] + delta * s1;
+            q2[k] = s2;
+            const double s3 = qold[k];
+            qnew[k] = g1 * s1 + g2 * s2 + g3 * s3 + beta * dt * L[k];
+        }
+    }
+    return FINESS_OK;
+}
+
 /* One full RK step (all stages) from qnew into qnew, lib/2d/FinSolveRK.cpp:
- * 115-211 (orders 1-3). qstar, L, smax are scratch. */
+ * 115-305 (orders 1-5). qstar, L, smax are scratch. */
 static int rk_step(const finess_params* p, double* qnew, double* qstar, double* aux, double* L,
                    double* smax, double dt)
 {
@@ -732,6 +801,15 @@ static int rk_step(const finess_params* p, double* qnew, double* qstar, double* 
         orc_set_bnd(p, qstar, aux);
         if ((rc = orc_construct_l(p, qstar, aux, L, smax))) return rc;
         update_soln(numel, RK_A1[k][2], RK_A2[k][2], RK_B[k][2], dt, qstar, L, qnew);
+    } else if (to == 4 || to == 5) {
+        double* q2 = (double*)malloc(numel * sizeof(double));
+        double* q0 = (double*)malloc(numel * sizeof(double));
+        if (!q2 || !q0) { free(q2); free(q0); return FINESS_ERR_ARG; }
+        memcpy(q0, qnew, numel * sizeof(double));
+        rc = to == 4 ? rk4_step(p, qnew, aux, L, smax, dt, q2) : rk5_step(p, qnew, aux, L, smax, dt, q2, q0);
+        free(q2);
+        free(q0);
+        return rc;
     } else {
         return FINESS_ERR_CONFIG;
     }

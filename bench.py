@@ -245,6 +245,7 @@ def copy_calibration(mib=256, reps=5):
 
 
 def main():
+    global ROUTE_INTERVAL
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -256,6 +257,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shallow-water workload")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Lax-Wendroff Kelvin-Helmholtz and the shock-bubble workloads")
+    ap.add_argument("--route-interval", type=int, default=ROUTE_INTERVAL,
+                    help="steps between indicator evaluations of the learned form")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -270,6 +273,7 @@ def main():
         sys.exit(2)
     from finess_amd.indicator import train_indicator
 
+    ROUTE_INTERVAL = args.route_interval
     mx = args.mx
     cells = mx * mx
     ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride)

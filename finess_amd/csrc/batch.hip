@@ -46,6 +46,8 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         for (int k = 0; k < 4; k++) a.bc[k] = b.bc[k];
         a.dx = pr->dx;
         a.dy = pr->dy;
+        a.idx = 1.0 / pr->dx;
+        a.idy = 1.0 / pr->dy;
         a.alpha_scaling = b.alpha_scaling;
         a.eps = b.eps;
         a.power = b.power;

@@ -142,6 +142,8 @@ StageArgs base_stage_args(finess_ctx* c)
     for (int k = 0; k < 4; k++) a.bc[k] = p.bc[k];
     a.dx = p.dx;
     a.dy = p.dy;
+    a.idx = 1.0 / p.dx;
+    a.idy = 1.0 / p.dy;
     a.alpha_scaling = p.alpha_scaling;
     a.eps = p.epsilon;
     a.power = p.power_param;

@@ -114,7 +114,7 @@ __host__ __device__ __forceinline__ void face_features<Euler2D, float>(const flo
         }
         mean[k] *= (1.0f / 6.0f);
     }
-    const float cref = sqrtf(fabsf(float(pp.gamma) * mean[3] / mean[0]));
+    const float cref = sqrt_lp(fabsf(div_lp(float(pp.gamma) * mean[3], mean[0])));
     const float ref[4] = {fabsf(mean[0]), cref, cref, fabsf(mean[3])};
     for (int k = 0; k < 4; k++) {
         const float inv = frcp(hi[k] - lo[k] + 0.01f * ref[k] + 1e-30f);

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "fastmath.h"
+
 namespace finess {
 
 template <typename R>
@@ -45,16 +47,21 @@ struct Euler2D {
     // cell sound speed SetWaveSpd uses for the left and right states
     // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Always formed in FP64 (the
     // pressure recovery keeps FP64 in the FP32 form as well).
+    // LP: the record is rounded to FP32 afterwards, so 1/rho comes from the
+    // refined reciprocal estimate (within an FP64 ulp) and c from the FP32
+    // square root of the FP64 c^2.
+    template <bool LP = false>
     __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[8])
     {
         const double gamma = pp.gamma;
         const double rho = q[0];
-        const double irho = 1.0 / rho;
+        const double irho = LP ? rcp_nr(rho) : 1.0 / rho;
         const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
         const double press = (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
         v[5] = irho; v[6] = press;
-        v[7] = sqrt(fabs(gamma * press * irho));
+        const double c2 = fabs(gamma * press * irho);
+        v[7] = LP ? double(sqrt_lp(float(c2))) : sqrt(c2);
     }
 
     // LDS variable index of rotated component k (dir 0: x, 1: y)
@@ -102,18 +109,18 @@ struct Euler2D {
     {
         const R gamma = R(pp.gamma);
         const R rho = qa[0];
-        const R irho = R(1) / rho;
+        const R irho = div_lp(R(1), rho);
         e.u1 = qa[1] * irho; e.u2 = qa[2] * irho; e.u3 = qa[3] * irho;
         const R energy = qa[4];
         const R umag2 = e.u1 * e.u1 + e.u2 * e.u2 + e.u3 * e.u3;
         const R press = (gamma - R(1)) * (energy - R(0.5) * rho * umag2);
         const R c2 = gamma * press * irho;
-        e.cR = sqrt(fabs(c2));                                   // ProjectRightEig.cpp:42
+        e.cR = sqrt_lp(fabs(c2));                                // ProjectRightEig.cpp:42
         const R c = c2 >= R(0) ? e.cR : R(__builtin_nan(""));   // ProjectLeftEig.cpp:43
         e.H = (energy + press) * irho;
         e.hu = umag2 * R(0.5);
         const R den = R(2) * e.H - umag2;
-        const R r = R(1) / (c * den);
+        const R r = div_lp(R(1), c * den);
         const R inv2 = R(2) * c * r;                             // 2 / den
         e.a1 = (e.hu - e.H - e.u1 * c) * r;
         e.b1 = (e.hu * (c - e.u1) + e.H * e.u1) * r;
@@ -160,7 +167,7 @@ struct Euler2D {
         const R rho = R(0.5) * (ql[0] + qr[0]);
         const R un = R(0.5) * (unl + unr);
         const R press = R(0.5) * (cl[6] + cr[6]);
-        const R c = sqrt(fabs(gamma * press / rho));
+        const R c = sqrt_lp(fabs(div_lp(gamma * press, rho)));
         const R s1 = rmin(unl - cl[7], un - c);
         const R s2 = rmax(unr + cr[7], un + c);
         return rmax(fabs(s1), fabs(s2));
@@ -177,9 +184,10 @@ struct ShallowWater2D {
     static constexpr int NORMAL_X = 1;
     static constexpr int NORMAL_Y = 2;
 
+    template <bool LP = false>
     __host__ __device__ __forceinline__ static void cell_record(const double q[3], const PhysParams&, double v[5])
     {
-        const double ih = 1.0 / q[0];
+        const double ih = LP ? rcp_nr(q[0]) : 1.0 / q[0];
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
         v[3] = q[1] * ih; v[4] = q[2] * ih;
     }
@@ -200,7 +208,7 @@ struct ShallowWater2D {
     {
         (void)c;
         const R h = q[0];
-        const R un = q[1] / h, ut = q[2] / h;
+        const R un = div_lp(q[1], h), ut = div_lp(q[2], h);
         const R hun = h * un;
         f[0] = hun;
         f[1] = hun * un + R(0.5) * h * h;
@@ -216,13 +224,13 @@ struct ShallowWater2D {
     __host__ __device__ __forceinline__ static void eig(const R qa[3], const PhysParams&, Eig<R>& e)
     {
         const R h = qa[0];
-        e.u1 = qa[1] / h;
-        e.u2 = qa[2] / h;
-        e.sh = sqrt(h);
-        const R i2s = R(1) / (R(2) * e.sh);
+        e.u1 = div_lp(qa[1], h);
+        e.u2 = div_lp(qa[2], h);
+        e.sh = sqrt_lp(h);
+        const R i2s = div_lp(R(1), R(2) * e.sh);
         e.a1 = (e.sh + e.u1) * i2s;
         e.a3 = (e.sh - e.u1) * i2s;
-        e.b = R(0.5) / e.sh;
+        e.b = div_lp(R(0.5), e.sh);
     }
 
     template <typename R>
@@ -247,16 +255,16 @@ struct ShallowWater2D {
     {
         (void)cl; (void)cr;
         const R hl = ql[0], hr = qr[0];
-        const R unl = ql[1] / hl, utl = ql[2] / hl;
+        const R unl = div_lp(ql[1], hl), utl = div_lp(ql[2], hl);
         // apps/2d/shallow_water/lib/SetWaveSpd.cpp:27-28 divides the right
         // momenta by h_l; kept when sw_compat_hl is set.
         const R hden = pp.sw_compat_hl ? hl : hr;
-        const R unr = qr[1] / hden, utr = qr[2] / hden;
+        const R unr = div_lp(qr[1], hden), utr = div_lp(qr[2], hden);
         const R h = R(0.5) * (hl + hr);
-        const R shl = sqrt(hl), shr = sqrt(hr);
-        const R idn = R(1) / (shl + shr);
+        const R shl = sqrt_lp(hl), shr = sqrt_lp(hr);
+        const R idn = div_lp(R(1), shl + shr);
         const R un = (shl * unl + shr * unr) * idn;
-        const R sh = sqrt(h);
+        const R sh = sqrt_lp(h);
         (void)utl; (void)utr;
         const R s1 = rmin(un - sh, unl - shl);
         const R s2 = rmax(un + sh, unr + shr);

@@ -82,6 +82,7 @@ struct StageArgs {
     double beta;
     const unsigned long long* halt;
     double dx, dy;
+    double idx, idy;  // 1/dx, 1/dy (IEEE, formed on the host)
     double alpha_scaling;
     double eps, power;
     PhysParams pp;
@@ -191,7 +192,7 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
         for (int k = 0; k < K; k++) {
             if (!ok[k]) continue;
             double rec[P::NV];
-            P::cell_record(q[k], a.pp, rec);
+            P::template cell_record<sizeof(R) == 4>(q[k], a.pp, rec);
             for (int v = 0; v < P::NV; v++) sm.cv[v][c0 + k * nthreads] = R(rec[v]);
         }
     }
@@ -394,7 +395,7 @@ __host__ __device__ __forceinline__ void phase_lx(int tid, const StageArgs& a, c
 {
     const int ci = tid / TJ, cj = tid - ci * TJ;
     if (ci >= TI) return;
-    const double idx_ = 1.0 / a.dx;
+    const double idx_ = a.idx;
     for (int m = 0; m < P::MEQN; m++)
         Lx[m] = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
 }
@@ -413,7 +414,7 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
     const long long idx = (long long)gi * a.my + gj;
     const int c = (ci + 3) * S::HJ + (cj + 3);
     const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
-    const double idy_ = 1.0 / a.dy;
+    const double idy_ = a.idy;
     const double bdt = stage_bdt(a);
     for (int m = 0; m < M; m++) {
         const double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
             const long long idx = (long long)gi * a.my + gj;
             const int c = (ci + 3) * S::HJ + (cj + 3);
             const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
-            const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+            const double idx_ = a.idx, idy_ = a.idy;
             const double bdt = stage_bdt(a);
             for (int m = 0; m < M; m++) {
                 const double Lx = -(double(sm.s.face[m][(ci + 1) * TJ + cj]) - double(sm.s.face[m][ci * TJ + cj])) * idx_;
@@ -383,7 +383,7 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
         for (int s = 0; s < 6; s++) {
             double q[M], rec[NV];
             for (int m = 0; m < M; m++) q[m] = (a.src + m * a.plane)[c0 + s * stride];
-            P::cell_record(q, a.pp, rec);
+            P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
             for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
         }
     } else {
@@ -391,7 +391,7 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
             double q[M], rec[NV];
             if (DIR == 0) load_state<P>(a, gi - 3 + s, gj, q);
             else load_state<P>(a, gi, gj - 3 + s, q);
-            P::cell_record(q, a.pp, rec);
+            P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
             for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
         }
     }
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
             const long long idx = (long long)gi * a.my + gj;
             const int c = (ci + 3) * S::HJ + (cj + 3);
             const bool exact_lds = sizeof(R) == sizeof(double);
-            const double idx_ = 1.0 / a.dx, idy_ = 1.0 / a.dy;
+            const double idx_ = a.idx, idy_ = a.idy;
             const double bdt = stage_bdt(a);
             for (int m = 0; m < M; m++) {
                 const double Lx = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;

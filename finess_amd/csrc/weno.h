@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "fastmath.h"
+
 namespace finess {
 
 enum WenoKind : int { WENO_JS5_P2 = 0, WENO_JS5_POW = 1, WENO_Z5 = 2, WENO_FD5 = 3 };
@@ -53,28 +55,6 @@ __host__ __device__ __forceinline__ R fmin3(R a, R b, R c)
     return fmin(fmin(a, b), c);
 }
 
-__host__ __device__ __forceinline__ float frcp(float x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rcpf(x);
-#else
-    return 1.0f / x;
-#endif
-}
-
-// 1/x to full FP64 accuracy (within an ulp) without the IEEE division
-// sequence: hardware reciprocal estimate refined by two Newton steps.
-__host__ __device__ __forceinline__ double rcp_nr(double x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(fma(-x, r, 1.0), r, r);
-    return fma(fma(-x, r, 1.0), r, r);
-#else
-    return 1.0 / x;
-#endif
-}
-__host__ __device__ __forceinline__ float rcp_nr(float x) { return frcp(x); }
 
 // Unnormalised JS5 weights for power_param == 2 (see the header comment).
 __host__ __device__ __forceinline__ void js5_p2_terms(double b0, double b1, double b2, double eps,

@@ -107,7 +107,8 @@ int emu_construct_l(const finess_params* p, const double* q, double* Lstar, doub
     memset(&a, 0, sizeof a);
     a.src = qg.data(); a.src_ghosts = g; a.mx = p->mx; a.my = p->my; a.plane = plane;
     for (int k = 0; k < 4; k++) a.bc[k] = p->bc[k];
-    a.dx = p->dx; a.dy = p->dy; a.alpha_scaling = p->alpha_scaling; a.eps = p->epsilon; a.power = p->power_param;
+    a.dx = p->dx; a.dy = p->dy; a.idx = 1.0 / p->dx; a.idy = 1.0 / p->dy;
+        a.alpha_scaling = p->alpha_scaling; a.eps = p->epsilon; a.power = p->power_param;
     a.pp.gamma = p->gamma; a.pp.sw_compat_hl = p->sw_compat_hl;
     a.Lout = Lo.data(); a.smax_faces_x = ax.data(); a.smax_faces_y = ay.data();
     const int wk = wk_of(p);
@@ -192,7 +193,8 @@ int emu_run_fixed_steps(const finess_params* p, double* q, double dt, int nsteps
         a.out = buf[out].data(); a.a1 = a1; a.a2 = a2; a.bdt = beta * dt;
         a.mx = p->mx; a.my = p->my; a.plane = plane;
         for (int k = 0; k < 4; k++) a.bc[k] = p->bc[k];
-        a.dx = p->dx; a.dy = p->dy; a.alpha_scaling = p->alpha_scaling; a.eps = p->epsilon; a.power = p->power_param;
+        a.dx = p->dx; a.dy = p->dy; a.idx = 1.0 / p->dx; a.idy = 1.0 / p->dy;
+        a.alpha_scaling = p->alpha_scaling; a.eps = p->epsilon; a.power = p->power_param;
         a.pp.gamma = p->gamma; a.pp.sw_compat_hl = p->sw_compat_hl;
         if (lxw) { a.lxw_F = Fo.data(); a.lxw_G = Go.data(); }
         if (p->physics == FINESS_PHYS_EULER) {

@@ -110,6 +110,12 @@ int fail(finess_ctx* c, int code, const char* msg)
 
 int weno_kind(const finess_params& p)
 {
+    if (p.space_order == 1) return WENO_O1;
+    if (p.space_order == 7)
+        return p.weno_version == FINESS_WENO_FD ? WENO_FD7 : p.weno_version == FINESS_WENO_Z ? WENO_Z7 : WENO_JS7;
+    if (p.space_order == 9)
+        return p.weno_version == FINESS_WENO_FD ? WENO_FD9 : p.weno_version == FINESS_WENO_Z ? WENO_Z9 : WENO_JS9;
+    if (p.space_order == 11) return WENO_JS11;
     if (p.weno_version == FINESS_WENO_FD) return WENO_FD5;
     if (p.weno_version == FINESS_WENO_Z) return WENO_Z5;
     return p.power_param == 2.0 ? WENO_JS5_P2 : WENO_JS5_POW;
@@ -617,10 +623,23 @@ int finess_params_finalize(finess_params* p)
     if (!(p->nv > 0) || !(p->initial_dt > 0.0) || !(p->max_dt >= p->initial_dt)) return FINESS_ERR_CONFIG;
     if (!(p->desired_cfl > 0.0) || !(p->max_cfl >= p->desired_cfl)) return FINESS_ERR_CONFIG;
     if (!(p->alpha_scaling >= 1.0) || !(p->epsilon > 0.0)) return FINESS_ERR_CONFIG;
-    // what the device path implements (GetWenoReconstruct throws for the rest)
-    if (p->space_order != 5) return FINESS_ERR_CONFIG;
+    // what the device path implements (GetWenoReconstruct throws for the rest,
+    // lib/WenoReconstruct.cpp:39-73)
     if (p->weno_version != FINESS_WENO_JS && p->weno_version != FINESS_WENO_FD && p->weno_version != FINESS_WENO_Z)
         return FINESS_ERR_CONFIG;
+    const int so = p->space_order;
+    const bool have = so == 1 || so == 5 || (p->weno_version == FINESS_WENO_JS && (so == 7 || so == 9 || so == 11)) ||
+                      (p->weno_version != FINESS_WENO_JS && (so == 7 || so == 9));
+    if (!have) return FINESS_ERR_CONFIG;
+    // orders other than 5 run the face-list kernels: 2D Runge-Kutta and
+    // ConstructL in FP64, mbc >= r (lib/2d/ConstructL.cpp:31-33), and every
+    // stencil within the boundary map's range
+    if (so != 5) {
+        const int r = (so + 1) / 2;
+        if (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->routing != FINESS_ROUTE_NONE ||
+            p->precision != FINESS_PREC_FP64 || p->mbc < r || p->mx < r || p->my < r)
+            return FINESS_ERR_CONFIG;
+    }
     if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
     if (p->ndims == 1) {
         if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;

@@ -22,7 +22,12 @@
 
 namespace finess {
 
-enum WenoKind : int { WENO_JS5_P2 = 0, WENO_JS5_POW = 1, WENO_Z5 = 2, WENO_FD5 = 3 };
+enum WenoKind : int {
+    WENO_JS5_P2 = 0, WENO_JS5_POW = 1, WENO_Z5 = 2, WENO_FD5 = 3,
+    // the face-list path of stage2d_wide.hip
+    WENO_JS7 = 4, WENO_JS9 = 5, WENO_JS11 = 6, WENO_Z7 = 7, WENO_Z9 = 8, WENO_FD7 = 9, WENO_FD9 = 10,
+    WENO_O1 = 11
+};
 
 template <typename R>
 struct WenoParams {

@@ -144,6 +144,172 @@ double orc_weno5(int version, double eps, double pw, const double* g)
     }
 }
 
+/* Wider reconstructions: Jiang-Shu orders 7, 9, 11 (lib/WenoReconstruct.cpp:
+ * 150-197, 199-279, 281-375), WENO-Z 7 and 9 (:440-495, :497-572), linear
+ * weights 7 and 9 (:618-656, :658-700) and the first-order value
+ * (WenoReconstructLLF, :702-713).
+ *
+ * Table form, r = (order + 1) / 2, stencil g[0..2r-2] with g[r-1] = u_i:
+ * candidate k uses g[k..k+r-1];
+ *   u_k    = sum_j CAND[k][j] g[k+j]               (value at x_{i+1/2})
+ *   beta_k = sum_{a<=b} B_k[a][b] g[k+a] g[k+b]    (smoothness indicator)
+ *   D[k]   linear weights.
+ * The indicators carry the reference's scaling: 240 (r = 4), 5040 (r = 5)
+ * and 120960 (r = 6) times sum_l int_cell dx^(2l-1) (p_k^(l))^2.  Only the
+ * upper triangles of stencils k <= (r-1)/2 are listed (row by row); stencil
+ * r-1-k is the mirror image, B_{r-1-k}[a][b] = B_k[r-1-b][r-1-a]. */
+static const double CAND4[4][4] = {{-1. / 4., 13. / 12., -23. / 12., 25. / 12.},
+                                   {1. / 12., -5. / 12., 13. / 12., 1. / 4.},
+                                   {-1. / 12., 7. / 12., 7. / 12., -1. / 12.},
+                                   {1. / 4., 13. / 12., -5. / 12., 1. / 12.}};
+static const double D4[4] = {1. / 35., 12. / 35., 18. / 35., 4. / 35.};
+static const double BH4[2][10] = {
+    {547, -3882, 4642, -1854, 7043, -17246, 7042, 11003, -9402, 2107},
+    {267, -1642, 1602, -494, 2843, -5966, 1922, 3443, -2522, 547}};
+
+static const double CAND5[5][5] = {{1. / 5., -21. / 20., 137. / 60., -163. / 60., 137. / 60.},
+                                   {-1. / 20., 17. / 60., -43. / 60., 77. / 60., 1. / 5.},
+                                   {1. / 30., -13. / 60., 47. / 60., 9. / 20., -1. / 20.},
+                                   {-1. / 20., 9. / 20., 47. / 60., -13. / 60., 1. / 30.},
+                                   {1. / 5., 77. / 60., -43. / 60., 17. / 60., -1. / 20.}};
+static const double D5[5] = {1. / 126., 10. / 63., 10. / 21., 20. / 63., 5. / 126.};
+static const double BH5[3][15] = {
+    {22658, -208501, 364863, -288007, 86329, 482963, -1704396, 1358458, -411487, 1521393, -2462076, 758823,
+     1020563, -649501, 107918},
+    {6908, -60871, 99213, -70237, 18079, 138563, -464976, 337018, -88297, 406293, -611976, 165153, 242723,
+     -140251, 22658},
+    {6908, -51001, 67923, -38947, 8209, 104963, -299076, 179098, -38947, 231153, -299076, 67923, 104963, -51001,
+     6908}};
+
+static const double CAND6[6][6] = {{-1. / 6., 31. / 30., -163. / 60., 79. / 20., -71. / 20., 49. / 20.},
+                                   {1. / 30., -13. / 60., 37. / 60., -21. / 20., 29. / 20., 1. / 6.},
+                                   {-1. / 60., 7. / 60., -23. / 60., 19. / 20., 11. / 30., -1. / 30.},
+                                   {1. / 60., -2. / 15., 37. / 60., 37. / 60., -2. / 15., 1. / 60.},
+                                   {-1. / 30., 11. / 30., 19. / 20., -23. / 60., 7. / 60., -1. / 60.},
+                                   {1. / 6., 29. / 20., -21. / 20., 37. / 60., -13. / 60., 1. / 30.}};
+static const double D6[6] = {1. / 462., 5. / 77., 25. / 77., 100. / 231., 25. / 154., 1. / 77.};
+static const double BH6[3][21] = {
+    {1152561, -12950184, 29442256, -33918804, 19834350, -4712740, 36480687, -166461044, 192596472, -113206788,
+     27060170, 190757572, -444003904, 262901672, -63394124, 260445372, -311771244, 76206736, 94851237, -47460464,
+     6150211},
+    {271779, -3015728, 6694608, -7408908, 4067018, -880548, 8449957, -37913324, 42405032, -23510468, 5134574,
+     43093692, -97838784, 55053752, -12183636, 56662212, -65224244, 14742480, 19365967, -9117992, 1152561},
+    {139633, -1429976, 2863984, -2792660, 1325006, -245620, 3824847, -15880404, 15929912, -7727988, 1458762,
+     17195652, -35817664, 17905032, -3462252, 19510972, -20427884, 4086352, 5653317, -2380800, 271779}};
+
+/* The tables of order 7, 9 or 11: cand[r][r], beta[r][r][r] (full upper
+ * triangles, mirrored stencils filled in, lower triangles zero), d[r].
+ * Returns r, or 0 for an order without a table. */
+int orc_weno_tables(int order, double* cand, double* beta, double* d)
+{
+    const int r = (order + 1) / 2;
+    const double *C, *D, *BH;
+    int nh;
+    switch (order) {
+    case 7: C = &CAND4[0][0]; D = D4; BH = &BH4[0][0]; nh = 2; break;
+    case 9: C = &CAND5[0][0]; D = D5; BH = &BH5[0][0]; nh = 3; break;
+    case 11: C = &CAND6[0][0]; D = D6; BH = &BH6[0][0]; nh = 3; break;
+    default: return 0;
+    }
+    const int tri = r * (r + 1) / 2;
+    for (int k = 0; k < r; k++) {
+        d[k] = D[k];
+        for (int j = 0; j < r; j++) cand[k * r + j] = C[k * r + j];
+        for (int a = 0; a < r; a++)
+            for (int b = 0; b < r; b++) beta[(k * r + a) * r + b] = 0.0;
+    }
+    for (int k = 0; k < nh; k++) {
+        int t = 0;
+        for (int a = 0; a < r; a++)
+            for (int b = a; b < r; b++, t++) {
+                const double v = BH[k * tri + t];
+                beta[(k * r + a) * r + b] = v;
+                /* mirror: stencil r-1-k, entry (r-1-b, r-1-a) */
+                beta[((r - 1 - k) * r + (r - 1 - b)) * r + (r - 1 - a)] = v;
+            }
+    }
+    return r;
+}
+
+typedef struct wide_tab {
+    int r;
+    double cand[6][6], beta[6][6][6], d[6];
+} wide_tab;
+
+static const wide_tab* wide_table(int order)
+{
+    static wide_tab tabs[3];
+    static int ready = 0;
+#pragma omp critical(orc_wide_tables)
+    if (!ready) {
+        for (int t = 0; t < 3; t++) {
+            const int o = 7 + 2 * t, r = (o + 1) / 2;
+            double c[36], b[216], dd[6];
+            tabs[t].r = orc_weno_tables(o, c, b, dd);
+            for (int k = 0; k < r; k++) {
+                tabs[t].d[k] = dd[k];
+                for (int j = 0; j < r; j++) tabs[t].cand[k][j] = c[k * r + j];
+                for (int a = 0; a < r; a++)
+                    for (int bb = 0; bb < r; bb++) tabs[t].beta[k][a][bb] = b[(k * r + a) * r + bb];
+            }
+        }
+        ready = 1;
+    }
+    return (order == 7 || order == 9 || order == 11) ? &tabs[(order - 7) / 2] : NULL;
+}
+
+/* WENO of order 7, 9 or 11: JS weights d_k (eps + beta_k)^-p; Z weights
+ * d_k (1 + (tau / (eps + beta_k))^p) with tau = |beta_0 + 3 (beta_1 - beta_2)
+ * - beta_3| (order 7, :470) or |beta_4 - beta_0| (order 9, :549; the
+ * reference's std::abs of a double); FD the linear weights. */
+static double weno_wide(int version, int order, double eps, double pw, const double* g)
+{
+    const wide_tab* T = wide_table(order);
+    const int r = T->r;
+    double u[6], beta[6];
+    for (int k = 0; k < r; k++) {
+        double s = 0.0, b = 0.0;
+        for (int j = 0; j < r; j++) s += T->cand[k][j] * g[k + j];
+        for (int a = 0; a < r; a++)
+            for (int c = a; c < r; c++) b += T->beta[k][a][c] * g[k + a] * g[k + c];
+        u[k] = s;
+        beta[k] = b;
+    }
+    double num = 0.0, den = 0.0;
+    if (version == FINESS_WENO_FD) {
+        for (int k = 0; k < r; k++) num += T->d[k] * u[k];
+        return num;
+    }
+    double tau = 0.0;
+    if (version == FINESS_WENO_Z)
+        tau = order == 7 ? fabs(beta[0] + 3. * (beta[1] - beta[2]) - beta[3]) : fabs(beta[4] - beta[0]);
+    for (int k = 0; k < r; k++) {
+        const double omt = version == FINESS_WENO_Z ? T->d[k] * (1. + pow(tau / (eps + beta[k]), pw))
+                                                    : T->d[k] * pow(eps + beta[k], -pw);
+        num += omt * u[k];
+        den += omt;
+    }
+    return num / den;
+}
+
+/* Reconstruction of any implemented order (GetWenoReconstruct,
+ * lib/WenoReconstruct.cpp:39-73): g[0..order-1]. */
+double orc_weno(int version, int order, double eps, double pw, const double* g)
+{
+    if (order == 1) return g[0];
+    if (order == 5) return orc_weno5(version, eps, pw, g);
+    return weno_wide(version, order, eps, pw, g);
+}
+
+/* Whether GetWenoReconstruct has the (version, order) pair. */
+int orc_weno_supported(int version, int order)
+{
+    if (order == 1 || order == 5) return 1;
+    if (version == FINESS_WENO_JS) return order == 7 || order == 9 || order == 11;
+    if (version == FINESS_WENO_Z || version == FINESS_WENO_FD) return order == 7 || order == 9;
+    return 0;
+}
+
 /* ====================================================================== */
 /* physics callbacks                                                        */
 /* ====================================================================== */
@@ -486,50 +652,62 @@ void orc_set_bnd(const finess_params* p, double* q, double* aux)
 /* ConstructL                                                               */
 /* ====================================================================== */
 
+/* The configurations the oracle restates: every order GetWenoReconstruct
+ * has for the 2D ConstructL (mbc >= r, lib/2d/ConstructL.cpp:31-33), order 5
+ * for everything else (check_config5). */
 static int check_config(const finess_params* p)
 {
-    if (p->space_order != 5) return FINESS_ERR_CONFIG;
     if (p->weno_version != FINESS_WENO_JS && p->weno_version != FINESS_WENO_FD && p->weno_version != FINESS_WENO_Z)
         return FINESS_ERR_CONFIG;
-    if (p->mbc < 3) return FINESS_ERR_CONFIG;
+    if (!orc_weno_supported(p->weno_version, p->space_order)) return FINESS_ERR_CONFIG;
+    if (p->space_order != 5 && p->ndims != 2) return FINESS_ERR_CONFIG;
+    if (p->mbc < 3 || p->mbc < (p->space_order + 1) / 2) return FINESS_ERR_CONFIG;
     if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
     if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
+static int check_config5(const finess_params* p)
+{
+    const int rc = check_config(p);
+    return rc ? rc : (p->space_order == 5 ? FINESS_OK : FINESS_ERR_CONFIG);
+}
+
 /* One interface: characteristic-wise WENO with local Lax-Friedrichs splitting
  * (lib/2d/ConstructL.cpp:63-208 for ixy=1, :215-358 for ixy=2).
- * qs[s][m], fs[s][m]: the 6 stencil states / fluxes (cells i-3..i+2),
- * Ql = qs[2], Qr = qs[3]. Returns alpha; writes fhat[meqn]. */
-static double interface_flux(const finess_params* p, int ixy, const double qs[6][MAXEQ],
-                             const double fs[6][MAXEQ], const double* auxl, const double* auxr,
+ * qs[s][m], fs[s][m]: the ws+1 stencil states / fluxes (cells i-r..i+r-1 for
+ * ws = space_order = 2r-1), Ql = qs[r-1], Qr = qs[r]. Returns alpha; writes
+ * fhat[meqn]. */
+#define ORC_NS 12
+static double interface_flux(const finess_params* p, int ixy, const double (*qs)[MAXEQ],
+                             const double (*fs)[MAXEQ], const double* auxl, const double* auxr,
                              const double* auxavg, double* fhat)
 {
-    const int meqn = p->meqn, ws = 5;
+    const int meqn = p->meqn, ws = p->space_order, ns = ws + 1, r = ns / 2;
     double Qavg[MAXEQ];
-    for (int m = 0; m < meqn; m++) Qavg[m] = 0.5 * (qs[3][m] + qs[2][m]);
+    for (int m = 0; m < meqn; m++) Qavg[m] = 0.5 * (qs[r][m] + qs[r - 1][m]);
     (void)auxavg;
-    double qvals[MAXEQ * 6], fvals[MAXEQ * 6], wvals[MAXEQ * 6], gvals[MAXEQ * 6];
+    double qvals[MAXEQ * ORC_NS], fvals[MAXEQ * ORC_NS], wvals[MAXEQ * ORC_NS], gvals[MAXEQ * ORC_NS];
     for (int m = 0; m < meqn; m++)
-        for (int s = 0; s < 6; s++) {
-            qvals[m * 6 + s] = qs[s][m];
-            fvals[m * 6 + s] = fs[s][m];
+        for (int s = 0; s < ns; s++) {
+            qvals[m * ns + s] = qs[s][m];
+            fvals[m * ns + s] = fs[s][m];
         }
-    orc_project_left(p, ixy, Qavg, 6, qvals, wvals);
-    orc_project_left(p, ixy, Qavg, 6, fvals, gvals);
+    orc_project_left(p, ixy, Qavg, ns, qvals, wvals);
+    orc_project_left(p, ixy, Qavg, ns, fvals, gvals);
     double s1, s2;
-    orc_wave_speed(p, ixy, qs[2], qs[3], auxl, auxr, &s1, &s2);
+    orc_wave_speed(p, ixy, qs[r - 1], qs[r], auxl, auxr, &s1, &s2);
     const double alpha = Max(0.0, Max(fabs(s1), fabs(s2)));
     const double l_alpha = p->alpha_scaling * alpha;
     double ghat[MAXEQ];
     for (int m = 0; m < meqn; m++) {
-        double gp[5], gm[5];
+        double gp[ORC_NS], gm[ORC_NS];
         for (int s = 1; s <= ws; s++) {
-            gp[s - 1] = 0.5 * (gvals[m * 6 + s - 1] + l_alpha * wvals[m * 6 + s - 1]);
-            gm[s - 1] = 0.5 * (gvals[m * 6 + ws - s + 1] - l_alpha * wvals[m * 6 + ws - s + 1]);
+            gp[s - 1] = 0.5 * (gvals[m * ns + s - 1] + l_alpha * wvals[m * ns + s - 1]);
+            gm[s - 1] = 0.5 * (gvals[m * ns + ws - s + 1] - l_alpha * wvals[m * ns + ws - s + 1]);
         }
-        const double dGp = orc_weno5(p->weno_version, p->epsilon, p->power_param, gp);
-        const double dGm = orc_weno5(p->weno_version, p->epsilon, p->power_param, gm);
+        const double dGp = orc_weno(p->weno_version, ws, p->epsilon, p->power_param, gp);
+        const double dGm = orc_weno(p->weno_version, ws, p->epsilon, p->power_param, gm);
         ghat[m] = dGp + dGm;
     }
     orc_project_right(p, ixy, Qavg, 1, ghat, fhat);
@@ -562,12 +740,13 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
     double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
     if (!Fhat || !Ghat) { free(Fhat); free(Ghat); return FINESS_ERR_ARG; }
 
+    const int ns = p->space_order + 1, r = ns / 2;
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx + 1; i++)
         for (int j = 1; j <= my; j++) {
-            double qs[6][MAXEQ], fs[6][MAXEQ], gdum[MAXEQ];
-            for (int s = 1; s <= 6; s++) {
-                const int is = i - 1 + s - 3;
+            double qs[ORC_NS][MAXEQ], fs[ORC_NS][MAXEQ], gdum[MAXEQ];
+            for (int s = 1; s <= ns; s++) {
+                const int is = i - 1 + s - r;
                 for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix3(is, j, m, my, meqn, mbc)];
                 if (Fsrc)
                     for (int m = 1; m <= meqn; m++) fs[s - 1][m - 1] = Fsrc[ix3(is, j, m, my, meqn, mbc)];
@@ -585,9 +764,9 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx; i++)
         for (int j = 1; j <= my + 1; j++) {
-            double qs[6][MAXEQ], fs[6][MAXEQ], fdum[MAXEQ];
-            for (int s = 1; s <= 6; s++) {
-                const int js = j - 1 + s - 3;
+            double qs[ORC_NS][MAXEQ], fs[ORC_NS][MAXEQ], fdum[MAXEQ];
+            for (int s = 1; s <= ns; s++) {
+                const int js = j - 1 + s - r;
                 for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix3(i, js, m, my, meqn, mbc)];
                 if (Gsrc)
                     for (int m = 1; m <= meqn; m++) fs[s - 1][m - 1] = Gsrc[ix3(i, js, m, my, meqn, mbc)];
@@ -1021,7 +1200,7 @@ static int lxw_mpp_faces(const finess_params* p, double dt, const double* q, con
 int orc_lxw_mpp_faces(const finess_params* p, double dt, const double* q, const double* aux, double* Fh,
                       double* Gh, double* Fl, double* Gl, double* Fm, double* Gm)
 {
-    int rc = check_config(p);
+    int rc = check_config5(p);
     if (rc) return rc;
     const size_t numel = state_numel(p, p->meqn);
     double* F = (double*)calloc(numel, sizeof(double));
@@ -1224,6 +1403,7 @@ static int any_step(const finess_params* p, int ts, double* q, double* qstar, do
 
 static int check_stepper(const finess_params* p, int ts)
 {
+    if (ts != FINESS_TS_RK && p->space_order != 5) return FINESS_ERR_CONFIG;
     if (ts == FINESS_TS_LXW && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
     if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4)) return FINESS_ERR_CONFIG;
     return FINESS_OK;
@@ -1589,7 +1769,7 @@ static int construct_l_routed_mask(const finess_params* p, const double* q, cons
                                    long long* n_troubled)
 {
     const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
-    if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
+    if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || check_config5(p)) return FINESS_ERR_CONFIG;
     double* Fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn, sizeof(double));
     double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
     long long nt = 0;
@@ -1652,7 +1832,7 @@ int orc_construct_l_routed(const finess_params* p, const double* q, const double
 int orc_run_fixed_steps_routed(const finess_params* p, double* q, double* aux, double dt, int nsteps, double tau,
                                int per_step, double* cfl)
 {
-    int rc = check_config(p);
+    int rc = check_config5(p);
     if (rc) return rc;
     if (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->time_order != 3) return FINESS_ERR_CONFIG;
     const size_t numel = state_numel(p, p->meqn);
@@ -1776,7 +1956,7 @@ double orc_mlp_gradient(const float* P, const float* X, const float* y, int n, d
 int orc_construct_lxw_l(const finess_params* p, const double* q, const double* aux, const double* F,
                         const double* G, double* Lstar, double* smax)
 {
-    const int rc = check_config(p);
+    const int rc = check_config5(p);
     if (rc) return rc;
     return construct_l_2d(p, q, aux, F, G, Lstar, smax);
 }

@@ -22,6 +22,13 @@ extern "C" {
 double orc_weno5(int version, double eps, double power, const double* g);
 /* Nonlinear weights of JS5 (omega_k = omt_k/omts, lib/WenoReconstruct.cpp:131-134). */
 void   orc_weno5_js_weights(double eps, double power, const double* g, double* w);
+/* Any order GetWenoReconstruct has (lib/WenoReconstruct.cpp:39-73): 1, 5,
+ * 7, 9, 11 (JS), 7, 9 (Z, FD); g[0..order-1]. */
+double orc_weno(int version, int order, double eps, double power, const double* g);
+int    orc_weno_supported(int version, int order);
+/* Candidate, smoothness-indicator and linear-weight tables of order 7, 9, 11
+ * (cand[r][r], beta[r][r][r] upper triangles, d[r]); returns r or 0. */
+int    orc_weno_tables(int order, double* cand, double* beta, double* d);
 
 /* Physics callbacks of the configured app. Q, F, G are meqn-vectors. */
 void orc_flux(const finess_params* p, const double* Q, const double* aux, double* F, double* G);

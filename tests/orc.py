@@ -46,6 +46,9 @@ def oracle():
         sig = {
             "orc_weno5": (ctypes.c_double, [ctypes.c_int, ctypes.c_double, ctypes.c_double, PD]),
             "orc_weno5_js_weights": (None, [ctypes.c_double, ctypes.c_double, PD, PD]),
+            "orc_weno": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, PD]),
+            "orc_weno_supported": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+            "orc_weno_tables": (ctypes.c_int, [ctypes.c_int, PD, PD, PD]),
             "orc_flux": (None, [PP, PD, PD, PD, PD]),
             "orc_project_left": (None, [PP, ctypes.c_int, PD, ctypes.c_int, PD, PD]),
             "orc_project_right": (None, [PP, ctypes.c_int, PD, ctypes.c_int, PD, PD]),

@@ -65,6 +65,8 @@ struct finess_ctx {
     // limited Lax-Wendroff: interface fluxes (high-order, then limited in
     // place; Lax-Friedrichs), per-cell fractions, global wave-speed words
     double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
+    // interface fluxes of the face-list stage (WENO orders other than 5)
+    double *wide_fx = nullptr, *wide_fy = nullptr;
     unsigned long long* mpp_gmax = nullptr;
     double *pif_fx = nullptr, *pif_fy = nullptr;  // RK4-PIF interface fluxes of a non-characteristic stage
     // per-step conservation log (ConSoln after every accepted step)
@@ -155,6 +157,10 @@ StageArgs base_stage_args(finess_ctx* c)
     a.power = p.power_param;
     a.pp.gamma = p.gamma;
     a.pp.sw_compat_hl = p.sw_compat_hl;
+    if (p.space_order != 5) {  // the face-list stage's interface fluxes
+        a.face_x = c->wide_fx;
+        a.face_y = c->wide_fy;
+    }
     return a;
 }
 
@@ -747,6 +753,10 @@ int finess_create(const finess_params* pin, finess_ctx** out)
             if (!rc) rc = alloc(c, (void**)&c->mpp_gmax, 2 * CFL_SPREAD * sizeof(unsigned long long));
         }
     }
+    if (!rc && p.space_order != 5) {
+        rc = alloc(c, (void**)&c->wide_fx, (size_t)(p.mx + 1) * p.my * c->M * sizeof(double));
+        if (!rc) rc = alloc(c, (void**)&c->wide_fy, (size_t)p.mx * (p.my + 1) * c->M * sizeof(double));
+    }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
     if (!rc && p.ndims == 2) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
     if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2) {
@@ -789,6 +799,8 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->ctl);
     hipFree(c->mpp_fh);
     hipFree(c->mpp_gh);
+    hipFree(c->wide_fx);
+    hipFree(c->wide_fy);
     hipFree(c->mpp_fl);
     hipFree(c->mpp_gl);
     hipFree(c->mpp_amin);

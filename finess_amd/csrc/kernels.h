@@ -13,6 +13,8 @@ struct LxwArgs;
 
 // 2D fused stage.  wk: WenoKind; fp32: stencil arithmetic in FP32.
 int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st);
+// WENO orders other than 5 (face-list kernels, FP64, Runge-Kutta)
+int launch_stage2d_wide(int physics, int wk, const StageArgs& a, hipStream_t st);
 // 1D stage (two launches: faces, update).
 int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
 // Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].

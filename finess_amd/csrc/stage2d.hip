@@ -114,6 +114,7 @@ static int dispatch_weno(int wk, const StageArgs& a, hipStream_t st)
 
 int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st)
 {
+    if (wk >= WENO_JS7) return (fp32 || lxw) ? -1 : launch_stage2d_wide(physics, wk, a, st);
     if (physics == PHYS_EULER) {
         if (!fp32) return lxw ? dispatch_weno<Euler2D, double, true>(wk, a, st)
                               : dispatch_weno<Euler2D, double, false>(wk, a, st);

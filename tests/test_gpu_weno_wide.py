@@ -1,0 +1,127 @@
+"""GPU parity of the wider WENO orders (face-list kernels, stage2d_wide.hip)
+against the CPU oracle: JS 7/9/11, Z 7/9, linear 7/9 and first order
+(lib/WenoReconstruct.cpp:39-73).  Tolerance: 1e-10 relative L-inf per field
+(the FP64 bar) for ConstructL, with the face speeds to 1e-13; after the
+stated Runge-Kutta steps 1e-10 or ten times the oracle's own spread under
+1e-15 input noise, whichever is larger, with the same accepted-step count.  The
+operator bar is not the 1e-12 of the fifth-order kernel: the wide smoothness
+indicators are integer quadratic forms whose terms cancel by up to eight
+digits, and the device sums them in another order (with fused multiply-adds)
+than the oracle, so the weights differ in their last digits by more."""
+TOL = 1e-10
+import numpy as np
+import pytest
+
+import orc
+from finess_amd import (BC_EXTRAP, BC_PERIODIC, BC_REFLECT, PREC_FP32, ROUTE_ORACLE, TS_LXW, WENO_FD, WENO_JS,
+                        WENO_Z, FinessError, Solver, problems)
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2718)
+CASES = [(WENO_JS, 7), (WENO_JS, 9), (WENO_JS, 11), (WENO_Z, 7), (WENO_Z, 9), (WENO_FD, 7), (WENO_FD, 9),
+         (WENO_JS, 1)]
+
+
+def _mbc(order):
+    return max(3, (order + 1) // 2)
+
+
+def _perturbed_riemann(mx, my, version, order, **kw):
+    p, q, _ = problems.riemann4(mx=mx, my=my, space_order=order, weno_version=version, mbc=_mbc(order), **kw)
+    q = q.copy()
+    q[..., 0] *= 1.0 + 0.02 * RNG.random(q.shape[:-1])
+    return p, q
+
+
+@pytest.mark.parametrize("version,order", CASES)
+def test_construct_l_wide_euler(version, order):
+    print("case", version, order, flush=True)
+    p, q = _perturbed_riemann(37, 30, version, order)
+    orc.set_bnd(p, q)
+    L0, s0 = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("construct_l rel err", err)
+    assert np.all(err <= TOL), err
+    assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
+
+
+@pytest.mark.parametrize("bc", [BC_PERIODIC, BC_REFLECT, BC_EXTRAP])
+def test_construct_l_order11_boundaries(bc):
+    """Eleventh order reaches six cells into the ghosts on every side."""
+    p, q = _perturbed_riemann(24, 26, WENO_JS, 11, bc=[bc] * 4)
+    # momenta in both directions (the reflecting walls flip them), small
+    # enough to keep the pressure positive everywhere
+    q[..., 1] += 0.01 * q[..., 0] * (RNG.random(q.shape[:-1]) - 0.5)
+    q[..., 2] -= 0.01 * q[..., 0] * (RNG.random(q.shape[:-1]) - 0.5)
+    orc.set_bnd(p, q)
+    L0, _ = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, _ = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("bc", bc, "rel err", err)
+    assert np.all(err <= TOL), err
+
+
+@pytest.mark.parametrize("version,order", [(WENO_JS, 7), (WENO_Z, 9)])
+def test_construct_l_wide_shallow_water(version, order):
+    p, q, _ = problems.sw_bump(nx=33, ny=28, space_order=order, weno_version=version, mbc=_mbc(order))
+    q[..., 1] = 0.1 * RNG.random(q.shape[:-1])
+    q[..., 2] = 0.1 * RNG.random(q.shape[:-1])
+    orc.set_bnd(p, q)
+    L0, _ = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, _ = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("sw rel err", err)
+    assert np.all(err <= TOL), err
+
+
+@pytest.mark.parametrize("version,order,time_order", [(WENO_JS, 7, 3), (WENO_Z, 9, 3), (WENO_JS, 11, 2),
+                                                      (WENO_FD, 7, 4), (WENO_FD, 9, 3), (WENO_JS, 1, 1)])
+def test_fin_solve_rk_wide(version, order, time_order):
+    """FinSolveRK from t = 0 to 0.01 at 64^2 (the device controller for time
+    orders up to 3, the host controller above): the four-quadrant problem for
+    the nonlinear weights, the smooth density wave for the linear ones (which
+    have no shock capturing)."""
+    print("case", version, order, time_order, flush=True)
+    make = problems.density_wave if version == WENO_FD else problems.riemann4
+    p, q, _ = make(mx=64, space_order=order, weno_version=version, mbc=_mbc(order), time_order=time_order)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.01, dtv0)
+    assert rc == 0
+    # the oracle's own spread under input noise at the rounding level: the
+    # wide indicators' cancellation amplifies it over the steps (JS11 on the
+    # four-quadrant problem: 1e-15 -> ~1e-9 in four steps)
+    spread = 0.0
+    noise = np.random.default_rng(99)
+    for _ in range(2):
+        qn = q * (1.0 + 1e-15 * (noise.random(q.shape) - 0.5))
+        dtn = np.array([0.0, p.initial_dt, p.max_dt])
+        assert orc.fin_solve(p, qn, 0.0, 0.01, dtn)[0] == 0
+        spread = max(spread, np.max(orc.rel_linf(orc.interior(p, qn), orc.interior(p, q0))))
+    tol = max(TOL, 10.0 * spread)
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, 0.01, dtv1)
+    assert abs(t1 - t0) <= 1e-14
+    assert abs(dtv1[1] - dtv0[1]) <= tol * dtv0[1]
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("steps", st0.steps_accepted, "rel err", err, "oracle spread", spread, "tol", tol)
+    assert np.all(err <= tol), err
+
+
+def test_wide_orders_rejected_where_the_device_has_no_path():
+    """Orders other than 5 run the 2D Runge-Kutta / ConstructL path in FP64
+    only; the rest is refused at configuration time, as are the pairs
+    GetWenoReconstruct lacks."""
+    bad = [dict(space_order=7, precision=PREC_FP32), dict(space_order=7, time_stepping_method=TS_LXW, mbc=5),
+           dict(space_order=7, routing=ROUTE_ORACLE), dict(space_order=9, mbc=4),
+           dict(space_order=11, weno_version=WENO_Z, mbc=6), dict(space_order=3)]
+    for kw in bad:
+        with pytest.raises(FinessError):
+            problems.riemann4(mx=32, **kw)

@@ -225,3 +225,27 @@ def test_construct_l_rejects_what_the_reference_lacks():
         sm = np.zeros(p.shape_smax())
         rc = orc.oracle().orc_construct_l(ctypes.byref(p), orc.ptr(q), orc.ptr(aux), orc.ptr(L), orc.ptr(sm))
         assert rc != 0, kw
+
+
+def test_wide_indicators_are_ill_conditioned_over_steps():
+    """Why the GPU bar of the wide orders after several steps is set relative
+    to the oracle's own spread: a 1e-15 relative input perturbation moves the
+    oracle's state after four SSP-RK steps of the 64^2 four-quadrant problem
+    by more than 1e-10 with WENO-JS11 (the integer indicators of
+    lib/WenoReconstruct.cpp:309-352 cancel by up to eight digits), while the
+    fifth-order scheme stays at the rounding level."""
+    rng = np.random.default_rng(5)
+
+    def spread(order, to):
+        p, q, _ = problems.riemann4(mx=64, space_order=order, weno_version=WENO_JS, mbc=max(3, (order + 1) // 2),
+                                    time_order=to)
+        outs = []
+        for k in range(2):
+            qq = q * (1.0 + 1e-15 * (rng.random(q.shape) - 0.5)) if k else q.copy()
+            dtv = np.array([0.0, p.initial_dt, p.max_dt])
+            assert orc.fin_solve(p, qq, 0.0, 0.01, dtv)[0] == 0
+            outs.append(qq)
+        return np.max(orc.rel_linf(orc.interior(p, outs[1]), orc.interior(p, outs[0])))
+
+    assert spread(11, 2) > 1e-10
+    assert spread(5, 3) < 1e-13

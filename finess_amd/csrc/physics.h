@@ -144,6 +144,20 @@ struct Euler2D {
         w[4] = e.a5 * q[1] + e.b5 * q[0] + t;
     }
 
+    // Row k of left() alone (the same expression), for a field index that
+    // is uniform over the wave.
+    template <typename R>
+    __host__ __device__ __forceinline__ static R left_row(const Eig<R>& e, const R q[5], int k)
+    {
+        switch (k) {
+        case 0: return e.a1 * q[1] + e.b1 * q[0] + e.cc * (q[4] - e.u2 * q[2] - e.u3 * q[3]);
+        case 1: return e.k2 * q[0] + e.e1 * q[1] + e.e2 * q[2] + e.e3 * q[3] + e.m5 * q[4];
+        case 2: return q[2] - e.u2 * q[0];
+        case 3: return q[3] - e.u3 * q[0];
+        default: return e.a5 * q[1] + e.b5 * q[0] + e.cc * (q[4] - e.u2 * q[2] - e.u3 * q[3]);
+        }
+    }
+
     template <typename R>
     __host__ __device__ __forceinline__ static void right(const Eig<R>& e, const R w[5], R q[5])
     {
@@ -239,6 +253,16 @@ struct ShallowWater2D {
         w[0] = e.a1 * q[0] - e.b * q[1];
         w[1] = -e.u2 * q[0] + q[2];
         w[2] = e.a3 * q[0] + e.b * q[1];
+    }
+
+    template <typename R>
+    __host__ __device__ __forceinline__ static R left_row(const Eig<R>& e, const R q[3], int k)
+    {
+        switch (k) {
+        case 0: return e.a1 * q[0] - e.b * q[1];
+        case 1: return -e.u2 * q[0] + q[2];
+        default: return e.a3 * q[0] + e.b * q[1];
+        }
     }
 
     template <typename R>

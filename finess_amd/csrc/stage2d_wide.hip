@@ -4,14 +4,12 @@
 // the first-order value (:702-713), in FP64.
 //
 // The stencil of an interface grows to 2r cells (r = (order + 1) / 2, up to
-// 12), which no longer fits the LDS tile of the fifth-order kernel, so these
-// orders run as a face list: one thread per interface gathers its stencil
-// from global memory (L1/L2 serve the neighbours' overlapping reads), forms
-// the interface flux of lib/2d/ConstructL.cpp:63-208 (x faces) / :215-358
-// (y faces) and writes it to a face array; a second kernel takes the flux
-// differences and does the stage update.  The characteristic fields are
-// reconstructed one at a time, each from a fresh pass over the stencil, so a
-// thread holds 2 (2r - 1) split values at a time instead of all of them.
+// 12) and the fused fifth-order kernel's register budget does not stretch
+// that far, so these orders run in two steps: a face kernel per direction
+// (one thread per interface of a 16 x 16 tile, the tile's stencil cells
+// staged in LDS in FP64) forms the interface flux of lib/2d/ConstructL.cpp:
+// 63-208 (x faces) / :215-358 (y faces) into a face array, and an update
+// kernel takes the flux differences and does the stage update.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -140,75 +138,86 @@ __device__ __forceinline__ double wide_beta(const double* g, int k)
     return b;
 }
 
-// Reconstruction at x_{i+1/2} from g[0..2R-2] (g[R-1] = u_i).
+template <int R>
+__device__ __forceinline__ double wide_cand(const double* g, int k)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < R; j++) s = fma(WideTab<R>::c(k, j), g[k + j], s);
+    return s;
+}
+
+// Reconstruction at x_{i+1/2} from g[0..2R-2] (g[R-1] = u_i).  JS and the
+// linear weights run one candidate at a time (nothing but the running sums
+// stays live); Z needs every indicator for tau first.
 template <int R, int VER>
 __device__ __forceinline__ double weno_wide(const double* g, double eps, double pw)
 {
     using T = WideTab<R>;
-    double u[R];
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < R; j++) s = fma(T::c(k, j), g[k + j], s);
-        u[k] = s;
-    }
+    double num = 0.0, den = 0.0;
     if (VER == WV_FD) {
-        double v = 0.0;
 #pragma unroll
-        for (int k = 0; k < R; k++) v = fma(T::d(k), u[k], v);
-        return v;
+        for (int k = 0; k < R; k++) num = fma(T::d(k), wide_cand<R>(g, k), num);
+        return num;
+    }
+    if (VER == WV_JS) {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const double a = eps + wide_beta<R>(g, k);
+            const double omt = T::d(k) * (pw == 2.0 ? 1.0 / (a * a) : pow(a, -pw));
+            num = fma(omt, wide_cand<R>(g, k), num);
+            den += omt;
+        }
+        return num / den;
     }
     double beta[R];
 #pragma unroll
     for (int k = 0; k < R; k++) beta[k] = wide_beta<R>(g, k);
-    double tau = 0.0;
-    if (VER == WV_Z) {
-        // lib/WenoReconstruct.cpp:470 (order 7) and :549 (order 9)
-        if (R == 4) tau = fabs(beta[0] + 3. * (beta[1] - beta[2]) - beta[3]);
-        else tau = fabs(beta[R - 1] - beta[0]);
-    }
-    double num = 0.0, den = 0.0;
+    // lib/WenoReconstruct.cpp:470 (order 7) and :549 (order 9)
+    const double tau = R == 4 ? fabs(beta[0] + 3. * (beta[1] - beta[2]) - beta[3]) : fabs(beta[R - 1] - beta[0]);
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        const double a = eps + beta[k];
-        double omt;
-        if (VER == WV_Z) {
-            const double z = tau / a;
-            omt = T::d(k) * (1.0 + (pw == 2.0 ? z * z : pow(z, pw)));
-        } else {
-            omt = T::d(k) * (pw == 2.0 ? 1.0 / (a * a) : pow(a, -pw));
-        }
-        num = fma(omt, u[k], num);
+        const double z = tau / (eps + beta[k]);
+        const double omt = T::d(k) * (1.0 + (pw == 2.0 ? z * z : pow(z, pw)));
+        num = fma(omt, wide_cand<R>(g, k), num);
         den += omt;
     }
     return num / den;
 }
 
-// Rotated state and flux of stencil cell s of the face (cells i-R..i+R-1
-// along the normal).
-template <class P, int DIR>
-__device__ __forceinline__ void wide_cell(const StageArgs& a, int i, int j, int s, int R, double q[P::MEQN],
-                                          double f[P::MEQN], double rec[P::NV])
-{
-    double qc[P::MEQN];
-    if (DIR == 0) load_state<P>(a, i - R + s, j, qc);
-    else load_state<P>(a, i, j - R + s, qc);
-    P::cell_record(qc, a.pp, rec);
-    P::state(rec, DIR, q);
-    P::flux(q, rec, f);
-}
+// A workgroup takes a tile of WT x WT faces of one direction and stages
+// every cell their stencils reach in LDS once: the cell record and the flux
+// of the state rotated to the face normal, in FP64 (lib/2d/ConstructL.cpp:
+// 88-137 samples q and f over the stencil the same way, per face).
+constexpr int WT = 16;
 
-// Interface flux of face (i, j) of direction DIR; returns alpha.  ORDER 1:
-// the split parts are the upwind values (R = 1).
-template <class P, int R, int VER, int DIR>
-__device__ __forceinline__ double wide_face_flux(const StageArgs& a, int i, int j, double fh[P::MEQN])
+template <class P, int R, int DIR>
+struct WideSmem {
+    static constexpr int HI = DIR == 0 ? WT + 2 * R - 1 : WT;  // cells along i
+    static constexpr int HJ = DIR == 0 ? WT : WT + 2 * R - 1;  // cells along j
+    static constexpr int NC = HI * HJ;
+    double rec[P::NV][NC];
+    double f[P::MEQN][NC];
+};
+
+// Interface flux of the face whose stencil cell s sits at LDS index
+// base + s * stride (cells i-R..i+R-1 along the normal); returns alpha.  The
+// characteristic fields are reconstructed one per pass (a loop the compiler
+// keeps), each pass projecting the stencil with one row of the left
+// eigenvectors, so a thread holds the 2 (2R - 1) split values of one field
+// at a time.  R = 1 (first order): the split parts are the upwind values.
+template <class P, int R, int VER, int DIR, class S>
+__device__ __forceinline__ double wide_face_flux(const S& sm, int base, int stride, const StageArgs& a,
+                                                 double fh[P::MEQN])
 {
     constexpr int M = P::MEQN, NV = P::NV, WS = 2 * R - 1, NS = 2 * R;
-    // eigen-system at the mean of the two cells beside the face, LF speed
-    double ql[M], qr[M], fl[M], fr[M], cl[NV], cr[NV];
-    wide_cell<P, DIR>(a, i, j, R - 1, R, ql, fl, cl);
-    wide_cell<P, DIR>(a, i, j, R, R, qr, fr, cr);
+    double cl[NV], cr[NV], ql[M], qr[M];
+    for (int v = 0; v < NV; v++) {
+        cl[v] = sm.rec[v][base + (R - 1) * stride];
+        cr[v] = sm.rec[v][base + R * stride];
+    }
+    P::state(cl, DIR, ql);
+    P::state(cr, DIR, qr);
     double qa[M];
     for (int m = 0; m < M; m++) qa[m] = 0.5 * (qr[m] + ql[m]);
     typename P::template Eig<double> e;
@@ -216,22 +225,31 @@ __device__ __forceinline__ double wide_face_flux(const StageArgs& a, int i, int 
     const double alpha = P::wave_speed(ql, qr, cl, cr, a.pp);
     const double la = a.alpha_scaling * alpha;
     double ghat[M];
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < M; m++) {
         double gp[WS], gm[WS];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            double q[M], f[M], rec[NV], w[M], gw[M];
-            wide_cell<P, DIR>(a, i, j, s, R, q, f, rec);
-            P::left(e, q, w);
-            P::left(e, f, gw);
+            const int c = base + s * stride;
+            double rec[NV], q[M], f[M];
+            for (int v = 0; v < M; v++) rec[v] = sm.rec[v][c];  // state() reads the conserved part only
+            P::state(rec, DIR, q);
+            for (int k = 0; k < M; k++) f[k] = sm.f[k][c];
+            const double w = P::left_row(e, q, m), gw = P::left_row(e, f, m);
             // lib/2d/ConstructL.cpp:179-185: gp from points 1..ws, gm from
             // points ws+1..2 (reversed)
-            if (s < WS) gp[s] = 0.5 * (gw[m] + la * w[m]);
-            if (s >= 1) gm[WS - s] = 0.5 * (gw[m] - la * w[m]);
+            if (s < WS) gp[s] = 0.5 * (gw + la * w);
+            if (s >= 1) gm[WS - s] = 0.5 * (gw - la * w);
+            // keep each cell's LDS reads in their iteration: hoisting all of
+            // them ahead of the arithmetic costs 10 registers per cell
+            asm volatile("" ::: "memory");
         }
-        if constexpr (R == 1) ghat[m] = gp[0] + gm[0];
-        else ghat[m] = weno_wide<R, VER>(gp, a.eps, a.power) + weno_wide<R, VER>(gm, a.eps, a.power);
+        double gh;
+        if constexpr (R == 1) gh = gp[0] + gm[0];
+        else gh = weno_wide<R, VER>(gp, a.eps, a.power) + weno_wide<R, VER>(gm, a.eps, a.power);
+#pragma unroll
+        for (int k = 0; k < M; k++)
+            if (m == k) ghat[k] = gh;
     }
     double fr_[M];
     P::right(e, ghat, fr_);
@@ -248,23 +266,53 @@ __device__ __forceinline__ double wave_max_w(double v)
     return v;
 }
 
-// One thread per face of direction DIR: the face flux into a.face_x / a.face_y
-// ([m][nf], x face (i, j) at i*my + j, y face at i*(my+1) + j), the face speed
-// into smax_faces_* when set, and the workgroup maximum over the counted faces
-// into a.smax[DIR * CFL_SPREAD + block % CFL_SPREAD].
+// Tiles of faces along each axis for direction DIR.
+__host__ __device__ __forceinline__ int wide_tiles_j(int dir, int my) { return (my + (dir == 1) + WT - 1) / WT; }
+__host__ __device__ __forceinline__ int wide_tiles_i(int dir, int mx) { return (mx + (dir == 0) + WT - 1) / WT; }
+
+// One workgroup per WT x WT tile of faces of direction DIR: the face flux
+// into a.face_x / a.face_y ([m][nf], x face (i, j) at i*my + j, y face at
+// i*(my+1) + j), the face speed into smax_faces_* when set, and the
+// workgroup maximum over the counted faces into
+// a.smax[DIR * CFL_SPREAD + block % CFL_SPREAD].
 template <class P, int R, int VER, int DIR>
 __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
 {
     if (stage_halted(a)) return;
-    constexpr int M = P::MEQN;
-    const int ni = DIR == 0 ? a.mx + 1 : a.mx, nj = DIR == 0 ? a.my : a.my + 1;
-    const long long nf = (long long)ni * nj;
-    const long long f = (long long)blockIdx.x * 256 + threadIdx.x;
+    constexpr int M = P::MEQN, NV = P::NV;
+    using S = WideSmem<P, R, DIR>;
+    __shared__ S sm;
+    const int ntj = wide_tiles_j(DIR, a.my);
+    const int ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
+    const int i0 = ti * WT, j0 = tj * WT;
+    const int t = threadIdx.x;
+    // LDS cell (ci, cj) is cell (i0 + ci - R, j0 + cj) for x faces and
+    // (i0 + ci, j0 + cj - R) for y faces; cells beyond the last face's
+    // stencil (index n + R - 1 along the normal) are not loaded
+    for (int c = t; c < S::NC; c += 256) {
+        const int ci = c / S::HJ, cj = c - ci * S::HJ;
+        const int gi = i0 + ci - (DIR == 0 ? R : 0), gj = j0 + cj - (DIR == 1 ? R : 0);
+        const bool ok = DIR == 0 ? (gi <= a.mx + R - 1 && gj < a.my) : (gi < a.mx && gj <= a.my + R - 1);
+        if (!ok) continue;
+        double q[M], rec[NV], qr[M], f[M];
+        load_state<P>(a, gi, gj, q);
+        P::cell_record(q, a.pp, rec);
+        P::state(rec, DIR, qr);
+        P::flux(qr, rec, f);
+        for (int v = 0; v < NV; v++) sm.rec[v][c] = rec[v];
+        for (int k = 0; k < M; k++) sm.f[k][c] = f[k];
+    }
+    __syncthreads();
+    const int fi = t / WT, fj = t - fi * WT;
+    const int i = i0 + fi, j = j0 + fj;
+    const bool valid = DIR == 0 ? (i <= a.mx && j < a.my) : (i < a.mx && j <= a.my);
     double amax = 0.0;
-    if (f < nf) {
-        const int i = (int)(f / nj), j = (int)(f - (long long)i * nj);
+    if (valid) {
+        const int nj = DIR == 0 ? a.my : a.my + 1;
+        const long long nf = DIR == 0 ? (long long)(a.mx + 1) * a.my : (long long)a.mx * (a.my + 1);
+        const long long f = (long long)i * nj + j;
         double fh[M];
-        const double alpha = wide_face_flux<P, R, VER, DIR>(a, i, j, fh);
+        const double alpha = wide_face_flux<P, R, VER, DIR>(sm, fi * S::HJ + fj, DIR == 0 ? S::HJ : 1, a, fh);
         double* dst = DIR == 0 ? a.face_x : a.face_y;
         for (int m = 0; m < M; m++) dst[m * nf + f] = fh[m];
         double* sf = DIR == 0 ? a.smax_faces_x : a.smax_faces_y;
@@ -276,10 +324,10 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
     if (!a.smax) return;
     __shared__ double red[4];
     amax = wave_max_w(amax);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = t & 63, w = t >> 6;
     if (lane == 0) red[w] = amax;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         double v = red[0];
         for (int k = 1; k < 4; k++) v = red[k] > v ? red[k] : v;
         atomicMax(&a.smax[DIR * CFL_SPREAD + blockIdx.x % CFL_SPREAD], (unsigned long long)__double_as_longlong(v));
@@ -312,9 +360,10 @@ __global__ __launch_bounds__(256) void k_wide_update(StageArgs a)
 template <class P, int R, int VER>
 static int launch_wide_t(const StageArgs& a, hipStream_t st)
 {
-    const long long nfx = (long long)(a.mx + 1) * a.my, nfy = (long long)a.mx * (a.my + 1);
-    hipLaunchKernelGGL((k_wide_faces<P, R, VER, 0>), dim3((unsigned)((nfx + 255) / 256)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((k_wide_faces<P, R, VER, 1>), dim3((unsigned)((nfy + 255) / 256)), dim3(256), 0, st, a);
+    const unsigned bx = (unsigned)wide_tiles_i(0, a.mx) * wide_tiles_j(0, a.my);
+    const unsigned by = (unsigned)wide_tiles_i(1, a.mx) * wide_tiles_j(1, a.my);
+    hipLaunchKernelGGL((k_wide_faces<P, R, VER, 0>), dim3(bx), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_wide_faces<P, R, VER, 1>), dim3(by), dim3(256), 0, st, a);
     hipLaunchKernelGGL((k_wide_update<P::MEQN>), dim3((unsigned)((a.plane + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }

@@ -1,9 +1,9 @@
 """GPU parity of the wider WENO orders (face-list kernels, stage2d_wide.hip)
 against the CPU oracle: JS 7/9/11, Z 7/9, linear 7/9 and first order
-(lib/WenoReconstruct.cpp:39-73).  Tolerance: 1e-10 relative L-inf per field
-(the FP64 bar) for ConstructL, with the face speeds to 1e-13; after the
-stated Runge-Kutta steps 1e-10 or ten times the oracle's own spread under
-1e-15 input noise, whichever is larger, with the same accepted-step count.  The
+(lib/WenoReconstruct.cpp:39-73).  Tolerance, relative L-inf per field: 1e-10
+(the FP64 bar) or ten times the oracle's own spread under 1e-15 relative
+input noise, whichever is larger, for ConstructL (face speeds to 1e-13) and
+after the stated Runge-Kutta steps (with the same accepted-step count).  The
 operator bar is not the 1e-12 of the fifth-order kernel: the wide smoothness
 indicators are integer quadratic forms whose terms cancel by up to eight
 digits, and the device sums them in another order (with fused multiply-adds)
@@ -27,6 +27,18 @@ def _mbc(order):
     return max(3, (order + 1) // 2)
 
 
+def _l_tol(p, q, L0):
+    """max(TOL, ten times the oracle's own ConstructL spread under 1e-15
+    relative input noise)."""
+    noise = np.random.default_rng(98)
+    spread = 0.0
+    for _ in range(2):
+        qn = q * (1.0 + 1e-15 * (noise.random(q.shape) - 0.5))
+        Ln, _ = orc.construct_l(p, qn)
+        spread = max(spread, np.max(orc.rel_linf(orc.interior(p, Ln), orc.interior(p, L0))))
+    return max(TOL, 10.0 * spread), spread
+
+
 def _perturbed_riemann(mx, my, version, order, **kw):
     p, q, _ = problems.riemann4(mx=mx, my=my, space_order=order, weno_version=version, mbc=_mbc(order), **kw)
     q = q.copy()
@@ -43,8 +55,9 @@ def test_construct_l_wide_euler(version, order):
     with Solver(p) as s:
         L1, s1 = s.construct_l(q)
     err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
-    print("construct_l rel err", err)
-    assert np.all(err <= TOL), err
+    tol, spread = _l_tol(p, q, L0)
+    print("construct_l rel err", err, "oracle spread", spread)
+    assert np.all(err <= tol), err
     assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
 
 
@@ -61,8 +74,9 @@ def test_construct_l_order11_boundaries(bc):
     with Solver(p) as s:
         L1, _ = s.construct_l(q)
     err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
-    print("bc", bc, "rel err", err)
-    assert np.all(err <= TOL), err
+    tol, spread = _l_tol(p, q, L0)
+    print("bc", bc, "rel err", err, "oracle spread", spread)
+    assert np.all(err <= tol), err
 
 
 @pytest.mark.parametrize("version,order", [(WENO_JS, 7), (WENO_Z, 9)])
@@ -75,8 +89,9 @@ def test_construct_l_wide_shallow_water(version, order):
     with Solver(p) as s:
         L1, _ = s.construct_l(q)
     err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
-    print("sw rel err", err)
-    assert np.all(err <= TOL), err
+    tol, spread = _l_tol(p, q, L0)
+    print("sw rel err", err, "oracle spread", spread)
+    assert np.all(err <= tol), err
 
 
 @pytest.mark.parametrize("version,order,time_order", [(WENO_JS, 7, 3), (WENO_Z, 9, 3), (WENO_JS, 11, 2),

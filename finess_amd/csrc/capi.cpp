@@ -67,6 +67,9 @@ struct finess_ctx {
     double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
     // interface fluxes of the face-list stage (WENO orders other than 5)
     double *wide_fx = nullptr, *wide_fy = nullptr;
+    // global_alpha: the two global wave speeds and their spread words
+    double* galpha = nullptr;
+    unsigned long long* gspread = nullptr;
     unsigned long long* mpp_gmax = nullptr;
     double *pif_fx = nullptr, *pif_fy = nullptr;  // RK4-PIF interface fluxes of a non-characteristic stage
     // per-step conservation log (ConSoln after every accepted step)
@@ -160,6 +163,10 @@ StageArgs base_stage_args(finess_ctx* c)
     if (p.space_order != 5) {  // the face-list stage's interface fluxes
         a.face_x = c->wide_fx;
         a.face_y = c->wide_fy;
+    }
+    if (p.global_alpha) {
+        a.galpha = c->galpha;
+        a.gspread = c->gspread;
     }
     return a;
 }
@@ -646,7 +653,12 @@ int finess_params_finalize(finess_params* p)
             p->precision != FINESS_PREC_FP64 || p->mbc < r || p->mx < r || p->my < r)
             return FINESS_ERR_CONFIG;
     }
-    if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
+    if (p->source_term) return FINESS_ERR_CONFIG;
+    // global_alpha (GlobalWaveSpd before each operator call) runs in the 2D
+    // stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), ConstructL
+    if (p->global_alpha && (p->ndims != 2 || p->routing != FINESS_ROUTE_NONE ||
+                            p->time_stepping_method == FINESS_TS_PIF))
+        return FINESS_ERR_CONFIG;
     if (p->ndims == 1) {
         if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;
         if (p->time_stepping_method != FINESS_TS_RK) return FINESS_ERR_CONFIG;
@@ -753,6 +765,10 @@ int finess_create(const finess_params* pin, finess_ctx** out)
             if (!rc) rc = alloc(c, (void**)&c->mpp_gmax, 2 * CFL_SPREAD * sizeof(unsigned long long));
         }
     }
+    if (!rc && p.global_alpha) {
+        rc = alloc(c, (void**)&c->galpha, 2 * sizeof(double));
+        if (!rc) rc = alloc(c, (void**)&c->gspread, 2 * CFL_SPREAD * sizeof(unsigned long long));
+    }
     if (!rc && p.space_order != 5) {
         rc = alloc(c, (void**)&c->wide_fx, (size_t)(p.mx + 1) * p.my * c->M * sizeof(double));
         if (!rc) rc = alloc(c, (void**)&c->wide_fy, (size_t)p.mx * (p.my + 1) * c->M * sizeof(double));
@@ -801,6 +817,8 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->mpp_gh);
     hipFree(c->wide_fx);
     hipFree(c->wide_fy);
+    hipFree(c->galpha);
+    hipFree(c->gspread);
     hipFree(c->mpp_fl);
     hipFree(c->mpp_gl);
     hipFree(c->mpp_amin);

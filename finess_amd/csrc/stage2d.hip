@@ -1,6 +1,9 @@
 // stage2d.hip -- gfx950 kernels of the 2D path: the fused stage kernel and
 // the small helpers around it (layout conversion, CFL-slot reset).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+
 #include "kernels.h"
 #include "stage2d_body.h"
 
@@ -112,8 +115,64 @@ static int dispatch_weno(int wk, const StageArgs& a, hipStream_t st)
     }
 }
 
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v);
+
+// GlobalWaveSpd (lib/2d/GlobalWaveSpd.cpp:20-80): max over the interior
+// cells of the SetWaveSpd speed with both sides equal to the cell, per
+// direction, folded into gspread[dir][block % CFL_SPREAD].
+template <class P>
+__global__ __launch_bounds__(256) void k_global_speed(StageArgs a)
+{
+    if (stage_halted(a)) return;
+    double ax = 0.0, ay = 0.0;
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < a.plane; c += (long long)gridDim.x * 256) {
+        const int i = (int)(c / a.my), j = (int)(c - (long long)i * a.my);
+        double q[P::MEQN], rec[P::NV], qx[P::MEQN], qy[P::MEQN];
+        load_state<P>(a, i, j, q);
+        P::cell_record(q, a.pp, rec);
+        P::state(rec, 0, qx);
+        P::state(rec, 1, qy);
+        const double sx = P::wave_speed(qx, qx, rec, rec, a.pp), sy = P::wave_speed(qy, qy, rec, rec, a.pp);
+        // Max(alpha, s) as the reference folds it; a NaN speed must not vanish
+        ax = sx != sx ? __builtin_inf() : (ax > sx ? ax : sx);
+        ay = sy != sy ? __builtin_inf() : (ay > sy ? ay : sy);
+    }
+    block_max2<256>(ax, ay, a.gspread);
+}
+
+// Folds the spread words into the two speeds (one workgroup).
+__global__ __launch_bounds__(64) void k_global_speed_final(StageArgs a)
+{
+    if (stage_halted(a)) return;
+    const int t = threadIdx.x;
+    double vx = __longlong_as_double((long long)a.gspread[t]);
+    double vy = __longlong_as_double((long long)a.gspread[CFL_SPREAD + t]);
+    vx = wave_max(vx);
+    vy = wave_max(vy);
+    if (t == 0) {
+        a.galpha[0] = vx;
+        a.galpha[1] = vy;
+    }
+}
+
+static int launch_global_speed(int physics, const StageArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(2 * CFL_SPREAD), 0, st, a.gspread, 2 * CFL_SPREAD, 0ull);
+    const unsigned nb = (unsigned)std::min<long long>(1024, (a.plane + 255) / 256);
+    if (physics == PHYS_EULER) hipLaunchKernelGGL(k_global_speed<Euler2D>, dim3(nb), dim3(256), 0, st, a);
+    else if (physics == PHYS_SHALLOW_WATER) hipLaunchKernelGGL(k_global_speed<ShallowWater2D>, dim3(nb), dim3(256), 0, st, a);
+    else return -1;
+    hipLaunchKernelGGL(k_global_speed_final, dim3(1), dim3(64), 0, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st)
 {
+    if (a.galpha) {
+        if (!a.gspread) return -1;
+        const int rc = launch_global_speed(physics, a, st);
+        if (rc) return rc;
+    }
     if (wk >= WENO_JS7) return (fp32 || lxw) ? -1 : launch_stage2d_wide(physics, wk, a, st);
     if (physics == PHYS_EULER) {
         if (!fp32) return lxw ? dispatch_weno<Euler2D, double, true>(wk, a, st)

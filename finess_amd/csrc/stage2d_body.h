@@ -122,6 +122,11 @@ struct StageArgs {
     // limiter sets to the global wave speeds
     double* face_x;
     double* face_y;
+    // global_alpha (null: off): galpha[0..1] the global wave speeds of the
+    // stage source in x and y (lib/2d/GlobalWaveSpd.cpp:20-80), formed by
+    // launch_stage2d through the [2][CFL_SPREAD] words of gspread
+    double* galpha;
+    unsigned long long* gspread;
 };
 
 // beta * dt of the stage update (dt from the device controller when set)
@@ -220,10 +225,12 @@ __host__ __device__ __forceinline__ void phase_prefetch(int tid, int i0, int j0,
 // Mean state of a face from the two cells beside it, its eigen-system and
 // its Lax-Friedrichs speed (lib/2d/ConstructL.cpp:72-77 and 149-174 for x,
 // :224-229 and 300-325 for y); returns alpha and sets la = ascale * alpha.
+// amin: the global speed of the direction when global_alpha is set, else 0
+// (alpha = Max(alpha1, local), lib/2d/ConstructL.cpp:172,323).
 template <class P, typename R, int NC>
 __host__ __device__ __forceinline__ R face_setup(const R (*cv)[NC], int base, int stride, int dir,
                                                  const PhysParams& pp, R ascale,
-                                                 typename P::template Eig<R>& e, R& la)
+                                                 typename P::template Eig<R>& e, R& la, R amin = R(0))
 {
     constexpr int M = P::MEQN, NV = P::NV;
     R cl[NV], cr[NV], ql[M], qr[M];
@@ -236,7 +243,8 @@ __host__ __device__ __forceinline__ R face_setup(const R (*cv)[NC], int base, in
     R qa[M];
     for (int k = 0; k < M; k++) qa[k] = R(0.5) * (qr[k] + ql[k]);
     P::eig(qa, pp, e);
-    const R alpha = P::wave_speed(ql, qr, cl, cr, pp);
+    const R ws = P::wave_speed(ql, qr, cl, cr, pp);
+    const R alpha = amin > ws ? amin : ws;  // a NaN speed stays NaN
     la = ascale * alpha;
     return alpha;
 }
@@ -282,12 +290,13 @@ template <class P, typename R, int WK, bool LXW, int NC>
 __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int stride, int dir,
                                                 const double* lflux, long long lbase, long long lstride,
                                                 long long lplane, const PhysParams& pp,
-                                                const WenoParams<R>& wp, R ascale, R fh[P::MEQN])
+                                                const WenoParams<R>& wp, R ascale, R fh[P::MEQN],
+                                                R amin = R(0))
 {
     constexpr int M = P::MEQN;
     typename P::template Eig<R> e;
     R la;
-    const R alpha = face_setup<P, R, NC>(cv, base, stride, dir, pp, ascale, e, la);
+    const R alpha = face_setup<P, R, NC>(cv, base, stride, dir, pp, ascale, e, la, amin);
     const WenoParams<R> wp4{R(4) * wp.eps, wp.power};
     // The two split parts are reconstructed one after the other: only one
     // part's 25 projected values are live at a time, at the price of forming
@@ -364,8 +373,9 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
     bool count = g.count;
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
+    const R amin = a.galpha ? R(a.galpha[dir]) : R(0);
     const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F, lbase,
-                                                    lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh);
+                                                    lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh, amin);
     (void)sizeof(S);
     for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
     if constexpr (LXW) {

@@ -222,7 +222,9 @@ __device__ __forceinline__ double wide_face_flux(const S& sm, int base, int stri
     for (int m = 0; m < M; m++) qa[m] = 0.5 * (qr[m] + ql[m]);
     typename P::template Eig<double> e;
     P::eig(qa, a.pp, e);
-    const double alpha = P::wave_speed(ql, qr, cl, cr, a.pp);
+    const double ws = P::wave_speed(ql, qr, cl, cr, a.pp);
+    const double amin = a.galpha ? a.galpha[DIR] : 0.0;  // global_alpha
+    const double alpha = amin > ws ? amin : ws;
     const double la = a.alpha_scaling * alpha;
     double ghat[M];
 #pragma unroll 1

@@ -663,7 +663,8 @@ static int check_config(const finess_params* p)
     if (p->space_order != 5 && p->ndims != 2) return FINESS_ERR_CONFIG;
     if (p->mbc < 3 || p->mbc < (p->space_order + 1) / 2) return FINESS_ERR_CONFIG;
     if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
-    if (p->global_alpha || p->source_term) return FINESS_ERR_CONFIG;
+    if (p->source_term) return FINESS_ERR_CONFIG;
+    if (p->global_alpha && p->ndims != 2) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -681,12 +682,11 @@ static int check_config5(const finess_params* p)
 #define ORC_NS 12
 static double interface_flux(const finess_params* p, int ixy, const double (*qs)[MAXEQ],
                              const double (*fs)[MAXEQ], const double* auxl, const double* auxr,
-                             const double* auxavg, double* fhat)
+                             double alpha_g, double* fhat)
 {
     const int meqn = p->meqn, ws = p->space_order, ns = ws + 1, r = ns / 2;
     double Qavg[MAXEQ];
     for (int m = 0; m < meqn; m++) Qavg[m] = 0.5 * (qs[r][m] + qs[r - 1][m]);
-    (void)auxavg;
     double qvals[MAXEQ * ORC_NS], fvals[MAXEQ * ORC_NS], wvals[MAXEQ * ORC_NS], gvals[MAXEQ * ORC_NS];
     for (int m = 0; m < meqn; m++)
         for (int s = 0; s < ns; s++) {
@@ -697,7 +697,9 @@ static double interface_flux(const finess_params* p, int ixy, const double (*qs)
     orc_project_left(p, ixy, Qavg, ns, fvals, gvals);
     double s1, s2;
     orc_wave_speed(p, ixy, qs[r - 1], qs[r], auxl, auxr, &s1, &s2);
-    const double alpha = Max(0.0, Max(fabs(s1), fabs(s2)));
+    /* alpha_g: the global speed of the direction with global_alpha, else 0
+     * (lib/2d/ConstructL.cpp:48-54,172,323) */
+    const double alpha = Max(alpha_g, Max(fabs(s1), fabs(s2)));
     const double l_alpha = p->alpha_scaling * alpha;
     double ghat[MAXEQ];
     for (int m = 0; m < meqn; m++) {
@@ -741,6 +743,10 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
     if (!Fhat || !Ghat) { free(Fhat); free(Ghat); return FINESS_ERR_ARG; }
 
     const int ns = p->space_order + 1, r = ns / 2;
+    /* GlobalWaveSpd on the state the operator reads (lib/2d/ConstructL.cpp:48-54,
+     * lib/2d/ConstructLxWL.cpp:88-96) */
+    double alpha1 = 0.0, alpha2 = 0.0;
+    if (p->global_alpha) orc_global_wave_spd(p, q, aux, &alpha1, &alpha2);
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx + 1; i++)
         for (int j = 1; j <= my; j++) {
@@ -755,7 +761,7 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
             }
             double fh[MAXEQ];
             const double alpha = interface_flux(p, 1, qs, fs, &aux[ix3(i - 1, j, 1, my, na, mbc)],
-                                                &aux[ix3(i, j, 1, my, na, mbc)], NULL, fh);
+                                                &aux[ix3(i, j, 1, my, na, mbc)], alpha1, fh);
             double* sm = &smax[ix3(i, j, 1, my + 1, 2, mbc)];
             *sm = Max(*sm, alpha);
             for (int m = 1; m <= meqn; m++) Fhat[ix3(i, j, m, my, meqn, mbc)] = fh[m - 1];
@@ -775,7 +781,7 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
             }
             double gh[MAXEQ];
             const double alpha = interface_flux(p, 2, qs, fs, &aux[ix3(i, j - 1, 1, my, na, mbc)],
-                                                &aux[ix3(i, j, 1, my, na, mbc)], NULL, gh);
+                                                &aux[ix3(i, j, 1, my, na, mbc)], alpha2, gh);
             double* sm = &smax[ix3(i, j, 2, my + 1, 2, mbc)];
             *sm = Max(*sm, alpha);
             for (int m = 1; m <= meqn; m++) Ghat[ix3(i, j, m, my + 1, meqn, mbc)] = gh[m - 1];
@@ -815,7 +821,7 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
         }
         double fh[MAXEQ];
         const double alpha = interface_flux(p, 1, qs, fs, &aux[ix2(i - 1, 1, na, mbc)],
-                                            &aux[ix2(i, 1, na, mbc)], NULL, fh);
+                                            &aux[ix2(i, 1, na, mbc)], 0.0, fh);
         smax[i + mbc - 1] = alpha;
         for (int m = 1; m <= meqn; m++) fhat[ix2(i, m, meqn, mbc)] = fh[m - 1];
     }
@@ -1405,7 +1411,7 @@ static int check_stepper(const finess_params* p, int ts)
 {
     if (ts != FINESS_TS_RK && p->space_order != 5) return FINESS_ERR_CONFIG;
     if (ts == FINESS_TS_LXW && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
-    if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4)) return FINESS_ERR_CONFIG;
+    if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4 || p->global_alpha)) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -1769,7 +1775,8 @@ static int construct_l_routed_mask(const finess_params* p, const double* q, cons
                                    long long* n_troubled)
 {
     const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
-    if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || check_config5(p)) return FINESS_ERR_CONFIG;
+    if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || check_config5(p) || p->global_alpha)
+        return FINESS_ERR_CONFIG;
     double* Fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn, sizeof(double));
     double* Ghat = (double*)calloc((size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn, sizeof(double));
     long long nt = 0;
@@ -1798,7 +1805,7 @@ static int construct_l_routed_mask(const finess_params* p, const double* q, cons
                 if (full) {
                     const int il = dir == 1 ? i - 1 : i, jl = dir == 1 ? j : j - 1;
                     alpha = interface_flux(p, dir, qs, fs, &aux[ix3(il, jl, 1, my, na, mbc)],
-                                           &aux[ix3(i, j, 1, my, na, mbc)], NULL, fh);
+                                           &aux[ix3(i, j, 1, my, na, mbc)], 0.0, fh);
                 } else {
                     alpha = interface_flux_linear(p, dir, qs, fs, fh);
                 }

@@ -41,8 +41,9 @@ def test_defaults_follow_reference_schema():
 
 
 @pytest.mark.parametrize("kw", [
-    dict(space_order=7),                # GetWenoReconstruct has no device path for it
-    dict(global_alpha=1),
+    dict(space_order=3),                # GetWenoReconstruct has no third order
+    dict(space_order=7),                # needs mbc >= 4 (lib/2d/ConstructL.cpp:31-33)
+    dict(global_alpha=1, routing=finess_amd.ROUTE_ORACLE),  # routing has no global-alpha path
     dict(max_cfl=0.3, desired_cfl=0.4),  # finess.py: max_cfl >= desired_cfl
     dict(initial_dt=1.0, max_dt=0.5),    # finess.py: max_dt >= initial_dt
     dict(alpha_scaling=0.9),             # weno.py: alpha_scaling >= 1

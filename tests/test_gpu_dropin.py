@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import orc
-from finess_amd import TS_PIF, problems
+from finess_amd import TS_PIF, WENO_Z, problems
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -105,6 +105,37 @@ def test_euler_quadrants_app_pif(tmp_path):
     err = orc.rel_linf(q1, orc.interior(p, q0))
     print("PIF app vs oracle", err)
     assert np.all(err <= 1e-10), err
+
+
+def test_euler_quadrants_app_wide_weno_global_alpha(tmp_path):
+    """The app with `space_order = 9`, `weno_version = Z`, `mbc = 5` and
+    `global_alpha = true` read from parameters.ini: the face-tile kernels with
+    the global wave speeds, frames against the oracle (1e-10 or ten times the
+    oracle's own spread under 1e-15 input noise: the ninth-order indicators
+    cancel by several digits)."""
+    ini = open(os.path.join(APPS, "euler_quadrants", "parameters.ini")).read()
+    ini = ini.replace("space_order = 5", "space_order = 9").replace("mbc   = 3", "mbc   = 5")
+    ini = ini.replace("weno_version  = JS", "weno_version  = Z")
+    ini = ini.replace("source_term = false", "source_term = false\nglobal_alpha = true")
+    (tmp_path / "parameters.ini").write_text(ini)
+    r = subprocess.run([os.path.join(APPS, "euler_quadrants", "finess.exe")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    kw = dict(mx=64, initial_dt=1e-3, max_dt=1.0, nv=500000, space_order=9, mbc=5, weno_version=WENO_Z,
+              global_alpha=1)
+    p, q, aux = problems.riemann4(**kw)
+    t0, q0 = oracle_frames(p, q.copy(), aux, 2, 0.02)
+    noise = np.random.default_rng(5)
+    spread = 0.0
+    for _ in range(2):
+        _, qn = oracle_frames(p, q * (1.0 + 1e-15 * (noise.random(q.shape) - 0.5)), aux, 2, 0.02)
+        spread = max(spread, np.max(orc.rel_linf(orc.interior(p, qn), orc.interior(p, q0))))
+    t1, q1 = read_frame(tmp_path / "output" / "q0002.dat", (64, 64), 5)
+    assert abs(t1 - t0) < 1e-15
+    err = orc.rel_linf(q1, orc.interior(p, q0))
+    print("wide app vs oracle", err, "oracle spread", spread)
+    assert np.all(err <= max(1e-10, 10.0 * spread)), err
 
 
 def test_kelvin_helmholtz_lxw_app(tmp_path):

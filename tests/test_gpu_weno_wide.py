@@ -79,6 +79,25 @@ def test_construct_l_order11_boundaries(bc):
     assert np.all(err <= tol), err
 
 
+@pytest.mark.parametrize("order,bc", [(11, BC_PERIODIC), (11, BC_REFLECT), (9, BC_EXTRAP), (7, BC_PERIODIC)])
+def test_construct_l_smallest_grid(order, bc):
+    """The smallest grid an order accepts (mx = max(r, 5), my = mx + 1; 2D
+    grids need 5 cells a side): at r = 5, 6 the stencils reach the boundary
+    map's whole range [-n, 2n-1]."""
+    r = (order + 1) // 2
+    n = max(r, 5)
+    p, q = _perturbed_riemann(n, n + 1, WENO_JS, order, bc=[bc] * 4)
+    orc.set_bnd(p, q)
+    L0, s0 = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    tol, spread = _l_tol(p, q, L0)
+    print("order", order, "bc", bc, "rel err", err, "oracle spread", spread)
+    assert np.all(err <= tol), err
+    assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
+
+
 @pytest.mark.parametrize("version,order", [(WENO_JS, 7), (WENO_Z, 9)])
 def test_construct_l_wide_shallow_water(version, order):
     p, q, _ = problems.sw_bump(nx=33, ny=28, space_order=order, weno_version=version, mbc=_mbc(order))

@@ -231,6 +231,21 @@ def shock_bubble_learned(ind, mx=4096, my=1024, steps=40, warmup=5):
     return out
 
 
+def plain_fp64_2048(steps=20, warmup=3):
+    """The plain FP64 form on the four-quadrant problem at 2048^2 (timed from
+    t = 0.15 like the headline): Mcell-updates/s and the fused stage kernel's
+    fraction of the HBM roofline (algorithmic bytes over its event-timed
+    average launch)."""
+    mx = 2048
+    el, rej, launch = timed_steps("plain", mx, steps, warmup)
+    cells = mx * mx
+    bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / STAGES_PER_STEP
+    return {"value": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 4),
+            "steps": steps, "steps_rejected": rej, "avg_stage_launch_us": round(launch * 1e6, 2),
+            "hbm_frac": round(bytes_per_launch / launch / 1e9 / HBM_PEAK_GBS, 4),
+            "workload": "2D Euler four-quadrant Riemann 2048x2048, char-WENO5-JS + LLF, SSP-RK3, plain FP64"}
+
+
 def copy_calibration(mib=256, reps=5):
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters of a profiled run to bytes."""
@@ -350,6 +365,7 @@ def main():
     if not args.no_batch:
         out["batched_shallow_water"] = batched_shallow_water()
     if not args.no_extra:
+        out["plain_fp64_2048"] = plain_fp64_2048()
         out["kelvin_helmholtz_lxw_2048"] = kelvin_helmholtz_lxw()
         out["shock_bubble_4096x1024"] = shock_bubble_learned(ind)
     if not args.no_cpu:

@@ -16,15 +16,6 @@
 
 namespace finess {
 
-__device__ __forceinline__ double wave_max_b(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 template <class P, typename R, int TI, int TJ, int STAGE>
 __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
 {
@@ -80,8 +71,8 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
         if (ax != ax) ax = __builtin_inf();
         if (ay != ay) ay = __builtin_inf();
-        ax = wave_max_b(ax);
-        ay = wave_max_b(ay);
+        ax = wave_max64(ax);
+        ay = wave_max64(ay);
         if ((t & 63) == 0) { red[0][t >> 6] = ax; red[1][t >> 6] = ay; }
         __syncthreads();
         if (t == 0) {

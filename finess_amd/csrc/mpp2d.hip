@@ -44,15 +44,6 @@ struct MppArgs {
     unsigned long long* smax;  // the step's CFL words (slot 0)
 };
 
-__device__ __forceinline__ double mpp_wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 // the state of cell (i, j) through the boundary map
 __device__ __forceinline__ void mpp_load(const MppArgs& a, int i, int j, double q[5])
 {
@@ -83,8 +74,8 @@ __global__ __launch_bounds__(256) void k_mpp_speed(MppArgs a)
         if (sx != sx) sx = __builtin_inf();
         if (sy != sy) sy = __builtin_inf();
     }
-    sx = mpp_wave_max(sx);
-    sy = mpp_wave_max(sy);
+    sx = wave_max64(sx);
+    sy = wave_max64(sy);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) { red[0][w] = sx; red[1][w] = sy; }
     __syncthreads();
@@ -103,8 +94,8 @@ __global__ __launch_bounds__(256) void k_mpp_speed(MppArgs a)
 __device__ __forceinline__ void mpp_global_speeds(const MppArgs& a, double& ax, double& ay)
 {
     const int lane = threadIdx.x & 63;
-    ax = mpp_wave_max(__longlong_as_double((long long)a.gmax[lane]));
-    ay = mpp_wave_max(__longlong_as_double((long long)a.gmax[CFL_SPREAD + lane]));
+    ax = wave_max64(__longlong_as_double((long long)a.gmax[lane]));
+    ay = wave_max64(__longlong_as_double((long long)a.gmax[CFL_SPREAD + lane]));
 }
 
 __global__ __launch_bounds__(256) void k_mpp_lf(MppArgs a)

@@ -9,25 +9,6 @@
 
 namespace finess {
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  Remapping the
-// launch index gives each XCD a contiguous band of tiles, so the halo rows
-// that neighbouring tiles share are found in that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int b, int n)
-{
-    constexpr int NXCD = 8;
-    if (n % NXCD) return b;
-    return (b % NXCD) * (n / NXCD) + b / NXCD;
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 // Reduces two per-thread maxima over the workgroup and folds them into
 // slot[b] and slot[CFL_SPREAD + b], b = block index mod CFL_SPREAD
 // (non-negative doubles compare like their bit patterns).
@@ -35,8 +16,8 @@ template <int NT>
 __device__ __forceinline__ void block_max2(double ax, double ay, unsigned long long* slot)
 {
     __shared__ double red[2][NT / 64];
-    ax = wave_max(ax);
-    ay = wave_max(ay);
+    ax = wave_max64(ax);
+    ay = wave_max64(ay);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) { red[0][wid] = ax; red[1][wid] = ay; }
     __syncthreads();
@@ -147,8 +128,8 @@ __global__ __launch_bounds__(64) void k_global_speed_final(StageArgs a)
     const int t = threadIdx.x;
     double vx = __longlong_as_double((long long)a.gspread[t]);
     double vy = __longlong_as_double((long long)a.gspread[CFL_SPREAD + t]);
-    vx = wave_max(vx);
-    vy = wave_max(vy);
+    vx = wave_max64(vx);
+    vy = wave_max64(vy);
     if (t == 0) {
         a.galpha[0] = vx;
         a.galpha[1] = vy;

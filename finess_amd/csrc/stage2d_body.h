@@ -56,6 +56,27 @@ __host__ __device__ __forceinline__ int bc_map(int bc_lo, int bc_hi, int g, int 
 // The host reduces the CFL_SPREAD words when it reads them.
 constexpr int CFL_SPREAD = 64;
 
+// Maximum over the 64 lanes of a wave (the wave-speed maxima: non-negative
+// doubles or +inf).
+__device__ __forceinline__ double wave_max64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remapping the
+// launch index gives each XCD a contiguous band of tiles, so the halo rows
+// that neighbouring tiles share are found in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    constexpr int NXCD = 8;
+    if (n % NXCD) return b;
+    return (b % NXCD) * (n / NXCD) + b / NXCD;
+}
+
 struct StageArgs {
     // state planes, field-major: q[m][i][j] at m*plane + i*ld + j (ld = my)
     const double* src;     // stencil operand

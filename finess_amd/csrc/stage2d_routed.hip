@@ -24,22 +24,6 @@
 
 namespace finess {
 
-__device__ __forceinline__ int xcd_remap_r(int b, int n)
-{
-    constexpr int NXCD = 8;
-    if (n % NXCD) return b;
-    return (b % NXCD) * (n / NXCD) + b / NXCD;
-}
-
-__device__ __forceinline__ double wave_max_r(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 template <class P, typename R, int TI, int TJ>
 struct RouteTile {
     static constexpr int NFX = (TI + 1) * TJ, NFY = TI * (TJ + 1), NF = NFX + NFY;
@@ -103,7 +87,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     static_assert(T::NFX <= NT && T::NFY <= NT, "one face per thread and direction");
     __shared__ T sm;
     if (stage_halted(a)) return;
-    const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -252,8 +236,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     if (a.smax) {
         if (ax != ax) ax = __builtin_inf();
         if (ay != ay) ay = __builtin_inf();
-        ax = wave_max_r(ax);
-        ay = wave_max_r(ay);
+        ax = wave_max64(ax);
+        ay = wave_max64(ay);
         if (lane == 0) { sm.red[0][w] = ax; sm.red[1][w] = ay; }
         __syncthreads();
         if (t == 0) {
@@ -285,7 +269,7 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
     __shared__ int cnt[4][4];  // per wave: recorded troubled x, y; troubled faces, faces
     __shared__ int rbase[2];
     if (stage_halted(a)) return;
-    const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -433,7 +417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     __shared__ R fy[M][TI * (TJ + 1)];
     __shared__ double red[2][NT / 64];
     if (stage_halted(a)) return;
-    const int tile = xcd_remap_r(blockIdx.x, gridDim.x);
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
     const int i0 = ti * TI, j0 = tj * TJ;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -494,8 +478,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     if (a.smax) {
         if (ax != ax) ax = __builtin_inf();
         if (ay != ay) ay = __builtin_inf();
-        ax = wave_max_r(ax);
-        ay = wave_max_r(ay);
+        ax = wave_max64(ax);
+        ay = wave_max64(ay);
         if (lane == 0) { red[0][w] = ax; red[1][w] = ay; }
         __syncthreads();
         if (t == 0) {

@@ -259,15 +259,6 @@ __device__ __forceinline__ double wide_face_flux(const S& sm, int base, int stri
     return alpha;
 }
 
-__device__ __forceinline__ double wave_max_w(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
 // Tiles of faces along each axis for direction DIR.
 __host__ __device__ __forceinline__ int wide_tiles_j(int dir, int my) { return (my + (dir == 1) + WT - 1) / WT; }
 __host__ __device__ __forceinline__ int wide_tiles_i(int dir, int mx) { return (mx + (dir == 0) + WT - 1) / WT; }
@@ -325,7 +316,7 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
     }
     if (!a.smax) return;
     __shared__ double red[4];
-    amax = wave_max_w(amax);
+    amax = wave_max64(amax);
     const int lane = t & 63, w = t >> 6;
     if (lane == 0) red[w] = amax;
     __syncthreads();

@@ -644,13 +644,14 @@ int finess_params_finalize(finess_params* p)
     const bool have = so == 1 || so == 5 || (p->weno_version == FINESS_WENO_JS && (so == 7 || so == 9 || so == 11)) ||
                       (p->weno_version != FINESS_WENO_JS && (so == 7 || so == 9));
     if (!have) return FINESS_ERR_CONFIG;
-    // orders other than 5 run the face-list kernels: 2D Runge-Kutta and
-    // ConstructL in FP64, mbc >= r (lib/2d/ConstructL.cpp:31-33), and every
+    // orders other than 5 run the face-tile kernels in 2D and the face
+    // kernel in 1D: Runge-Kutta and ConstructL in FP64, mbc >= r
+    // (lib/2d/ConstructL.cpp:31-33, lib/1d/ConstructL.cpp:32-34), and every
     // stencil within the boundary map's range
     if (so != 5) {
         const int r = (so + 1) / 2;
-        if (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->routing != FINESS_ROUTE_NONE ||
-            p->precision != FINESS_PREC_FP64 || p->mbc < r || p->mx < r || p->my < r)
+        if (p->time_stepping_method != FINESS_TS_RK || p->routing != FINESS_ROUTE_NONE ||
+            p->precision != FINESS_PREC_FP64 || p->mbc < r || p->mx < r || (p->ndims == 2 && p->my < r))
             return FINESS_ERR_CONFIG;
     }
     if (p->source_term) return FINESS_ERR_CONFIG;

@@ -17,7 +17,7 @@ __global__ __launch_bounds__(256) void k_faces1d(Stage1DArgs a)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     double alpha = 0.0;
-    if (f <= a.mx) alpha = face1d<WK>(f, a);
+    if (f <= a.mx) alpha = face1d_any<WK>(f, a);
     if (a.smax_slot) {
         if (alpha != alpha) alpha = __builtin_inf();
         // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
@@ -47,6 +47,14 @@ int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
     case WENO_JS5_POW: return launch_wk<WENO_JS5_POW>(a, st);
     case WENO_Z5: return launch_wk<WENO_Z5>(a, st);
     case WENO_FD5: return launch_wk<WENO_FD5>(a, st);
+    case WENO_JS7: return launch_wk<WENO_JS7>(a, st);
+    case WENO_JS9: return launch_wk<WENO_JS9>(a, st);
+    case WENO_JS11: return launch_wk<WENO_JS11>(a, st);
+    case WENO_Z7: return launch_wk<WENO_Z7>(a, st);
+    case WENO_Z9: return launch_wk<WENO_Z9>(a, st);
+    case WENO_FD7: return launch_wk<WENO_FD7>(a, st);
+    case WENO_FD9: return launch_wk<WENO_FD9>(a, st);
+    case WENO_O1: return launch_wk<WENO_O1>(a, st);
     default: return -1;
     }
 }

@@ -4,6 +4,7 @@
 #include "physics.h"
 #include "stage2d_body.h"
 #include "weno.h"
+#include "weno_wide.h"
 
 namespace finess {
 
@@ -55,6 +56,42 @@ __host__ __device__ __forceinline__ double face1d(int f, const Stage1DArgs& a)
     a.fhat[f] = weno5<WK>(gp, wp) + weno5<WK>(gm, wp);
     if (a.smax_faces) a.smax_faces[f] = alpha;
     return alpha;
+}
+
+// The other orders (lib/WenoReconstruct.cpp:39-73): a stencil of 2R cells,
+// f-R .. f+R-1, split as lib/1d/ConstructL.cpp:153-158 does for any ws;
+// R = 1 is the first-order value.
+template <int R, int VER>
+__host__ __device__ __forceinline__ double face1d_wide(int f, const Stage1DArgs& a)
+{
+    constexpr int WS = 2 * R - 1, NS = 2 * R;
+    double q[NS], fl[NS];
+    for (int s = 0; s < NS; s++) {
+        const long long k = idx1d(a, f - R + s);
+        q[s] = a.src[k];
+        fl[s] = q[s] * a.aux[k];  // FluxFunc: Q(i,1)*Aux(i,1)
+    }
+    const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
+    const double s1 = rmin(al, 0.5 * (al + ar));
+    const double s2 = rmax(ar, 0.5 * (al + ar));
+    const double alpha = rmax(fabs(s1), fabs(s2));
+    const double la = a.alpha_scaling * alpha;
+    double gp[WS], gm[WS];
+    for (int s = 0; s < WS; s++) {
+        gp[s] = 0.5 * (fl[s] + la * q[s]);
+        gm[s] = 0.5 * (fl[WS - s] - la * q[WS - s]);
+    }
+    if constexpr (R == 1) a.fhat[f] = gp[0] + gm[0];
+    else a.fhat[f] = weno_wide<R, VER>(gp, a.eps, a.power) + weno_wide<R, VER>(gm, a.eps, a.power);
+    if (a.smax_faces) a.smax_faces[f] = alpha;
+    return alpha;
+}
+
+template <int WK>
+__host__ __device__ __forceinline__ double face1d_any(int f, const Stage1DArgs& a)
+{
+    if constexpr (WK <= WENO_FD5) return face1d<WK>(f, a);
+    else return face1d_wide<wide_radius(WK), wide_version(WK)>(f, a);
 }
 
 __host__ __device__ __forceinline__ void update1d(int i, const Stage1DArgs& a)

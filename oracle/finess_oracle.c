@@ -653,14 +653,14 @@ void orc_set_bnd(const finess_params* p, double* q, double* aux)
 /* ====================================================================== */
 
 /* The configurations the oracle restates: every order GetWenoReconstruct
- * has for the 2D ConstructL (mbc >= r, lib/2d/ConstructL.cpp:31-33), order 5
- * for everything else (check_config5). */
+ * has for ConstructL and the Runge-Kutta stepper (mbc >= r,
+ * lib/2d/ConstructL.cpp:31-33, lib/1d/ConstructL.cpp:32-34), order 5 for
+ * everything else (check_config5). */
 static int check_config(const finess_params* p)
 {
     if (p->weno_version != FINESS_WENO_JS && p->weno_version != FINESS_WENO_FD && p->weno_version != FINESS_WENO_Z)
         return FINESS_ERR_CONFIG;
     if (!orc_weno_supported(p->weno_version, p->space_order)) return FINESS_ERR_CONFIG;
-    if (p->space_order != 5 && p->ndims != 2) return FINESS_ERR_CONFIG;
     if (p->mbc < 3 || p->mbc < (p->space_order + 1) / 2) return FINESS_ERR_CONFIG;
     if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
     if (p->source_term) return FINESS_ERR_CONFIG;
@@ -811,11 +811,12 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
     const double dx = p->dx;
     double* fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * meqn, sizeof(double));
     if (!fhat) return FINESS_ERR_ARG;
+    const int ns = p->space_order + 1, r = ns / 2;
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx + 1; i++) {
-        double qs[6][MAXEQ], fs[6][MAXEQ];
-        for (int s = 1; s <= 6; s++) {
-            const int is = i - 1 + s - 3;
+        double qs[ORC_NS][MAXEQ], fs[ORC_NS][MAXEQ];
+        for (int s = 1; s <= ns; s++) {
+            const int is = i - 1 + s - r;
             for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix2(is, m, meqn, mbc)];
             orc_flux(p, qs[s - 1], &aux[ix2(is, 1, na, mbc)], fs[s - 1], NULL);
         }

@@ -149,6 +149,46 @@ def test_fin_solve_rk_wide(version, order, time_order):
     assert np.all(err <= tol), err
 
 
+@pytest.mark.parametrize("version,order", CASES)
+def test_construct_l_wide_1d(version, order):
+    """1D advection (the face kernel of stage1d.hip with a 2r-cell stencil):
+    ConstructL on a rough periodic state."""
+    r = (order + 1) // 2
+    p, q, aux = problems.advection_1d(mx=53, space_order=order, weno_version=version, mbc=max(3, r))
+    q = q + 0.3 * RNG.random(q.shape)
+    orc.set_bnd(p, q, aux)
+    L0, _ = orc.construct_l(p, q, aux)
+    with Solver(p) as s:
+        L1, _ = s.construct_l(q, aux=aux)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("1d construct_l rel err", err)
+    assert np.all(err <= TOL), err
+
+
+@pytest.mark.parametrize("version,order", [(WENO_JS, 11), (WENO_Z, 9), (WENO_FD, 7)])
+def test_fin_solve_rk_wide_1d(version, order):
+    """The 1D advection run to t = 1 (500 steps at CFL 0.4) against the oracle
+    and against the exact shifted sine."""
+    r = (order + 1) // 2
+    p, q, aux = problems.advection_1d(mx=200, space_order=order, weno_version=version, mbc=max(3, r),
+                                      initial_dt=0.002)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 1.0, dtv0, aux=aux)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, 1.0, dtv1, aux=aux)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    x = problems.cell_centres(p)
+    l1 = np.mean(np.abs(orc.interior(p, q1)[:, 0] - orc.interior(p, np.sin(2 * np.pi * (x - 1.0)))))
+    print("steps", st0.steps_accepted, "rel err", err, "L1 vs exact", l1)
+    assert np.all(err <= TOL), err
+    assert l1 < 2e-6
+
+
 def test_wide_orders_rejected_where_the_device_has_no_path():
     """Orders other than 5 run the 2D Runge-Kutta / ConstructL path in FP64
     only; the rest is refused at configuration time, as are the pairs

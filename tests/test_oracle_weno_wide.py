@@ -249,3 +249,23 @@ def test_wide_indicators_are_ill_conditioned_over_steps():
 
     assert spread(11, 2) > 1e-10
     assert spread(5, 3) < 1e-13
+
+
+@pytest.mark.parametrize("version,order,rate_min", [(WENO_FD, 7, 6.5), (WENO_FD, 9, 8.5), (WENO_Z, 7, 6.5),
+                                                    (WENO_Z, 9, 8.5), (WENO_JS, 9, 8.5), (WENO_JS, 11, 9.0),
+                                                    (WENO_JS, 7, 5.0), (WENO_JS, 1, 0.9)])
+def test_construct_l_1d_order(version, order, rate_min):
+    """1D ConstructL on sin(2 pi x) advected at speed 1 (periodic): L
+    approximates -2 pi cos(2 pi x) at the design order on 20 -> 40 cells
+    (first order: 1)."""
+    r = (order + 1) // 2
+    errs = []
+    for n in (20, 40):
+        p, q, aux = problems.advection_1d(mx=n, space_order=order, weno_version=version, mbc=max(3, r))
+        orc.set_bnd(p, q, aux)
+        L, _ = orc.construct_l(p, q, aux)
+        x = problems.cell_centres(p)
+        exact = -2 * math.pi * np.cos(2 * math.pi * x)
+        errs.append(np.max(np.abs(orc.interior(p, L)[:, 0] - orc.interior(p, exact))))
+    rate = math.log2(errs[0] / errs[1])
+    assert rate > rate_min, (errs, rate)

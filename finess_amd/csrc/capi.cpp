@@ -375,6 +375,8 @@ int pif_step(finess_ctx* c, int A, int B, int C, double dt)
     L.eps = p.epsilon;
     L.power = p.power_param;
     L.gamma = p.gamma;
+    L.physics = p.physics;
+    L.sw_compat_hl = p.sw_compat_hl;
     L.F = c->lxwF;
     L.G = c->lxwG;
     L.fx = c->pif_fx;
@@ -681,7 +683,7 @@ int finess_params_finalize(finess_params* p)
         if (p->time_order < 2 || p->time_order > 3) return FINESS_ERR_CONFIG;
     } else if (p->time_stepping_method == FINESS_TS_PIF) {
         // RK4-PIF exists for time_order 4 only (lib/2d/FinSolveRK_PIF.cpp:151-206)
-        if (p->ndims != 2 || p->physics != FINESS_PHYS_EULER || p->time_order != 4) return FINESS_ERR_CONFIG;
+        if (p->ndims != 2 || p->physics == FINESS_PHYS_ADVECTION || p->time_order != 4) return FINESS_ERR_CONFIG;
         if (p->precision != FINESS_PREC_FP64) return FINESS_ERR_CONFIG;
     } else if (p->time_order < 1 || p->time_order > (p->ndims == 2 ? 5 : 3)) {
         return FINESS_ERR_CONFIG;

@@ -67,6 +67,7 @@ struct PifLaunch {
     double dx, dy, cdt, w;
     int first, last;
     double eps, power, gamma;
+    int physics, sw_compat_hl;
     double *F, *G, *fx, *fy;
 };
 int launch_pif_flux_sum(const PifLaunch& L, hipStream_t st);

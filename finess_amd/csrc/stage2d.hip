@@ -163,8 +163,10 @@ int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, h
                    : launch_stage<Euler2D, float, WENO_JS5_P2, false, 7, 32>(a, st);
     }
     if (physics == PHYS_SHALLOW_WATER) {
+        // the Lax-Wendroff form of the stage serves RK4-PIF's final stage
+        if (!fp32) return lxw ? dispatch_weno<ShallowWater2D, double, true>(wk, a, st)
+                              : dispatch_weno<ShallowWater2D, double, false>(wk, a, st);
         if (lxw) return -1;
-        if (!fp32) return dispatch_weno<ShallowWater2D, double, false>(wk, a, st);
         if (wk != WENO_JS5_P2) return -1;
         return launch_stage<ShallowWater2D, float, WENO_JS5_P2, false, 7, 32>(a, st);
     }

@@ -48,3 +48,25 @@ def test_pif_controlled_run():
     err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
     print("PIF controlled run: steps", st0.steps_accepted, "rel err", err)
     assert np.all(err <= 1e-10), err
+
+
+def test_pif_shallow_water():
+    """Shallow water (g = 1): fixed steps on the periodic bump with random
+    momenta, and a controlled run."""
+    p, q, _ = problems.sw_bump(nx=40, ny=36, time_order=4, time_stepping_method=TS_PIF)
+    rng = np.random.default_rng(17)
+    q[..., 1] = 0.1 * rng.random(q.shape[:-1])
+    q[..., 2] = 0.1 * rng.random(q.shape[:-1])
+    _fixed(p, q.copy(), 4e-3, 10)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.05, dtv0, pif=True)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk_pif(q1, 0.0, 0.05, dtv1)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("SW PIF controlled run: steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err

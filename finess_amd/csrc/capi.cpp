@@ -658,9 +658,10 @@ int finess_params_finalize(finess_params* p)
     }
     if (p->source_term) return FINESS_ERR_CONFIG;
     // global_alpha (GlobalWaveSpd before each operator call) runs in the 2D
-    // stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), ConstructL
-    if (p->global_alpha && (p->ndims != 2 || p->routing != FINESS_ROUTE_NONE ||
-                            p->time_stepping_method == FINESS_TS_PIF))
+    // stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), RK4-PIF's
+    // final stage (ConstructL_NOC computes the speeds but does not use them,
+    // lib/2d/FinSolveRK_PIF.cpp:759-764), ConstructL
+    if (p->global_alpha && (p->ndims != 2 || p->routing != FINESS_ROUTE_NONE))
         return FINESS_ERR_CONFIG;
     if (p->ndims == 1) {
         if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;

@@ -1412,7 +1412,7 @@ static int check_stepper(const finess_params* p, int ts)
 {
     if (ts != FINESS_TS_RK && p->space_order != 5) return FINESS_ERR_CONFIG;
     if (ts == FINESS_TS_LXW && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
-    if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4 || p->global_alpha)) return FINESS_ERR_CONFIG;
+    if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4)) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 

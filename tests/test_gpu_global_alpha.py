@@ -109,9 +109,26 @@ def test_limited_lxw_global_alpha():
     assert np.all(err <= 2e-7), err
 
 
+def test_pif_global_alpha():
+    """RK4-PIF with global_alpha: the final stage's characteristic WENO takes
+    Max(global speed of q^n, local) (lib/2d/FinSolveRK_PIF.cpp:348-353,476)."""
+    p, q, _ = problems.riemann4(mx=64, global_alpha=1, time_order=4, time_stepping_method=TS_PIF)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.01, dtv0, pif=True)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk_pif(q1, 0.0, 0.01, dtv1)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("PIF global alpha steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err
+
+
 def test_global_alpha_rejected_where_the_device_has_no_path():
-    for kw in (dict(routing=ROUTE_ORACLE), dict(time_stepping_method=TS_PIF, time_order=4)):
-        with pytest.raises(FinessError):
-            problems.riemann4(mx=32, global_alpha=1, **kw)
+    with pytest.raises(FinessError):
+        problems.riemann4(mx=32, global_alpha=1, routing=ROUTE_ORACLE)
     with pytest.raises(FinessError):
         problems.advection_1d(mx=50, global_alpha=1)

@@ -184,6 +184,7 @@ Stage1DArgs base_stage1d_args(finess_ctx* c)
     a.eps = p.epsilon;
     a.power = p.power_param;
     a.aux = c->aux;
+    if (p.global_alpha) a.galpha = c->galpha;
     a.fhat = c->fhat1d;
     return a;
 }
@@ -657,12 +658,11 @@ int finess_params_finalize(finess_params* p)
             return FINESS_ERR_CONFIG;
     }
     if (p->source_term) return FINESS_ERR_CONFIG;
-    // global_alpha (GlobalWaveSpd before each operator call) runs in the 2D
-    // stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), RK4-PIF's
+    // global_alpha (GlobalWaveSpd before each operator call) runs in the 1D
+    // and 2D stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), RK4-PIF's
     // final stage (ConstructL_NOC computes the speeds but does not use them,
     // lib/2d/FinSolveRK_PIF.cpp:759-764), ConstructL
-    if (p->global_alpha && (p->ndims != 2 || p->routing != FINESS_ROUTE_NONE))
-        return FINESS_ERR_CONFIG;
+    if (p->global_alpha && p->routing != FINESS_ROUTE_NONE) return FINESS_ERR_CONFIG;
     if (p->ndims == 1) {
         if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;
         if (p->time_stepping_method != FINESS_TS_RK) return FINESS_ERR_CONFIG;

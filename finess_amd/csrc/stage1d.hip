@@ -32,9 +32,30 @@ __global__ __launch_bounds__(256) void k_update1d(Stage1DArgs a)
     if (i < a.mx) update1d(i, a);
 }
 
+// GlobalWaveSpd (lib/1d/GlobalWaveSpd.cpp): max |aux(1)| over the interior
+// (the advection SetWaveSpd with both sides equal), one workgroup.
+__global__ __launch_bounds__(256) void k_global_speed1d(Stage1DArgs a)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < a.mx; i += 256) {
+        const double s = fabs(a.aux[idx1d(a, i)]);
+        v = s != s ? __builtin_inf() : (v > s ? v : s);
+    }
+    v = wave_max64(v);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < 4; k++) m = red[k] > m ? red[k] : m;
+        *a.galpha = m;
+    }
+}
+
 template <int WK>
 static int launch_wk(const Stage1DArgs& a, hipStream_t st)
 {
+    if (a.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_faces1d<WK>, dim3((a.mx + 1 + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_update1d, dim3((a.mx + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;

@@ -23,7 +23,16 @@ struct Stage1DArgs {
     unsigned long long* smax_slot;
     double* smax_faces;  // optional: alpha of each face f in [0, mx]
     double* Lout;        // optional: L of each cell
+    double* galpha;      // global_alpha (null: off): the global wave speed, formed before the faces
 };
+
+// SetWaveSpd of the advection callbacks with both sides equal (|a|) and the
+// Max(alpha1, local) of lib/1d/ConstructL.cpp:148 when global_alpha is set.
+__host__ __device__ __forceinline__ double alpha1d(const Stage1DArgs& a, double local)
+{
+    const double g = a.galpha ? *a.galpha : 0.0;
+    return g > local ? g : local;  // a NaN speed stays NaN
+}
 
 __host__ __device__ __forceinline__ long long idx1d(const Stage1DArgs& a, int g)
 {
@@ -46,7 +55,7 @@ __host__ __device__ __forceinline__ double face1d(int f, const Stage1DArgs& a)
     const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
     const double s1 = rmin(al, 0.5 * (al + ar));
     const double s2 = rmax(ar, 0.5 * (al + ar));
-    const double alpha = rmax(fabs(s1), fabs(s2));
+    const double alpha = alpha1d(a, rmax(fabs(s1), fabs(s2)));
     const double la = a.alpha_scaling * alpha;
     double gp[5], gm[5];
     for (int s = 0; s < 5; s++) {
@@ -74,7 +83,7 @@ __host__ __device__ __forceinline__ double face1d_wide(int f, const Stage1DArgs&
     const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
     const double s1 = rmin(al, 0.5 * (al + ar));
     const double s2 = rmax(ar, 0.5 * (al + ar));
-    const double alpha = rmax(fabs(s1), fabs(s2));
+    const double alpha = alpha1d(a, rmax(fabs(s1), fabs(s2)));
     const double la = a.alpha_scaling * alpha;
     double gp[WS], gm[WS];
     for (int s = 0; s < WS; s++) {

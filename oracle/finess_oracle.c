@@ -664,7 +664,6 @@ static int check_config(const finess_params* p)
     if (p->mbc < 3 || p->mbc < (p->space_order + 1) / 2) return FINESS_ERR_CONFIG;
     if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
     if (p->source_term) return FINESS_ERR_CONFIG;
-    if (p->global_alpha && p->ndims != 2) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -812,6 +811,16 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
     double* fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * meqn, sizeof(double));
     if (!fhat) return FINESS_ERR_ARG;
     const int ns = p->space_order + 1, r = ns / 2;
+    /* GlobalWaveSpd, lib/1d/GlobalWaveSpd.cpp: max over the interior of the
+     * SetWaveSpd speed with both sides equal to the cell (lib/1d/ConstructL.cpp:45-49) */
+    double alpha1 = 0.0;
+    if (p->global_alpha)
+        for (int i = 1; i <= mx; i++) {
+            double s1, s2;
+            orc_wave_speed(p, 1, &q[ix2(i, 1, meqn, mbc)], &q[ix2(i, 1, meqn, mbc)], &aux[ix2(i, 1, na, mbc)],
+                           &aux[ix2(i, 1, na, mbc)], &s1, &s2);
+            alpha1 = Max(alpha1, Max(fabs(s1), fabs(s2)));
+        }
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx + 1; i++) {
         double qs[ORC_NS][MAXEQ], fs[ORC_NS][MAXEQ];
@@ -822,7 +831,7 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
         }
         double fh[MAXEQ];
         const double alpha = interface_flux(p, 1, qs, fs, &aux[ix2(i - 1, 1, na, mbc)],
-                                            &aux[ix2(i, 1, na, mbc)], 0.0, fh);
+                                            &aux[ix2(i, 1, na, mbc)], alpha1, fh);
         smax[i + mbc - 1] = alpha;
         for (int m = 1; m <= meqn; m++) fhat[ix2(i, m, meqn, mbc)] = fh[m - 1];
     }

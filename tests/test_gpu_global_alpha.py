@@ -127,8 +127,44 @@ def test_pif_global_alpha():
     assert np.all(err <= 1e-10), err
 
 
+def _advection_varying_speed(mx, **kw):
+    """1D advection with a speed that varies in space (aux(1) = 1 + 0.5 sin),
+    so that the global speed differs from most local ones."""
+    p, q, aux = problems.advection_1d(mx=mx, global_alpha=1, **kw)
+    x = problems.cell_centres(p)
+    aux = np.ascontiguousarray((1.0 + 0.5 * np.sin(2 * np.pi * x))[:, None])
+    return p, q, aux
+
+
+def test_construct_l_1d_global_alpha():
+    p, q, aux = _advection_varying_speed(61)
+    q = q + 0.2 * RNG.random(q.shape)
+    orc.set_bnd(p, q, aux)
+    L0, s0 = orc.construct_l(p, q, aux)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q, aux=aux)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("1d global alpha rel err", err)
+    assert np.all(err <= 1e-12), err
+    assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
+
+
+def test_fin_solve_rk_1d_global_alpha():
+    p, q, aux = _advection_varying_speed(100)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.2, dtv0, aux=aux)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, 0.2, dtv1, aux=aux)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("1d global alpha steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err
+
+
 def test_global_alpha_rejected_where_the_device_has_no_path():
     with pytest.raises(FinessError):
         problems.riemann4(mx=32, global_alpha=1, routing=ROUTE_ORACLE)
-    with pytest.raises(FinessError):
-        problems.advection_1d(mx=50, global_alpha=1)

@@ -68,3 +68,22 @@ def test_uniform_speed_state_equals_local_and_telescopes():
     La, sa = orc.construct_l(p, qu, aux)
     Lb, sb = orc.construct_l(pg, qu, aux)
     assert np.array_equal(La, Lb) and np.array_equal(sa, sb)
+
+
+def test_1d_face_speeds_are_max_of_global_and_local():
+    """1D (lib/1d/ConstructL.cpp:45-49,148; lib/1d/GlobalWaveSpd.cpp): the
+    global speed of advection is max |a| over the interior, and every face
+    speed (smax, overwritten per face in 1D) is max(global, local)."""
+    p, q, aux = problems.advection_1d(mx=40)
+    x = problems.cell_centres(p)
+    aux = np.ascontiguousarray((1.0 + 0.5 * np.sin(2 * np.pi * x))[:, None])
+    orc.set_bnd(p, q, aux)
+    L0, s0 = orc.construct_l(p, q, aux)
+    pg = p.copy()
+    pg.global_alpha = 1
+    L1, s1 = orc.construct_l(pg, q, aux)
+    g = p.mbc
+    amax = np.max(np.abs(aux[g:g + p.mx, 0]))
+    faces = slice(g, g + p.mx + 1)
+    assert np.array_equal(s1[faces], np.maximum(s0[faces], amax))
+    assert np.max(np.abs(orc.interior(p, L1 - L0))) > 1e-8

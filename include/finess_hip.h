@@ -81,12 +81,18 @@ typedef struct finess_params {
     double dx, dy;                      /* derived: (xhigh-xlow)/mx, ... */
     /* [finess] (python/finess/params/finess.py:13-170) */
     int    meqn, maux;
-    int    space_order;                 /* only 5 */
-    int    time_order;                  /* RK: 1..3; LxW: 2..3 */
+    int    space_order;                 /* 1, 5, 7, 9, 11 (JS), 1, 5, 7, 9 (Z, FD), as
+                                           GetWenoReconstruct (lib/WenoReconstruct.cpp:39-73);
+                                           orders other than 5: Runge-Kutta and ConstructL in
+                                           FP64, mbc >= (space_order + 1) / 2 */
+    int    time_order;                  /* RK: 1..5 in 2D, 1..3 in 1D; LxW: 2..3; PIF: 4 */
     int    time_stepping_method;        /* FINESS_TS_* */
     double initial_dt, max_dt, desired_cfl, max_cfl;
     int    nv;
-    int    global_alpha;                /* only 0 (default) */
+    int    global_alpha;                /* 1: GlobalWaveSpd before every operator call and
+                                           alpha = Max(global, local) at every interface
+                                           (lib/2d/ConstructL.cpp:48-54,172,323); not with routing
+                                           or the batched mode */
     int    source_term;                 /* only 0 */
     int    verbosity;
     /* [weno] (python/finess/params/weno.py:15-51) */

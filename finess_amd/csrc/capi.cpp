@@ -680,7 +680,11 @@ int finess_params_finalize(finess_params* p)
     if (p->mbc > p->mx || (p->ndims == 2 && p->mbc > p->my)) return FINESS_ERR_CONFIG;
     if (p->time_stepping_method < FINESS_TS_RK || p->time_stepping_method > FINESS_TS_PIF) return FINESS_ERR_CONFIG;
     if (p->time_stepping_method == FINESS_TS_LXW) {
-        if (p->ndims != 2 || p->mbc < 5 || p->physics != FINESS_PHYS_EULER) return FINESS_ERR_CONFIG;
+        if (p->ndims != 2 || p->mbc < 5 || p->physics == FINESS_PHYS_ADVECTION) return FINESS_ERR_CONFIG;
+        // the limiter (ApplyMPPLimiter2D) and the FP32 stencil form of this
+        // step exist for Euler only
+        if (p->physics != FINESS_PHYS_EULER && (p->mpp_limiter || p->precision != FINESS_PREC_FP64))
+            return FINESS_ERR_CONFIG;
         if (p->time_order < 2 || p->time_order > 3) return FINESS_ERR_CONFIG;
     } else if (p->time_stepping_method == FINESS_TS_PIF) {
         // RK4-PIF exists for time_order 4 only (lib/2d/FinSolveRK_PIF.cpp:151-206)
@@ -1058,7 +1062,8 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     (void)aux;
     if (!c || !q || !F || !G) return FINESS_ERR_ARG;
     const finess_params& p = c->p;
-    if (p.ndims != 2 || p.mbc < 5 || p.physics != FINESS_PHYS_EULER) return fail(c, FINESS_ERR_CONFIG, "LxW needs 2D Euler, mbc>=5");
+    if (p.ndims != 2 || p.mbc < 5 || p.physics == FINESS_PHYS_ADVECTION)
+        return fail(c, FINESS_ERR_CONFIG, "LxW needs 2D Euler or shallow water, mbc>=5");
     int rc;
     if ((rc = upload_ghosted(c, q))) return rc;
     double *Fd = c->lxwF, *Gd = c->lxwG;

@@ -10,7 +10,7 @@ namespace finess {
 template <class P, int TI, int TJ>
 __global__ __launch_bounds__(TI* TJ) void k_lxw_flux(LxwArgs a, int ntj)
 {
-    __shared__ LxwSmem<TI, TJ> sm;
+    __shared__ LxwSmem<TI, TJ, P::MEQN> sm;
     const int ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
     const int e0i = ti * TI, e0j = tj * TJ;
     lxw_phase_load<P, TI, TJ>(threadIdx.x, TI * TJ, e0i, e0j, a, sm);
@@ -21,9 +21,13 @@ __global__ __launch_bounds__(TI* TJ) void k_lxw_flux(LxwArgs a, int ntj)
 int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st)
 {
     constexpr int TI = 8, TJ = 32;
-    if (physics != PHYS_EULER) return -1;
     const int nti = (a.mx + 6 + TI - 1) / TI, ntj = (a.my + 6 + TJ - 1) / TJ;
-    hipLaunchKernelGGL((k_lxw_flux<Euler2D, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    if (physics == PHYS_EULER)
+        hipLaunchKernelGGL((k_lxw_flux<Euler2D, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    else if (physics == PHYS_SHALLOW_WATER)
+        hipLaunchKernelGGL((k_lxw_flux<ShallowWater2D, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    else
+        return -1;
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

@@ -57,6 +57,18 @@ __host__ __device__ __forceinline__ void euler_flux(const T q[5], T gm1, T F[5],
     G[0] = ru2; G[1] = ru2 * u1; G[2] = ru2 * u2 + press; G[3] = ru2 * u3; G[4] = u2 * ep;
 }
 
+// Shallow water, g = 1 (apps/2d/shallow_water/lib/FluxFunc.cpp:8-35).
+template <typename T>
+__host__ __device__ __forceinline__ void sw_flux(const T q[3], T F[3], T G[3])
+{
+    const T h = q[0];
+    const T ih = hinv(h);
+    const T u1 = q[1] * ih, u2 = q[2] * ih;
+    const T hh = typename T::scalar(0.5) * (h * h);
+    F[0] = h * u1; F[1] = h * u1 * u1 + hh; F[2] = h * u1 * u2;
+    G[0] = h * u2; G[1] = h * u2 * u1; G[2] = h * u2 * u2 + hh;
+}
+
 struct HDd : HD<double> {
     using scalar = double;
     __host__ __device__ HDd() {}
@@ -94,6 +106,28 @@ struct LxwPhys<Euler2D> {
     }
 };
 
+// Shallow water: DFluxFunc / D2FluxFunc (apps/2d/shallow_water/lib/
+// DFluxFunc.cpp, D2FluxFunc.cpp) are the exact derivatives of the flux, so
+// the hyper-dual evaluation reproduces them.
+template <>
+struct LxwPhys<ShallowWater2D> {
+    static constexpr int M = 3;
+    __host__ __device__ __forceinline__ static void flux(const double q[3], const PhysParams&, double F[3], double G[3])
+    {
+        const double h = q[0];
+        const double u1 = q[1] / h, u2 = q[2] / h;
+        F[0] = h * u1; F[1] = h * u1 * u1 + 0.5 * h * h; F[2] = h * u1 * u2;
+        G[0] = h * u2; G[1] = h * u2 * u1; G[2] = h * u2 * u2 + 0.5 * h * h;
+    }
+    __host__ __device__ __forceinline__ static void flux_hd(const double q[3], const double a[3], const double b[3],
+                                                            const PhysParams&, HDd F[3], HDd G[3])
+    {
+        HDd x[3];
+        for (int m = 0; m < 3; m++) x[m] = HDd(q[m], a[m], b[m], 0.0);
+        sw_flux<HDd>(x, F, G);
+    }
+};
+
 struct LxwArgs {
     const double* q;     // state planes (mx*my), boundary-mapped on load
     double* F;           // out: planes (mx+6)*(my+6), extended index (i+3)*(my+6)+(j+3)
@@ -107,12 +141,12 @@ struct LxwArgs {
     int src_ghosts;      // >0: q planes include that many ghosts per side (drop-in mode)
 };
 
-template <int TI, int TJ>
+template <int TI, int TJ, int M>
 struct LxwSmem {
     static constexpr int HI = TI + 4, HJ = TJ + 4, NC = HI * HJ;
-    double q[5][NC];
-    double f[5][NC];
-    double g[5][NC];
+    double q[M][NC];
+    double f[M][NC];
+    double g[M][NC];
 };
 
 template <class P>
@@ -139,9 +173,9 @@ __host__ __device__ __forceinline__ void lxw_load_state(const LxwArgs& a, int gi
 // included: the cross derivatives read the diagonal neighbours).
 template <class P, int TI, int TJ>
 __host__ __device__ __forceinline__ void lxw_phase_load(int tid, int nthreads, int e0i, int e0j, const LxwArgs& a,
-                                                        LxwSmem<TI, TJ>& sm)
+                                                        LxwSmem<TI, TJ, P::MEQN>& sm)
 {
-    using S = LxwSmem<TI, TJ>;
+    using S = LxwSmem<TI, TJ, P::MEQN>;
     for (int c = tid; c < S::NC; c += nthreads) {
         const int ci = c / S::HJ, cj = c - ci * S::HJ;
         const int gi = e0i + ci - 2 - 3, gj = e0j + cj - 2 - 3;
@@ -170,9 +204,9 @@ __host__ __device__ __forceinline__ double d2(double h, double f0, double f1, do
 // Phase 2: one output cell per thread.
 template <class P, int TI, int TJ>
 __host__ __device__ __forceinline__ void lxw_phase_cell(int tid, int e0i, int e0j, const LxwArgs& a,
-                                                        const LxwSmem<TI, TJ>& sm)
+                                                        const LxwSmem<TI, TJ, P::MEQN>& sm)
 {
-    using S = LxwSmem<TI, TJ>;
+    using S = LxwSmem<TI, TJ, P::MEQN>;
     constexpr int M = P::MEQN;
     const int ci = tid / TJ, cj = tid - ci * TJ;
     const int ei = e0i + ci, ej = e0j + cj;              // extended index, gi = ei - 3

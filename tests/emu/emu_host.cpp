@@ -150,7 +150,7 @@ int emu_construct_integrated_r(const finess_params* p, double dt, const double* 
     a.pp.gamma = p->gamma;
     for (int k = 0; k < 4; k++) a.bc[k] = p->bc[k];
     const int nti = (p->mx + 6 + LTI - 1) / LTI, ntj = (p->my + 6 + LTJ - 1) / LTJ;
-    std::vector<LxwSmem<LTI, LTJ>> smv(1);
+    std::vector<LxwSmem<LTI, LTJ, 5>> smv(1);
     for (int t = 0; t < nti * ntj; t++) {
         const int e0i = (t / ntj) * LTI, e0j = (t % ntj) * LTJ;
         for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_load<Euler2D, LTI, LTJ>(tid, LTI * LTJ, e0i, e0j, a, smv[0]);
@@ -213,7 +213,7 @@ int emu_run_fixed_steps(const finess_params* p, double* q, double dt, int nsteps
             for (int k = 0; k < 4; k++) la.bc[k] = p->bc[k];
             la.time_order = p->time_order; la.dt = dt; la.dx = p->dx; la.dy = p->dy; la.pp.gamma = p->gamma;
             const int nti = (p->mx + 6 + LTI - 1) / LTI, ntj = (p->my + 6 + LTJ - 1) / LTJ;
-            std::vector<LxwSmem<LTI, LTJ>> smv(1);
+            std::vector<LxwSmem<LTI, LTJ, 5>> smv(1);
             for (int t = 0; t < nti * ntj; t++) {
                 const int e0i = (t / ntj) * LTI, e0j = (t % ntj) * LTJ;
                 for (int tid = 0; tid < LTI * LTJ; tid++) lxw_phase_load<Euler2D, LTI, LTJ>(tid, LTI * LTJ, e0i, e0j, la, smv[0]);

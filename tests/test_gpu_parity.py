@@ -72,6 +72,46 @@ def test_construct_l_fp32_stencil():
     assert np.all(err <= 1e-5), err
 
 
+def test_construct_integrated_r_shallow_water():
+    """ConstructIntegratedR with the shallow-water flux: the hyper-dual
+    directional derivatives against the oracle's contraction of the explicit
+    Jacobian and Hessian (apps/2d/shallow_water/lib/DFluxFunc.cpp,
+    D2FluxFunc.cpp)."""
+    from finess_amd import TS_LXW
+    p, q, _ = problems.sw_bump(nx=40, ny=36, mbc=5, time_stepping_method=TS_LXW, time_order=3)
+    q[..., 1] = 0.1 * RNG.random(q.shape[:-1])
+    q[..., 2] = 0.1 * RNG.random(q.shape[:-1])
+    orc.set_bnd(p, q)
+    dt = 0.004
+    F0, G0 = orc.integrated_r(p, dt, q)
+    with Solver(p) as s:
+        F1, G1 = s.construct_integrated_r(dt, q)
+    g = p.mbc
+    sl = (slice(g - 3, g + p.mx + 3), slice(g - 3, g + p.my + 3))  # the extended grid the stage reads
+    for a, b in ((F1, F0), (G1, G0)):
+        err = orc.rel_linf(a[sl], b[sl])
+        print("sw integrated R rel err", err)
+        assert np.all(err <= 1e-12), err
+
+
+def test_fin_solve_lxw_shallow_water():
+    from finess_amd import TS_LXW
+    p, q, _ = problems.sw_bump(nx=48, ny=40, mbc=5, time_stepping_method=TS_LXW, time_order=3)
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 0.05, dtv0, lxw=True)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_lxw(q1, 0.0, 0.05, dtv1)
+    assert abs(t1 - t0) <= 1e-14
+    assert abs(dtv1[1] - dtv0[1]) <= 1e-12 * dtv0[1]
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("sw lxw steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err
+
+
 def test_construct_integrated_r():
     p, q, _ = problems.kelvin_helmholtz(mx=48, my=40)
     q = _perturb(q, 0.05)

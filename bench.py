@@ -246,6 +246,28 @@ def plain_fp64_2048(steps=20, warmup=3):
             "workload": "2D Euler four-quadrant Riemann 2048x2048, char-WENO5-JS + LLF, SSP-RK3, plain FP64"}
 
 
+def advection_1d_full(mx=200):
+    """1D advection of sin(2 pi x), 200 cells, WENO5 + SSP-RK3 at CFL 0.4 to
+    t = 1 (500 steps, the controller on the host): Mcell-updates/s and the L1
+    error against the exact shifted solution."""
+    from finess_amd import Solver, problems
+    p, q, aux = problems.advection_1d(mx=mx, initial_dt=0.4 / mx)
+    with Solver(p) as s:
+        dtv = np.array([0.0, p.initial_dt, p.max_dt])
+        qw = q.copy()
+        s.fin_solve_rk(qw, 0.0, 0.05, dtv.copy(), aux=aux)  # warm-up (first launches)
+        t0 = time.perf_counter()
+        t = s.fin_solve_rk(q, 0.0, 1.0, dtv, aux=aux)
+        el = time.perf_counter() - t0
+    g = p.mbc
+    x = p.xlow + (np.arange(mx) + 0.5) * p.dx
+    l1 = float(np.mean(np.abs(q[g:g + mx, 0] - np.sin(2 * np.pi * (x - t)))))
+    steps = int(round(1.0 / (0.4 / mx)))
+    return {"value": round(mx * steps / el / 1e6, 2), "unit": "Mcell-updates/s", "seconds": round(el, 4),
+            "steps": steps, "l1_error_vs_exact": l1,
+            "workload": "1D advection sin(2 pi x), 200 cells, WENO5-JS + LLF, SSP-RK3, CFL 0.4, t = 1, FP64"}
+
+
 def copy_calibration(mib=256, reps=5):
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters of a profiled run to bytes."""
@@ -366,6 +388,7 @@ def main():
         out["batched_shallow_water"] = batched_shallow_water()
     if not args.no_extra:
         out["plain_fp64_2048"] = plain_fp64_2048()
+        out["advection_1d_200"] = advection_1d_full()
         out["kelvin_helmholtz_lxw_2048"] = kelvin_helmholtz_lxw()
         out["shock_bubble_4096x1024"] = shock_bubble_learned(ind)
     if not args.no_cpu:

@@ -62,6 +62,31 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
     return r;
 }
 
+// ReLU of a packed BF16 pair as a signed 16-bit max with zero: a negative
+// value (sign bit set) becomes +0, others are unchanged, so it equals
+// rounding max(x, 0) for every finite x (-0 gives +0 instead of -0).
+__device__ __forceinline__ uint32_t relu_bf16_pair(uint32_t u)
+{
+    typedef short s16x2 __attribute__((ext_vector_type(2)));
+    s16x2 v;
+    __builtin_memcpy(&v, &u, 4);
+    const s16x2 z = {0, 0};
+    v = __builtin_elementwise_max(v, z);
+    uint32_t r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+
+// max(., 0) of four values, rounded to BF16 and packed (the hidden layers'
+// activation ahead of the next product).
+__device__ __forceinline__ bf16x4 pack4_relu(float a, float b, float c, float d)
+{
+    const uint32_t u[2] = {relu_bf16_pair(bf16_pair(a, b)), relu_bf16_pair(bf16_pair(c, d))};
+    bf16x4 r;
+    __builtin_memcpy(&r, u, 8);
+    return r;
+}
+
 // Weight fragments of the transposed network for the in-kernel inference,
 // prepared on the host from the FP32 parameter vector (finess_mlp layout):
 //   w1[mb][kb][lane][4]  W1^T rows 16mb.., cols (inputs) 16kb..  (2 x 2)

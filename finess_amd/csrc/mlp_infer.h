@@ -51,19 +51,19 @@ __device__ __forceinline__ float mlp_wave_logit(const float f[MLP_IN], const Mlp
             f32x4 acc = b1[mb];
             for (int kb = 0; kb < 2; kb++)
                 acc = mfma16(*reinterpret_cast<const bf16x4*>(fr.w1[mb][kb][lane]), xb[kb], acc);
-            for (int v = 0; v < 4; v++) h1[mb][nb][v] = fmaxf(acc[v], 0.f);
+            h1[mb][nb] = acc;  // the ReLU is applied when packing for layer 2
         }
     }
     // layer 2: H2^T = W2^T H1^T
     f32x4 h2[2][4];
     for (int nb = 0; nb < 4; nb++) {
         bf16x4 hb[2];
-        for (int kb = 0; kb < 2; kb++) hb[kb] = pack4(h1[kb][nb][0], h1[kb][nb][1], h1[kb][nb][2], h1[kb][nb][3]);
+        for (int kb = 0; kb < 2; kb++) hb[kb] = pack4_relu(h1[kb][nb][0], h1[kb][nb][1], h1[kb][nb][2], h1[kb][nb][3]);
         for (int mb = 0; mb < 2; mb++) {
             f32x4 acc = b2[mb];
             for (int kb = 0; kb < 2; kb++)
                 acc = mfma16(*reinterpret_cast<const bf16x4*>(fr.w2[mb][kb][lane]), hb[kb], acc);
-            for (int v = 0; v < 4; v++) h2[mb][nb][v] = fmaxf(acc[v], 0.f);
+            h2[mb][nb] = acc;
         }
     }
     // layer 3: z^T (row 0 of a 16-row tile) = w3^T H2^T
@@ -71,7 +71,7 @@ __device__ __forceinline__ float mlp_wave_logit(const float f[MLP_IN], const Mlp
         f32x4 acc = {fr.b3, 0.f, 0.f, 0.f};
         for (int kb = 0; kb < 2; kb++)
             acc = mfma16(*reinterpret_cast<const bf16x4*>(fr.w3[kb][lane]),
-                         pack4(h2[kb][nb][0], h2[kb][nb][1], h2[kb][nb][2], h2[kb][nb][3]), acc);
+                         pack4_relu(h2[kb][nb][0], h2[kb][nb][1], h2[kb][nb][2], h2[kb][nb][3]), acc);
         if (q == 0) lo[16 * nb + r] = acc[0];
     }
     wave_lds_sync();

@@ -48,8 +48,9 @@ struct Euler2D {
     // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Always formed in FP64 (the
     // pressure recovery keeps FP64 in the FP32 form as well).
     // LP: the record is rounded to FP32 afterwards, so 1/rho comes from the
-    // refined reciprocal estimate (within an FP64 ulp) and c from the FP32
-    // square root of the FP64 c^2.
+    // refined reciprocal estimate (within an FP64 ulp), the pressure from the
+    // momenta directly (still in FP64) and c from the FP32 square root of the
+    // FP64 c^2.
     template <bool LP = false>
     __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[8])
     {
@@ -57,7 +58,9 @@ struct Euler2D {
         const double rho = q[0];
         const double irho = LP ? rcp_nr(rho) : 1.0 / rho;
         const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
-        const double press = (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+        // LP: the kinetic energy from the momenta, |m|^2 / (2 rho), without the velocities
+        const double press = LP ? (gamma - 1.0) * (q[4] - 0.5 * ((q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * irho))
+                                : (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
         v[5] = irho; v[6] = press;
         const double c2 = fabs(gamma * press * irho);

@@ -463,8 +463,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
             const double idx_ = a.idx, idy_ = a.idy;
             const double bdt = stage_bdt(a);
             for (int m = 0; m < M; m++) {
-                const double Lx = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
-                const double L = Lx - (double(fy[m][ci * (TJ + 1) + cj + 1]) - double(fy[m][ci * (TJ + 1) + cj])) * idy_;
+                double L;
+                if constexpr (sizeof(R) == 4) {
+                    // FP32 form: the flux differences in FP32 (the fluxes are FP32
+                    // values), the stage accumulation below in FP64
+                    const R lx = (sm.face[m][(ci + 1) * TJ + cj] - sm.face[m][ci * TJ + cj]) * R(idx_);
+                    const R ly = (fy[m][ci * (TJ + 1) + cj + 1] - fy[m][ci * (TJ + 1) + cj]) * R(idy_);
+                    L = -double(lx + ly);
+                } else {
+                    const double Lx = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
+                    L = Lx - (double(fy[m][ci * (TJ + 1) + cj + 1]) - double(fy[m][ci * (TJ + 1) + cj])) * idy_;
+                }
                 const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : pa[m];
                 double v = a.a1 * qa;
                 if (a.use_b) {

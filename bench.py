@@ -268,6 +268,29 @@ def advection_1d_full(mx=200):
             "workload": "1D advection sin(2 pi x), 200 cells, WENO5-JS + LLF, SSP-RK3, CFL 0.4, t = 1, FP64"}
 
 
+def wide_order(order, version="js", mx=1024, steps=20, warmup=3):
+    """The plain FP64 form with another WENO order (the face-tile kernels of
+    stage2d_wide.hip) on the four-quadrant problem: Mcell-updates/s over fixed
+    steps from t = 0 and the stage's event-timed average."""
+    from finess_amd import WENO_FD, WENO_JS, WENO_Z, Solver, problems
+    ver = {"js": WENO_JS, "z": WENO_Z, "fd": WENO_FD}[version]
+    p, q, _ = problems.riemann4(mx=mx, space_order=order, weno_version=ver, mbc=max(3, (order + 1) // 2))
+    with Solver(p) as s:
+        s.set_state(q)
+        dt = float(p.initial_dt)
+        s.run_fixed_steps(dt, warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run_fixed_steps(dt, steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        ms, launches = s.last_kernel_time()
+    return {"workload": "2D Euler four-quadrant Riemann %dx%d, WENO-%s%d + LLF, SSP-RK3, plain FP64, fixed dt"
+                        % (mx, mx, version.upper(), order),
+            "value": round(mx * mx * steps / el / 1e6, 1), "unit": "Mcell-updates/s", "steps": steps,
+            "avg_stage_us": round(ms / launches * 1e3, 2)}
+
+
 def copy_calibration(mib=256, reps=5):
     """A plain device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters of a profiled run to bytes."""
@@ -298,11 +321,17 @@ def main():
                     help="steps between indicator evaluations of the learned form")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
+    ap.add_argument("--wide", type=int, default=0, metavar="ORDER",
+                    help="only time the plain FP64 form with WENO order 1, 7, 9 or 11 (face-tile kernels)")
+    ap.add_argument("--wide-version", default="js", choices=["js", "z", "fd"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.wide:
+        print(json.dumps(wide_order(args.wide, args.wide_version)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "

@@ -109,7 +109,9 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
     using S = WideSmem<P, R, DIR>;
     __shared__ S sm;
     const int ntj = wide_tiles_j(DIR, a.my);
-    const int ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
+    // neighbouring tiles on one XCD: the halo lines they share stay in its L2
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int ti = tile / ntj, tj = tile - ti * ntj;
     const int i0 = ti * WT, j0 = tj * WT;
     const int t = threadIdx.x;
     // LDS cell (ci, cj) is cell (i0 + ci - R, j0 + cj) for x faces and

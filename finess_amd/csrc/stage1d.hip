@@ -4,8 +4,9 @@
 // UpdateSoln (lib/1d/UpdateSoln.cpp:17-54) and the periodic halo
 // (apps/1d/advection/smooth_example/SetBndValues.cpp:6-63).
 //
-// 1D grids are small (the advection case has 200 cells), so the stage is two
-// plain launches: one thread per face, then one thread per cell.
+// 1D grids are small (the advection case has 200 cells): up to SMALL1D cells
+// the stage is one workgroup (faces, barrier, cells), beyond it two plain
+// launches, one thread per face, then one thread per cell.
 #include <hip/hip_runtime.h>
 #include "kernels.h"
 #include "stage1d_body.h"
@@ -52,10 +53,45 @@ __global__ __launch_bounds__(256) void k_global_speed1d(Stage1DArgs a)
     }
 }
 
+// Grids up to SMALL1D cells run the whole stage in one workgroup: the faces,
+// a workgroup barrier (which orders the face fluxes written to global memory
+// before they are read), then the cells.  One launch instead of two.
+constexpr int SMALL1D = 8192;
+
+constexpr int SMALL1D_NT = 512;  // 2 waves per SIMD: the wide orders fit without spilling
+
+template <int WK>
+__global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
+{
+    double amax = 0.0;
+    for (int f = threadIdx.x; f <= a.mx; f += SMALL1D_NT) {
+        double alpha = face1d_any<WK>(f, a);
+        if (alpha != alpha) alpha = __builtin_inf();
+        // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
+        if (f < a.mx) amax = alpha > amax ? alpha : amax;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.mx; i += SMALL1D_NT) update1d(i, a);
+    if (!a.smax_slot) return;
+    __shared__ double red[SMALL1D_NT / 64];
+    amax = wave_max64(amax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < SMALL1D_NT / 64; k++) m = red[k] > m ? red[k] : m;
+        atomicMax(a.smax_slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 template <int WK>
 static int launch_wk(const Stage1DArgs& a, hipStream_t st)
 {
     if (a.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, a);
+    if (a.mx <= SMALL1D) {
+        hipLaunchKernelGGL(k_stage1d_small<WK>, dim3(1), dim3(SMALL1D_NT), 0, st, a);
+        return hipGetLastError() == hipSuccess ? 0 : -4;
+    }
     hipLaunchKernelGGL(k_faces1d<WK>, dim3((a.mx + 1 + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_update1d, dim3((a.mx + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;

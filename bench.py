@@ -248,7 +248,7 @@ def plain_fp64_2048(steps=20, warmup=3):
 
 def advection_1d_full(mx=200):
     """1D advection of sin(2 pi x), 200 cells, WENO5 + SSP-RK3 at CFL 0.4 to
-    t = 1 (500 steps, the controller on the host): Mcell-updates/s and the L1
+    t = 1 (500 steps, the controller on the device): Mcell-updates/s and the L1
     error against the exact shifted solution."""
     from finess_amd import Solver, problems
     p, q, aux = problems.advection_1d(mx=mx, initial_dt=0.4 / mx)

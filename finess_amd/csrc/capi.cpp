@@ -77,7 +77,7 @@ struct finess_ctx {
     double* slog = nullptr;      // device records (t, sums) of one chunk
     double* spart = nullptr;     // partial sums
     std::vector<double> hlog;    // records not yet read by the caller
-    StepCtl* ctl = nullptr;                       // device controller block (2D)
+    StepCtl* ctl = nullptr;                       // device controller block
     bool devctl = false;                          // the device controller drives the current run
     bool route_fresh = true;                      // next routed stage starts a step
     int rec_age = -1;                             // steps since the record was decided (-1: none)
@@ -253,6 +253,11 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.a2 = a2;
         a.bdt = beta * dt;
         a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
+        if (c->devctl) {
+            a.dtp = &c->ctl->dt;
+            a.beta = beta;
+            a.halt = &c->ctl->halt;
+        }
         if (tmark(c)) return FINESS_ERR_HIP;
         const int rc = launch_stage1d(weno_kind(p), a, c->st);
         if (tmark(c)) return FINESS_ERR_HIP;
@@ -782,7 +787,7 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         if (!rc) rc = alloc(c, (void**)&c->wide_fy, (size_t)p.mx * (p.my + 1) * c->M * sizeof(double));
     }
     if (!rc) rc = alloc(c, (void**)&c->route_counts, 2 * CFL_SPREAD * sizeof(unsigned long long));
-    if (!rc && p.ndims == 2) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
+    if (!rc) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
     if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2) {
         const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
         rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
@@ -1164,6 +1169,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
     cp.cfl_target = p.desired_cfl;
     cp.dx = p.dx;
     cp.dy = p.dy;
+    cp.one_d = p.ndims == 1;
     cp.nv = p.nv;
     cp.target = by_steps ? fixed_steps : 0;
     int rc = 0;
@@ -1279,7 +1285,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
 
 static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
 {
-    if (c && dtv && c->have_state && c->ctl && c->p.ndims == 2 && c->p.time_stepping_method == FINESS_TS_RK &&
+    if (c && dtv && c->have_state && c->ctl && c->p.time_stepping_method == FINESS_TS_RK &&
         c->p.time_order <= 3 && !c->p.verbosity && !c->p.host_controller)
         return advance_dev(c, tend, dtv, fixed_steps, stats);
     if (!c || !dtv) return FINESS_ERR_ARG;

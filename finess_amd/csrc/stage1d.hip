@@ -16,6 +16,7 @@ namespace finess {
 template <int WK>
 __global__ __launch_bounds__(256) void k_faces1d(Stage1DArgs a)
 {
+    if (halted1d(a)) return;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     double alpha = 0.0;
     if (f <= a.mx) alpha = face1d_any<WK>(f, a);
@@ -29,6 +30,7 @@ __global__ __launch_bounds__(256) void k_faces1d(Stage1DArgs a)
 
 __global__ __launch_bounds__(256) void k_update1d(Stage1DArgs a)
 {
+    if (halted1d(a)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.mx) update1d(i, a);
 }
@@ -37,6 +39,7 @@ __global__ __launch_bounds__(256) void k_update1d(Stage1DArgs a)
 // (the advection SetWaveSpd with both sides equal), one workgroup.
 __global__ __launch_bounds__(256) void k_global_speed1d(Stage1DArgs a)
 {
+    if (halted1d(a)) return;
     double v = 0.0;
     for (int i = threadIdx.x; i < a.mx; i += 256) {
         const double s = fabs(a.aux[idx1d(a, i)]);
@@ -63,6 +66,7 @@ constexpr int SMALL1D_NT = 512;  // 2 waves per SIMD: the wide orders fit withou
 template <int WK>
 __global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
 {
+    if (halted1d(a)) return;
     double amax = 0.0;
     for (int f = threadIdx.x; f <= a.mx; f += SMALL1D_NT) {
         double alpha = face1d_any<WK>(f, a);

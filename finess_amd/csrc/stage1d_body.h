@@ -24,7 +24,14 @@ struct Stage1DArgs {
     double* smax_faces;  // optional: alpha of each face f in [0, mx]
     double* Lout;        // optional: L of each cell
     double* galpha;      // global_alpha (null: off): the global wave speed, formed before the faces
+    // device-controlled stepping (null: off), as in the 2D stage: bdt = beta *
+    // (*dtp), and the kernels return at once while *halt is non-zero
+    const double* dtp;
+    double beta;
+    const unsigned long long* halt;
 };
+
+__host__ __device__ __forceinline__ bool halted1d(const Stage1DArgs& a) { return a.halt && *a.halt != 0ull; }
 
 // SetWaveSpd of the advection callbacks with both sides equal (|a|) and the
 // Max(alpha1, local) of lib/1d/ConstructL.cpp:148 when global_alpha is set.
@@ -110,7 +117,7 @@ __host__ __device__ __forceinline__ void update1d(int i, const Stage1DArgs& a)
     if (!a.out) return;
     double v = a.a1 * a.qa[i];
     if (a.use_b) v = v + a.a2 * a.qb[i];
-    a.out[i] = v + a.bdt * L;
+    a.out[i] = v + (a.dtp ? a.beta * *a.dtp : a.bdt) * L;
 }
 
 } // namespace finess

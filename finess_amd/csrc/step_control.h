@@ -24,6 +24,7 @@ struct StepCtl {
 
 struct StepCtlParams {
     double tend, max_dt, cfl_max, cfl_target, dx, dy;
+    int one_d;          // 1D GetCFL (lib/1d/GetCFL.cpp): x speeds only, bound 1e10
     long long nv;
     long long target;   // accepted steps to make; 0: run until t reaches tend
 };

@@ -54,9 +54,9 @@ __global__ __launch_bounds__(64) void k_step_control(StepCtl* c, unsigned long l
     if (!isfinite(sx) || !isfinite(sy)) { error(FINESS_ERR_NONFINITE, told + dt); return; }
     double cfl = -100.0;
     cfl = rmax(dt * sx / p.dx, cfl);
-    cfl = rmax(dt * sy / p.dy, cfl);
+    if (!p.one_d) cfl = rmax(dt * sy / p.dy, cfl);
     h.cfl = cfl;
-    if (cfl > 1.0e8) { error(FINESS_ERR_CFL, told + dt); return; }
+    if (cfl > (p.one_d ? 1.0e10 : 1.0e8)) { error(FINESS_ERR_CFL, told + dt); return; }
     double dtn;
     if (cfl > 0.0) {
         dtn = rmin(p.max_dt, dt * p.cfl_target / cfl);

@@ -113,8 +113,9 @@ typedef struct finess_params {
                                            later stages of that step and the next k-1 steps;
                                            0: it runs at every stage */
     int    host_controller;             /* 1: the host runs the time-step controller after
-                                           every step; 0 (default): 2D Runge-Kutta runs
-                                           without verbosity are controlled on the device
+                                           every step; 0 (default): Runge-Kutta runs of
+                                           time_order 1-3 (2D and 1D) without verbosity
+                                           are controlled on the device
                                            (same decisions, no host round trip per step) */
     int    mpp_limiter;                 /* Lax-Wendroff, 2D Euler: limit the fluxes toward the
                                            Lax-Friedrichs fluxes so that density and pressure stay

@@ -5,7 +5,10 @@ arithmetic is the host's operation for operation and the stage kernels are
 the same, so nothing may differ at all.  Cases: frames with tend clipping,
 a rejected first attempt, fixed step counts across the 64-step chunks, the
 routed form with a record reused over two steps (the routing state is
-restored after a rejection), and the error paths (nv limit, dt > max_dt)."""
+restored after a rejection), and the error paths (nv limit, dt > max_dt);
+in 1D (CFL from the x speeds only, bound 1e10; lib/1d/GetCFL.cpp) frames
+with a rejection at each time order, steps across chunks with
+global_alpha, and the nv limit."""
 import numpy as np
 import pytest
 
@@ -14,11 +17,11 @@ from finess_amd import PREC_FP32, ROUTE_ORACLE, FinessError, Solver, problems
 pytestmark = pytest.mark.gpu
 
 
-def _run(p, q, host, action, dtv0=None):
+def _run(p, q, host, action, dtv0=None, aux=None):
     p = p.copy()
     p.host_controller = 1 if host else 0
     with Solver(p) as s:
-        s.set_state(q)
+        s.set_state(q, aux)
         dtv = np.array([0.0, p.initial_dt, p.max_dt] if dtv0 is None else dtv0, dtype=np.float64)
         try:
             st = action(s, dtv)
@@ -37,9 +40,9 @@ def _stats(st):
     return (st.steps_accepted, st.steps_rejected, st.dt_min, st.dt_max, st.last_cfl, st.t)
 
 
-def _same(p, q, action, dtv0=None):
-    qh, th, dh, sh, eh = _run(p, q, True, action, dtv0)
-    qd, td, dd, sd, ed = _run(p, q, False, action, dtv0)
+def _same(p, q, action, dtv0=None, aux=None):
+    qh, th, dh, sh, eh = _run(p, q, True, action, dtv0, aux)
+    qd, td, dd, sd, ed = _run(p, q, False, action, dtv0, aux)
     print("host", _stats(sh), eh, "device", _stats(sd), ed)
     assert eh == ed
     assert th == td
@@ -90,3 +93,32 @@ def test_dt_above_max_dt():
     p, q, _ = problems.riemann4(mx=32)
     _, err = _same(p, q, lambda s, dtv: s.advance(1.0, dtv), dtv0=[0.0, 0.5, 0.1])
     assert err == -5
+
+
+def _advection_varying_speed(mx, **kw):
+    """1D advection with aux(1) = 1 + 0.5 sin(2 pi x): the step's CFL moves
+    with the state's speed field rather than staying fixed."""
+    p, q, _ = problems.advection_1d(mx=mx, **kw)
+    x = problems.cell_centres(p)
+    aux = np.ascontiguousarray((1.0 + 0.5 * np.sin(2 * np.pi * x))[:, None])
+    return p, q, aux
+
+
+@pytest.mark.parametrize("time_order", [1, 2, 3])
+def test_1d_frames_with_rejection(time_order):
+    # initial dt at CFL 6: the first attempt is rejected
+    p, q, aux = _advection_varying_speed(100, time_order=time_order, initial_dt=0.04)
+    st, _ = _same(p, q, lambda s, dtv: [s.advance(t, dtv) for t in (0.05, 0.1, 0.23)], aux=aux)
+    assert st[0].steps_rejected >= 1 and st[-1].t == 0.23
+
+
+def test_1d_fixed_steps_global_alpha():
+    p, q, aux = _advection_varying_speed(150, global_alpha=1)
+    st, _ = _same(p, q, lambda s, dtv: s.advance_steps(70, dtv), aux=aux)
+    assert st.steps_accepted == 70
+
+
+def test_1d_nv_limit():
+    p, q, aux = _advection_varying_speed(64, nv=3)
+    _, err = _same(p, q, lambda s, dtv: s.advance(1.0, dtv), aux=aux)
+    assert err == -3

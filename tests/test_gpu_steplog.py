@@ -1,7 +1,7 @@
 """ConSoln after every accepted step (finess_set_step_log /
 finess_read_step_log): one record (t, sum of dx dy q per field) per accepted
-step under the device controller (2D Runge-Kutta) and the host controller
-(Lax-Wendroff, 1D), none for rejected attempts; the sums are those of the
+step under the device controller (Runge-Kutta, 2D and 1D) and the host
+controller (Lax-Wendroff), none for rejected attempts; the sums are those of the
 states (1e-13 relative to the host sums in ConSoln's order) and stay constant
 on periodic problems."""
 import numpy as np

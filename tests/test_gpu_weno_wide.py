@@ -199,3 +199,20 @@ def test_wide_orders_rejected_where_the_device_has_no_path():
     for kw in bad:
         with pytest.raises(FinessError):
             problems.riemann4(mx=32, **kw)
+
+
+@pytest.mark.parametrize("version,order", [(WENO_JS, 5), (WENO_JS, 11), (WENO_Z, 9)])
+def test_construct_l_1d_large_grid(version, order):
+    """Above 8192 cells the 1D stage runs as two launches (one thread per face,
+    then one per cell) instead of one workgroup."""
+    r = (order + 1) // 2
+    p, q, aux = problems.advection_1d(mx=9001, space_order=order, weno_version=version, mbc=max(3, r))
+    q = q + 0.3 * RNG.random(q.shape)
+    orc.set_bnd(p, q, aux)
+    L0, s0 = orc.construct_l(p, q, aux)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q, aux=aux)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    print("1d large grid rel err", err)
+    assert np.all(err <= TOL), err
+    assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))

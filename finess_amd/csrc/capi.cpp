@@ -238,26 +238,34 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "routed stage launch failed") : 0;
 }
 
+// The arguments of one 1D stage: out = a1*qa + a2*qb + (beta*dt) L(src).
+Stage1DArgs stage1d_args(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2, double beta,
+                         double dt, int slot)
+{
+    Stage1DArgs a = base_stage1d_args(c);
+    a.src = c->buf[src];
+    a.qa = c->buf[qa];
+    a.qb = qb >= 0 ? c->buf[qb] : nullptr;
+    a.use_b = (qb >= 0 && a2 != 0.0);
+    a.out = c->buf[out];
+    a.a1 = a1;
+    a.a2 = a2;
+    a.bdt = beta * dt;
+    a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
+    if (c->devctl) {
+        a.dtp = &c->ctl->dt;
+        a.beta = beta;
+        a.halt = &c->ctl->halt;
+    }
+    return a;
+}
+
 // One stage: out = a1*qa + a2*qb + (beta*dt) L(src).  slot: CFL slot.
 int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2, double beta, double dt, int slot)
 {
     const finess_params& p = c->p;
     if (p.ndims == 1) {
-        Stage1DArgs a = base_stage1d_args(c);
-        a.src = c->buf[src];
-        a.qa = c->buf[qa];
-        a.qb = qb >= 0 ? c->buf[qb] : nullptr;
-        a.use_b = (qb >= 0 && a2 != 0.0);
-        a.out = c->buf[out];
-        a.a1 = a1;
-        a.a2 = a2;
-        a.bdt = beta * dt;
-        a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
-        if (c->devctl) {
-            a.dtp = &c->ctl->dt;
-            a.beta = beta;
-            a.halt = &c->ctl->halt;
-        }
+        const Stage1DArgs a = stage1d_args(c, src, qa, qb, out, a1, a2, beta, dt, slot);
         if (tmark(c)) return FINESS_ERR_HIP;
         const int rc = launch_stage1d(weno_kind(p), a, c->st);
         if (tmark(c)) return FINESS_ERR_HIP;
@@ -468,6 +476,23 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         return 0;
     }
     const bool one_d = p.ndims == 1;  // 1D: smax overwritten per stage
+    if (one_d && !c->timing && p.time_order <= 3) {
+        // the stages of SetRKinfo's SSP schemes (lib/2d/SetRKinfo.cpp:50-64),
+        // as the switch below enqueues them one by one, in one launch
+        Stage1DStep s;
+        s.n = p.time_order;
+        s.stage[0] = stage1d_args(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0);
+        if (p.time_order == 2) s.stage[1] = stage1d_args(c, B, B, A, C, 0.5, 0.5, 0.5, dt, 1);
+        if (p.time_order == 3) {
+            s.stage[1] = stage1d_args(c, B, A, B, C, 0.75, 0.25, 0.25, dt, 1);
+            s.stage[2] = stage1d_args(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, 2);
+        }
+        rc = launch_step1d(weno_kind(p), s, c->st);
+        if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D step launch failed");
+        *res = p.time_order == 2 ? C : B;
+        *slot = p.time_order - 1;
+        return 0;
+    }
     switch (p.time_order) {
     case 1:
         if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;

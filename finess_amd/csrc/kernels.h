@@ -9,14 +9,17 @@ enum : int { PHYS_ADVECTION = 0, PHYS_EULER = 1, PHYS_SHALLOW_WATER = 2 };
 
 struct StageArgs;
 struct Stage1DArgs;
+struct Stage1DStep;
 struct LxwArgs;
 
 // 2D fused stage.  wk: WenoKind; fp32: stencil arithmetic in FP32.
 int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, hipStream_t st);
 // WENO orders other than 5 (face-list kernels, FP64, Runge-Kutta)
 int launch_stage2d_wide(int physics, int wk, const StageArgs& a, hipStream_t st);
-// 1D stage (two launches: faces, update).
+// 1D stage (one workgroup up to 8192 cells, else two launches: faces, update).
 int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
+// The stages of one 1D time step (one launch up to 8192 cells).
+int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st);
 // Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].
 int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st);
 

@@ -5,9 +5,11 @@
 // (apps/1d/advection/smooth_example/SetBndValues.cpp:6-63).
 //
 // 1D grids are small (the advection case has 200 cells): up to SMALL1D cells
-// the stage is one workgroup (faces, barrier, cells), beyond it two plain
+// a stage is one workgroup (faces, barrier, cells), and a time step's stages
+// run in one launch of that workgroup; beyond it each stage is two plain
 // launches, one thread per face, then one thread per cell.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "kernels.h"
 #include "stage1d_body.h"
 
@@ -63,10 +65,10 @@ constexpr int SMALL1D = 8192;
 
 constexpr int SMALL1D_NT = 512;  // 2 waves per SIMD: the wide orders fit without spilling
 
+// One stage on a single workgroup: faces, barrier, cells, the CFL maximum.
 template <int WK>
-__global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
+__device__ __forceinline__ void stage1d_small_body(const Stage1DArgs& a, double* red)
 {
-    if (halted1d(a)) return;
     double amax = 0.0;
     for (int f = threadIdx.x; f <= a.mx; f += SMALL1D_NT) {
         double alpha = face1d_any<WK>(f, a);
@@ -77,7 +79,6 @@ __global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
     __syncthreads();
     for (int i = threadIdx.x; i < a.mx; i += SMALL1D_NT) update1d(i, a);
     if (!a.smax_slot) return;
-    __shared__ double red[SMALL1D_NT / 64];
     amax = wave_max64(amax);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
     __syncthreads();
@@ -85,6 +86,28 @@ __global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
         double m = red[0];
         for (int k = 1; k < SMALL1D_NT / 64; k++) m = red[k] > m ? red[k] : m;
         atomicMax(a.smax_slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+template <int WK>
+__global__ __launch_bounds__(SMALL1D_NT) void k_stage1d_small(Stage1DArgs a)
+{
+    if (halted1d(a)) return;
+    __shared__ double red[SMALL1D_NT / 64];
+    stage1d_small_body<WK>(a, red);
+}
+
+// Every stage of one time step in one workgroup: the barrier after each stage
+// orders its cell writes (and its reads of the shared face fluxes) before the
+// next stage's faces.  One launch per step instead of one per stage.
+template <int WK>
+__global__ __launch_bounds__(SMALL1D_NT) void k_step1d_small(Stage1DStep s)
+{
+    if (halted1d(s.stage[0])) return;
+    __shared__ double red[SMALL1D_NT / 64];
+    for (int k = 0; k < s.n; k++) {
+        stage1d_small_body<WK>(s.stage[k], red);
+        __syncthreads();
     }
 }
 
@@ -101,23 +124,50 @@ static int launch_wk(const Stage1DArgs& a, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
-int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
+// GlobalWaveSpd in 1D reads aux only, which no stage changes: one launch
+// serves all stages of a step.
+template <int WK>
+static int launch_step_wk(const Stage1DStep& s, hipStream_t st)
+{
+    if (s.stage[0].mx > SMALL1D) {
+        for (int k = 0; k < s.n; k++)
+            if (const int rc = launch_wk<WK>(s.stage[k], st)) return rc;
+        return 0;
+    }
+    if (s.stage[0].galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, s.stage[0]);
+    hipLaunchKernelGGL(k_step1d_small<WK>, dim3(1), dim3(SMALL1D_NT), 0, st, s);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+template <class F>
+static int with_kind(int wk, F f)
 {
     switch (wk) {
-    case WENO_JS5_P2: return launch_wk<WENO_JS5_P2>(a, st);
-    case WENO_JS5_POW: return launch_wk<WENO_JS5_POW>(a, st);
-    case WENO_Z5: return launch_wk<WENO_Z5>(a, st);
-    case WENO_FD5: return launch_wk<WENO_FD5>(a, st);
-    case WENO_JS7: return launch_wk<WENO_JS7>(a, st);
-    case WENO_JS9: return launch_wk<WENO_JS9>(a, st);
-    case WENO_JS11: return launch_wk<WENO_JS11>(a, st);
-    case WENO_Z7: return launch_wk<WENO_Z7>(a, st);
-    case WENO_Z9: return launch_wk<WENO_Z9>(a, st);
-    case WENO_FD7: return launch_wk<WENO_FD7>(a, st);
-    case WENO_FD9: return launch_wk<WENO_FD9>(a, st);
-    case WENO_O1: return launch_wk<WENO_O1>(a, st);
+    case WENO_JS5_P2: return f(std::integral_constant<int, WENO_JS5_P2>());
+    case WENO_JS5_POW: return f(std::integral_constant<int, WENO_JS5_POW>());
+    case WENO_Z5: return f(std::integral_constant<int, WENO_Z5>());
+    case WENO_FD5: return f(std::integral_constant<int, WENO_FD5>());
+    case WENO_JS7: return f(std::integral_constant<int, WENO_JS7>());
+    case WENO_JS9: return f(std::integral_constant<int, WENO_JS9>());
+    case WENO_JS11: return f(std::integral_constant<int, WENO_JS11>());
+    case WENO_Z7: return f(std::integral_constant<int, WENO_Z7>());
+    case WENO_Z9: return f(std::integral_constant<int, WENO_Z9>());
+    case WENO_FD7: return f(std::integral_constant<int, WENO_FD7>());
+    case WENO_FD9: return f(std::integral_constant<int, WENO_FD9>());
+    case WENO_O1: return f(std::integral_constant<int, WENO_O1>());
     default: return -1;
     }
+}
+
+int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
+{
+    return with_kind(wk, [&](auto k) { return launch_wk<decltype(k)::value>(a, st); });
+}
+
+int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st)
+{
+    if (s.n < 1 || s.n > 3) return -1;
+    return with_kind(wk, [&](auto k) { return launch_step_wk<decltype(k)::value>(s, st); });
 }
 
 } // namespace finess

@@ -33,6 +33,13 @@ struct Stage1DArgs {
 
 __host__ __device__ __forceinline__ bool halted1d(const Stage1DArgs& a) { return a.halt && *a.halt != 0ull; }
 
+// The stages of one time step, launched together (stage k reads what stage
+// k-1 wrote).
+struct Stage1DStep {
+    Stage1DArgs stage[3];
+    int n;
+};
+
 // SetWaveSpd of the advection callbacks with both sides equal (|a|) and the
 // Max(alpha1, local) of lib/1d/ConstructL.cpp:148 when global_alpha is set.
 __host__ __device__ __forceinline__ double alpha1d(const Stage1DArgs& a, double local)

@@ -79,6 +79,8 @@ struct finess_ctx {
     std::vector<double> hlog;    // records not yet read by the caller
     StepCtl* ctl = nullptr;                       // device controller block
     bool devctl = false;                          // the device controller drives the current run
+    const StepCtlParams* ctlp = nullptr;          // its parameters while devctl
+    bool ctl_enqueued = false;                    // enqueue_step launched the controller with the step
     bool route_fresh = true;                      // next routed stage starts a step
     int rec_age = -1;                             // steps since the record was decided (-1: none)
     int rec_par = 0;                              // counter pair of the current record
@@ -480,12 +482,18 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         // the stages of SetRKinfo's SSP schemes (lib/2d/SetRKinfo.cpp:50-64),
         // as the switch below enqueues them one by one, in one launch
         Stage1DStep s;
+        memset(&s, 0, sizeof s);
         s.n = p.time_order;
         s.stage[0] = stage1d_args(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0);
         if (p.time_order == 2) s.stage[1] = stage1d_args(c, B, B, A, C, 0.5, 0.5, 0.5, dt, 1);
         if (p.time_order == 3) {
             s.stage[1] = stage1d_args(c, B, A, B, C, 0.75, 0.25, 0.25, dt, 1);
             s.stage[2] = stage1d_args(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, 2);
+        }
+        if (c->devctl && c->ctlp) {  // the controller goes with the step
+            s.ctl = c->ctl;
+            s.cp = *c->ctlp;
+            c->ctl_enqueued = true;
         }
         rc = launch_step1d(weno_kind(p), s, c->st);
         if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D step launch failed");
@@ -1207,6 +1215,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         HIPCHK(c, hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->st));
         if (launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st)) return fail(c, FINESS_ERR_HIP, "slot reset failed");
         c->devctl = true;
+        c->ctlp = &cp;
     }
     struct Snap {
         int in, out, rec_age, rec_par;
@@ -1230,8 +1239,9 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
             Snap sn;
             sn.in = c->cur;
             int res = 0, slot = 0;
+            c->ctl_enqueued = false;
             if ((rc = enqueue_step(c, 0.0, &res, &slot))) break;
-            if (launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * slot, cp, c->st)) {
+            if (!c->ctl_enqueued && launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * slot, cp, c->st)) {
                 rc = fail(c, FINESS_ERR_HIP, "controller launch failed");
                 break;
             }
@@ -1295,6 +1305,8 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         break;
     }
     c->devctl = false;
+    c->ctlp = nullptr;
+    c->ctl_enqueued = false;
     if (!rc) dtv[1] = h.dt_next;
     c->t = h.t;
     if (stats) {

@@ -12,6 +12,7 @@
 #include <type_traits>
 #include "kernels.h"
 #include "stage1d_body.h"
+#include "step_control_body.h"
 
 namespace finess {
 
@@ -109,6 +110,9 @@ __global__ __launch_bounds__(SMALL1D_NT) void k_step1d_small(Stage1DStep s)
         stage1d_small_body<WK>(s.stage[k], red);
         __syncthreads();
     }
+    // wave 0 runs the controller on the last stage's maxima; the barrier
+    // above has ordered the stages' reads of dt before the block changes
+    if (s.ctl && threadIdx.x < 64) step_control_wave(s.ctl, s.stage[s.n - 1].smax_slot, s.cp, threadIdx.x);
 }
 
 template <int WK>
@@ -132,7 +136,7 @@ static int launch_step_wk(const Stage1DStep& s, hipStream_t st)
     if (s.stage[0].mx > SMALL1D) {
         for (int k = 0; k < s.n; k++)
             if (const int rc = launch_wk<WK>(s.stage[k], st)) return rc;
-        return 0;
+        return s.ctl ? launch_step_control(s.ctl, s.stage[s.n - 1].smax_slot, s.cp, st) : 0;
     }
     if (s.stage[0].galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, s.stage[0]);
     hipLaunchKernelGGL(k_step1d_small<WK>, dim3(1), dim3(SMALL1D_NT), 0, st, s);

@@ -3,6 +3,7 @@
 #pragma once
 #include "physics.h"
 #include "stage2d_body.h"
+#include "step_control.h"
 #include "weno.h"
 #include "weno_wide.h"
 
@@ -38,6 +39,10 @@ __host__ __device__ __forceinline__ bool halted1d(const Stage1DArgs& a) { return
 struct Stage1DStep {
     Stage1DArgs stage[3];
     int n;
+    // the device controller after the last stage, in the same launch (null:
+    // the host launches it, or there is none)
+    StepCtl* ctl;
+    StepCtlParams cp;
 };
 
 // SetWaveSpd of the advection callbacks with both sides equal (|a|) and the

@@ -166,6 +166,8 @@ SIGNATURES = {
     "finess_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_int)]),
     "finess_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _PD]),
+    "finess_construct_l_planes_noalloc": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_void_p,
+                                                        ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 

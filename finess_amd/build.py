@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfiness_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["stage2d.hip", "stage2d_routed.hip", "stage1d.hip", "lxw2d.hip", "mlp.hip", "batch.hip", "step_control.hip", "mpp2d.hip", "pif2d.hip", "rk_low_storage.hip", "stage2d_wide.hip",
+SOURCES = ["stage2d.hip", "stage2d_routed.hip", "stage1d.hip", "lxw2d.hip", "mlp.hip", "batch.hip", "step_control.hip", "mpp2d.hip", "pif2d.hip", "rk_low_storage.hip", "stage2d_wide.hip", "noalloc.hip",
            "capi.cpp", "mlp_api.cpp", "batch_api.cpp"]
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -195,6 +195,25 @@ int finess_apply_mpp_limiter_2d(finess_ctx* ctx, double dt, const double* q,
                                 const double* fLF, const double* gLF,
                                 double* fHat, double* gHat);
 
+/* ConstructL on device-resident field planes, allocating nothing.
+ * No counterpart in the reference: it exists for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  It computes what finess_construct_l computes for
+ * a 2D fifth-order context without routing, on device memory:
+ *   q      device, field planes of the interior, q[m*mx*my + i*my + j]
+ *          (0-based i, j); the halo is formed through p->bc
+ *   L      device, same layout, receives L = Lstar of the interior
+ *   smax2  device, 2 doubles: the maxima of the x- and y-face wave speeds that
+ *          GetCFL takes (faces mx+1/2 and my+1/2 excluded,
+ *          lib/2d/GetCFL.cpp:54-58)
+ *   stream hipStream_t (NULL: the null stream)
+ * Its only other device memory is a fixed 4 KiB array per device, of which
+ * it uses 2064 bytes for any grid size; every call on a device shares it, so
+ * calls that overlap on one device race.  The call is asynchronous on
+ * `stream` and launches kernels only (no memset, copy or allocation). */
+int finess_construct_l_planes_noalloc(const finess_params* p, const double* q, double* L, double* smax2,
+                                      void* stream);
+
 /* ---- device-resident stepping ------------------------------------------ */
 typedef struct finess_stats {
     long long steps_accepted;

@@ -29,6 +29,7 @@ PHYS_ADVECTION, PHYS_EULER, PHYS_SHALLOW_WATER = 0, 1, 2
 BC_EXTRAP, BC_PERIODIC, BC_REFLECT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 ROUTE_NONE, ROUTE_LEARNED, ROUTE_ORACLE = 0, 1, 2
+SRC_ZERO, SRC_SW_BATHYMETRY, SRC_DAMPING = 0, 1, 2
 
 _ERRNAMES = {
     ERR_CONFIG: "unsupported configuration",
@@ -77,6 +78,8 @@ class Params(ctypes.Structure):
         ("route_per_step", ctypes.c_int),
         ("host_controller", ctypes.c_int),
         ("mpp_limiter", ctypes.c_int),
+        ("source_kind", ctypes.c_int),
+        ("source_coef", ctypes.c_double),
     ]
 
     def copy(self):
@@ -166,6 +169,9 @@ SIGNATURES = {
     "finess_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(ctypes.c_int)]),
     "finess_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _PD]),
+    "finess_physics_eig": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int, _PD, _PD, _PD]),
+    "finess_physics_wave_speed": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int, _PD, _PD, _PD, _PD, _PD]),
+    "finess_physics_source": (ctypes.c_int, [ctypes.POINTER(Params), _PD, _PD, _PD]),
     "finess_construct_l_planes_noalloc": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_void_p,
                                                         ctypes.c_void_p, ctypes.c_void_p]),
 }

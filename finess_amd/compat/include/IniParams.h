@@ -66,6 +66,21 @@ public:
     // [gpu] (no counterpart in the reference)
     const std::string& get_gpu(const std::string& key, const std::string& dflt) const;
 
+    // Typed values of an app's own parameters.  The reference generates one
+    // accessor per parameter an app declares in its this_app_params.py; here
+    // gen_app_params.py (next to this library) writes those accessors into a
+    // file that an app build names with -DFINESS_APP_PARAMS="<file>", and
+    // they read their values through these: a missing key terminates (throws
+    // std::runtime_error) unless the declaration gives a default.
+    int app_int(const char* sec, const char* key, const char* dflt) const;
+    double app_double(const char* sec, const char* key, const char* dflt) const;
+    bool app_bool(const char* sec, const char* key, const char* dflt) const;
+    std::string app_string(const char* sec, const char* key, const char* dflt) const;
+    [[noreturn]] void app_check_failed(const char* name, const char* rule) const;
+#ifdef FINESS_APP_PARAMS
+#include FINESS_APP_PARAMS
+#endif
+
     // The C-ABI parameter block (include/finess_hip.h) for these settings;
     // physics and boundary fields are left for the caller to fill.
     void to_finess_params(finess_params* p) const;

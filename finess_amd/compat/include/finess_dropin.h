@@ -18,6 +18,8 @@ void FinSolveRK(StateVars& Qnew, double tend, double dtv[]);
 void FluxFunc(const dTensor1& xpts, const dTensor2& Q, const dTensor2& Aux, dTensor2& flux);
 void SetWaveSpd(const dTensor1& xedge, const dTensor1& Ql, const dTensor1& Qr, const dTensor1& Auxl,
                 const dTensor1& Auxr, double& s1, double& s2);
+// apps/1d/*/SourceTermFunc.cpp
+void SourceTermFunc(const dTensor1& xpts, const dTensor2& qvals, const dTensor2& auxvals, dTensor2& psi);
 #else
 // lib/2d/FinSolveRK.h:18
 void ConstructL(StateVars& Q, dTensorBC3& Lstar, dTensorBC3& smax);
@@ -33,6 +35,8 @@ void ProjectLeftEig(int ixy, const dTensor1& Aux_ave, const dTensor1& Q_ave, con
 void ProjectRightEig(int ixy, const dTensor1& Aux_ave, const dTensor1& Q_ave, const dTensor2& Wvals, dTensor2& Qvals);
 void SetWaveSpd(const dTensor1& nvec, const dTensor1& xedge, const dTensor1& Ql, const dTensor1& Qr,
                 const dTensor1& Auxl, const dTensor1& Auxr, double& s1, double& s2);
+// apps/2d/*/SourceTermFunc.cpp (sampled by lib/2d/SampleFunction.cpp)
+void SourceTermFunc(const dTensor2& xpts, const dTensor2& qvals, const dTensor2& auxvals, dTensor2& psi);
 #endif
 void SetBndValues(StateVars& Q);
 
@@ -43,11 +47,19 @@ struct ProbeReport {
     int bc[4];            // FINESS_BC_* (left, right, bottom, top)
     bool physics_checked; // the app's callbacks were linked and matched
     bool bc_checked;      // the app's SetBndValues was linked and classified
+    int sw_compat_hl;     // shallow water: 1 if the app's SetWaveSpd divides the right
+                          // momenta by h_l, 0 if by h_r, -1 not probed
+    int source_kind;      // FINESS_SRC_* matching the app's SourceTermFunc (-1: none probed)
+    double source_coef;   // its coefficient (FINESS_SRC_DAMPING)
     std::string notes;
 };
-// Identifies the physics and boundary conditions from the app's own callbacks
-// (or the [gpu] section when they are not linked).  Throws std::logic_error
-// when the app's physics or boundary cannot be run on the device.
+// Identifies the physics, boundary conditions and source term from the app's
+// own callbacks (or the [gpu] section when they are not linked): FluxFunc on
+// random states, ProjectLeftEig / ProjectRightEig applied to the identity and
+// SetWaveSpd on unequal states, all against the device physics evaluated on
+// the host (finess_physics_*), SetBndValues on coded cells, SourceTermFunc on
+// random states.  Throws std::logic_error when any of them cannot be run on
+// the device as the app defines it.
 ProbeReport probe_app();
 // Releases the device context held by the operators.
 void release();

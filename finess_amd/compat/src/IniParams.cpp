@@ -115,6 +115,31 @@ double as_double(const std::string& v, const std::string& name)
 }
 } // namespace
 
+int IniParams::app_int(const char* sec, const char* key, const char* dflt) const
+{
+    return as_int(raw(sec, key, dflt, dflt == nullptr), std::string(sec) + "." + key);
+}
+
+double IniParams::app_double(const char* sec, const char* key, const char* dflt) const
+{
+    return as_double(raw(sec, key, dflt, dflt == nullptr), std::string(sec) + "." + key);
+}
+
+bool IniParams::app_bool(const char* sec, const char* key, const char* dflt) const
+{
+    return as_bool(raw(sec, key, dflt, dflt == nullptr), std::string(sec) + "." + key);
+}
+
+std::string IniParams::app_string(const char* sec, const char* key, const char* dflt) const
+{
+    return raw(sec, key, dflt, dflt == nullptr);
+}
+
+void IniParams::app_check_failed(const char* name, const char* rule) const
+{
+    terminate(std::string(name) + " should be " + rule);
+}
+
 void IniParams::load()
 {
     // [finess]
@@ -239,4 +264,12 @@ void IniParams::to_finess_params(finess_params* p) const
     p->sw_compat_hl = as_bool(raw("gpu", "sw_compat_hl", "true", false), "gpu.sw_compat_hl");
     p->route_per_step = as_int(raw("gpu", "route_per_step", "1", false), "gpu.route_per_step");
     p->host_controller = as_bool(raw("gpu", "host_controller", "false", false), "gpu.host_controller") ? 1 : 0;
+    // the device form of SourceTermFunc when the app's is not linked (the
+    // drop-in probe identifies it from the app's own callback otherwise)
+    const std::string sk = raw("gpu", "source_kind", "zero", false);
+    if (sk == "zero") p->source_kind = FINESS_SRC_ZERO;
+    else if (sk == "sw_bathymetry") p->source_kind = FINESS_SRC_SW_BATHYMETRY;
+    else if (sk == "damping") p->source_kind = FINESS_SRC_DAMPING;
+    else throw std::runtime_error("gpu.source_kind: unknown source term form " + sk);
+    p->source_coef = as_double(raw("gpu", "source_coef", "0", false), "gpu.source_coef");
 }

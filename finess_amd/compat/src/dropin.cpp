@@ -35,6 +35,7 @@
 #pragma weak SetWaveSpd
 #endif
 #pragma weak SetBndValues
+#pragma weak SourceTermFunc
 
 namespace finess_dropin {
 
@@ -103,26 +104,126 @@ bool euler_matches(double gamma, std::string& notes)
         for (int m = 1; m <= 5; m++) err = std::max(err, std::max(rel(F.get(i, m, 1), f[m - 1]), rel(F.get(i, m, 2), g[m - 1])));
     }
     if (err > 1e-12) { notes += "FluxFunc is not the Euler flux; "; return false; }
-    if (&ProjectLeftEig && &ProjectRightEig) {
-        // left and right eigenvector projections must be inverse to each other
-        dTensor1 aux(1), qa(5);
-        for (int m = 1; m <= 5; m++) qa.set(m, Q.get(1, m));
-        aux.set(1, 0.0);
-        dTensor2 V(5, 5), W(5, 5), B(5, 5);
-        for (int a = 1; a <= 5; a++)
-            for (int b = 1; b <= 5; b++) V.set(a, b, a == b ? 1.0 : 0.0);
+    return true;
+}
+
+// Random admissible Euler (w != 0 included) or shallow-water states.
+std::vector<std::vector<double>> probe_states(int physics, double gamma, int n, unsigned seed)
+{
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<std::vector<double>> out;
+    for (int k = 0; k < n; k++) {
+        if (physics == FINESS_PHYS_EULER) {
+            const double rho = 0.3 + U(rng), u = 2 * U(rng) - 1, v = 2 * U(rng) - 1, w = U(rng) - 0.5,
+                         pr = 0.2 + U(rng);
+            out.push_back({rho, rho * u, rho * v, rho * w, pr / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)});
+        } else {
+            const double h = 0.4 + U(rng), u = 2 * U(rng) - 1, v = 2 * U(rng) - 1;
+            out.push_back({h, h * u, h * v});
+        }
+    }
+    return out;
+}
+
+double scaled_diff(double a, double b, double scale) { return std::fabs(a - b) / scale; }
+
+// ProjectLeftEig / ProjectRightEig applied to the identity at random mean
+// states against the matrices of the device physics: the normalisation and
+// the order of the characteristic fields change the WENO result, so the
+// app's eigenvectors must be the device's, not merely a pair of inverses.
+bool eig_conforms(const finess_params& p, std::string& notes)
+{
+    if (!&ProjectLeftEig || !&ProjectRightEig) return true;
+    const int M = p.meqn;
+    dTensor1 aux(std::max(1, p.maux)), qa(M);
+    for (int k = 1; k <= aux.getsize(); k++) aux.set(k, 0.0);
+    dTensor2 I(M, M), W(M, M);
+    for (int a = 1; a <= M; a++)
+        for (int b = 1; b <= M; b++) I.set(a, b, a == b ? 1.0 : 0.0);
+    std::vector<double> Lm(M * M), Rm(M * M);
+    for (const auto& st : probe_states(p.physics, p.gamma, 32, 11)) {
+        for (int m = 1; m <= M; m++) qa.set(m, st[m - 1]);
         for (int ixy = 1; ixy <= 2; ixy++) {
-            ProjectLeftEig(ixy, aux, qa, V, W);
-            ProjectRightEig(ixy, aux, qa, W, B);
-            for (int a = 1; a <= 5; a++)
-                for (int b = 1; b <= 5; b++)
-                    if (std::fabs(B.get(a, b) - (a == b ? 1.0 : 0.0)) > 1e-10) {
-                        notes += "ProjectRightEig is not the inverse of ProjectLeftEig; ";
-                        return false;
-                    }
+            if (finess_physics_eig(&p, ixy, st.data(), Lm.data(), Rm.data()) != FINESS_OK) {
+                notes += "no device eigenvectors for this physics; ";
+                return false;
+            }
+            for (int pass = 0; pass < 2; pass++) {
+                const std::vector<double>& D = pass ? Rm : Lm;
+                if (pass) ProjectRightEig(ixy, aux, qa, I, W);
+                else ProjectLeftEig(ixy, aux, qa, I, W);
+                double scale = 1e-300;
+                for (double v : D) scale = std::max(scale, std::fabs(v));
+                for (int a = 1; a <= M; a++)
+                    for (int b = 1; b <= M; b++)
+                        if (!(scaled_diff(W.get(a, b), D[(a - 1) * M + (b - 1)], scale) <= 1e-10)) {
+                            notes += pass ? "ProjectRightEig differs from the device's right eigenvectors; "
+                                          : "ProjectLeftEig differs from the device's left eigenvectors "
+                                            "(normalisation or field order); ";
+                            return false;
+                        }
+            }
         }
     }
     return true;
+}
+
+// SetWaveSpd on unequal states against the device's max(|s1|, |s2|) (the
+// only value the flux splitting and the CFL number use).
+bool speed_conforms(const finess_params& p, std::string& notes)
+{
+    if (!&SetWaveSpd) return true;
+    const int M = p.meqn;
+    const auto st = probe_states(p.physics, p.gamma, 64, 13);
+    dTensor1 nvec(2), xedge(2), Ql(M), Qr(M), Al(std::max(1, p.maux)), Ar(std::max(1, p.maux));
+    xedge.set(1, 0.5);
+    xedge.set(2, 0.5);
+    for (int k = 1; k <= Al.getsize(); k++) { Al.set(k, 0.0); Ar.set(k, 0.0); }
+    for (size_t k = 0; k + 1 < st.size(); k += 2) {
+        for (int m = 1; m <= M; m++) { Ql.set(m, st[k][m - 1]); Qr.set(m, st[k + 1][m - 1]); }
+        for (int ixy = 1; ixy <= 2; ixy++) {
+            nvec.set(1, ixy == 1 ? 1.0 : 0.0);
+            nvec.set(2, ixy == 2 ? 1.0 : 0.0);
+            double s1 = 0, s2 = 0, dev = 0;
+            SetWaveSpd(nvec, xedge, Ql, Qr, Al, Ar, s1, s2);
+            if (finess_physics_wave_speed(&p, ixy, st[k].data(), st[k + 1].data(), Al.data(), Ar.data(), &dev) !=
+                FINESS_OK)
+                return false;
+            const double app = std::max(std::fabs(s1), std::fabs(s2));
+            if (!(std::fabs(app - dev) <= 1e-10 * std::max(1.0, dev))) {
+                notes += "SetWaveSpd differs from the device's wave speeds; ";
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+bool euler_conforms(finess_params p, std::string& notes)
+{
+    p.physics = FINESS_PHYS_EULER;
+    return eig_conforms(p, notes) && speed_conforms(p, notes);
+}
+
+// Shallow water: the eigenvectors, then SetWaveSpd with the right momenta
+// divided by h_l (the reference's own SetWaveSpd.cpp:27-28) or by h_r;
+// *hl receives the form that matches.
+bool sw_conforms(finess_params p, int* hl, std::string& notes)
+{
+    p.physics = FINESS_PHYS_SHALLOW_WATER;
+    if (!eig_conforms(p, notes)) return false;
+    if (!&SetWaveSpd) return true;
+    for (int h : {1, 0}) {
+        p.sw_compat_hl = h;
+        std::string n;
+        if (speed_conforms(p, n)) {
+            *hl = h;
+            return true;
+        }
+    }
+    notes += "SetWaveSpd matches neither shallow-water wave-speed form; ";
+    return false;
 }
 
 bool sw_matches(std::string& notes)
@@ -199,6 +300,53 @@ bool classify_side(StateVars& Q, int side, int mx, int my, int mbc, int meqn, in
 double code(int i, int m) { return 1.0 + m + 8.0 * (double)(i + 64); }
 #endif
 
+// ------------------------------------------------------------ source term
+// The app's SourceTermFunc on random states and aux values against the
+// device forms (finess_physics_source): FINESS_SRC_* that matches, with *coef
+// its coefficient, or -1.
+int source_kind_of_app(finess_params p, double* coef)
+{
+    std::mt19937_64 rng(17);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const int n = 32, M = p.meqn, na = std::max(1, p.maux);
+#if NDIMS == 2
+    dTensor2 x(n, 2);
+#else
+    dTensor1 x(n);
+#endif
+    dTensor2 Q(n, M), A(n, na), psi(n, M);
+    for (int i = 1; i <= n; i++) {
+#if NDIMS == 2
+        x.set(i, 1, U(rng));
+        x.set(i, 2, U(rng));
+#else
+        x.set(i, U(rng));
+#endif
+        for (int m = 1; m <= M; m++) Q.set(i, m, (m == 1 ? 0.5 : -0.5) + U(rng));
+        for (int k = 1; k <= na; k++) A.set(i, k, 2 * U(rng) - 1);
+    }
+    SourceTermFunc(x, Q, A, psi);
+    // the relaxation coefficient, if it is one: psi = -k q
+    *coef = -psi.get(1, 1) / Q.get(1, 1);
+    const int kinds[3] = {FINESS_SRC_ZERO, FINESS_SRC_SW_BATHYMETRY, FINESS_SRC_DAMPING};
+    for (int kind : kinds) {
+        if (kind == FINESS_SRC_SW_BATHYMETRY && (M != 3 || p.maux < 3)) continue;
+        p.source_kind = kind;
+        p.source_coef = kind == FINESS_SRC_DAMPING ? *coef : 0.0;
+        bool ok = true;
+        for (int i = 1; i <= n && ok; i++) {
+            double q[8], a[16], d[8];
+            for (int m = 1; m <= M; m++) q[m - 1] = Q.get(i, m);
+            for (int k = 1; k <= na && k <= 16; k++) a[k - 1] = A.get(i, k);
+            if (finess_physics_source(&p, q, a, d) != FINESS_OK) { ok = false; break; }
+            for (int m = 1; m <= M; m++)
+                if (!(std::fabs(psi.get(i, m) - d[m - 1]) <= 1e-12 * (1.0 + std::fabs(d[m - 1])))) ok = false;
+        }
+        if (ok) return kind;
+    }
+    return -1;
+}
+
 } // namespace
 
 ProbeReport probe_app()
@@ -208,6 +356,11 @@ ProbeReport probe_app()
     r.physics = -1;
     r.physics_checked = false;
     r.bc_checked = false;
+    r.sw_compat_hl = -1;
+    r.source_kind = -1;
+    r.source_coef = 0.0;
+    finess_params ip;
+    ini.to_finess_params(&ip);
     for (int k = 0; k < 4; k++) r.bc[k] = FINESS_BC_EXTRAP;
     const std::string phys = ini.get_gpu("physics", "");
     const int meqn = ini.get_meqn();
@@ -219,6 +372,11 @@ ProbeReport probe_app()
         else if (r.physics == FINESS_PHYS_SHALLOW_WATER) r.physics_checked = sw_matches(r.notes);
         if (!r.physics_checked) throw std::logic_error("finess-hip: the app's physics matches no device physics: " + r.notes);
     }
+    // the eigenvectors and wave speeds the app defines must be the device's
+    bool conform = true;
+    if (r.physics == FINESS_PHYS_EULER) conform = euler_conforms(ip, r.notes);
+    else if (r.physics == FINESS_PHYS_SHALLOW_WATER) conform = sw_conforms(ip, &r.sw_compat_hl, r.notes);
+    if (!conform) throw std::logic_error("finess-hip: the app's callbacks differ from the device physics: " + r.notes);
 #else
     if (phys == "advection" || (phys.empty() && meqn == 1)) r.physics = FINESS_PHYS_ADVECTION;
     if (&FluxFunc) {
@@ -227,6 +385,12 @@ ProbeReport probe_app()
     }
 #endif
     if (r.physics < 0) throw std::logic_error("finess-hip: no device physics for meqn = " + std::to_string(meqn));
+    if (ini.get_source_term() && &SourceTermFunc) {
+        ip.physics = r.physics;
+        r.source_kind = source_kind_of_app(ip, &r.source_coef);
+        if (r.source_kind < 0)
+            throw std::logic_error("finess-hip: the app's SourceTermFunc matches no device source term");
+    }
 
     const std::string words = ini.get_gpu("boundary", "");
     if (!words.empty()) {
@@ -296,6 +460,11 @@ finess_ctx* context()
     global_ini_params.to_finess_params(&p);
     p.physics = g_report.physics;
     for (int k = 0; k < 4; k++) p.bc[k] = g_report.bc[k];
+    if (g_report.sw_compat_hl >= 0) p.sw_compat_hl = g_report.sw_compat_hl;
+    if (g_report.source_kind >= 0) {
+        p.source_kind = g_report.source_kind;
+        p.source_coef = g_report.source_coef;
+    }
     int rc = finess_params_finalize(&p);
     if (rc) fail_like_reference(rc, "parameter combination");
     if (g_ctx && std::memcmp(&p, &g_params, sizeof p) == 0) return g_ctx;

@@ -23,6 +23,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/finess_hip.h"
@@ -45,6 +46,7 @@ struct finess_ctx {
     // registers of the fourth- and fifth-order schemes (q2; L)
     double* buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double* aux = nullptr;      // 1D advection speed (interior cells)
+    double* aux2d = nullptr;    // 2D aux planes (interior cells), for the source term
     double* lxwF = nullptr;
     double* lxwG = nullptr;
     double* staging = nullptr;  // reference layout with ghosts
@@ -170,6 +172,9 @@ StageArgs base_stage_args(finess_ctx* c)
         a.galpha = c->galpha;
         a.gspread = c->gspread;
     }
+    a.src_kind = source_arg(p.source_term, p.source_kind);
+    a.src_coef = p.source_coef;
+    a.aux = c->aux2d;
     return a;
 }
 
@@ -188,6 +193,8 @@ Stage1DArgs base_stage1d_args(finess_ctx* c)
     a.aux = c->aux;
     if (p.global_alpha) a.galpha = c->galpha;
     a.fhat = c->fhat1d;
+    a.src_kind = source_arg(p.source_term, p.source_kind);
+    a.src_coef = p.source_coef;
     return a;
 }
 
@@ -253,7 +260,10 @@ Stage1DArgs stage1d_args(finess_ctx* c, int src, int qa, int qb, int out, double
     a.a1 = a1;
     a.a2 = a2;
     a.bdt = beta * dt;
-    a.smax_slot = c->smax + 2 * CFL_SPREAD * slot;
+    // 1D GetCFL reads the last stage's speeds only (lib/1d/ConstructL.cpp:149
+    // overwrites smax): the other stages pass slot -1 and reduce nothing
+    a.smax_slot = slot >= 0 ? c->smax + 2 * CFL_SPREAD * slot : nullptr;
+    if (c->p.time_stepping_method == FINESS_TS_LXW) a.lxwF = c->lxwF;  // the 1D ConstructLxWL
     if (c->devctl) {
         a.dtp = &c->ctl->dt;
         a.beta = beta;
@@ -284,7 +294,7 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
     a.a1 = a1;
     a.a2 = a2;
     a.bdt = beta * dt;
-    a.smax = c->smax + 2 * CFL_SPREAD * slot;
+    a.smax = slot >= 0 ? c->smax + 2 * CFL_SPREAD * slot : nullptr;
     if (c->devctl) {
         a.dtp = &c->ctl->dt;
         a.beta = beta;
@@ -312,6 +322,12 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
 int lxw_fluxes(finess_ctx* c, int src, double dt)
 {
     const finess_params& p = c->p;
+    if (p.ndims == 1) {  // ConstructIntegratedF (lib/1d/ConstructIntegratedF.cpp:54-212)
+        Stage1DArgs a = base_stage1d_args(c);
+        a.src = c->buf[src];
+        const int rc = launch_lxw_flux1d(a, dt, p.time_order, c->lxwF, c->st);
+        return rc ? fail(c, FINESS_ERR_HIP, "1D LxW flux launch failed") : 0;
+    }
     LxwArgs a;
     memset(&a, 0, sizeof a);
     a.q = c->buf[src];
@@ -433,15 +449,26 @@ const double RK5_BETA[8] = {8.653258038183180e-02, 9.544677980851571e-01, 2.6519
                             2.736914413910379e-01, 5.999778649323600e-01, 4.433177471748104e-03,
                             5.309971130968292e-03, 5.830861806762871e-01};
 
-// L(buf[src]) into Lbuf, with the wave-speed maxima of the step (slot 0).
-int stage_l(finess_ctx* c, int src, double* Lbuf)
+// L(buf[src]) into Lbuf, with the wave-speed maxima into `slot` (-1: none).
+int stage_l(finess_ctx* c, int src, double* Lbuf, int slot)
 {
     const finess_params& p = c->p;
+    if (p.ndims == 1) {
+        Stage1DArgs a = base_stage1d_args(c);
+        a.src = c->buf[src];
+        a.qa = c->buf[src];
+        a.Lout = Lbuf;
+        a.smax_slot = slot >= 0 ? c->smax + 2 * CFL_SPREAD * slot : nullptr;
+        if (tmark(c)) return FINESS_ERR_HIP;
+        const int rc = launch_stage1d(weno_kind(p), a, c->st);
+        if (tmark(c)) return FINESS_ERR_HIP;
+        return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D stage launch failed") : 0;
+    }
     StageArgs a = base_stage_args(c);
     a.src = c->buf[src];
     a.qa = c->buf[src];
     a.a_is_src = 1;
-    a.smax = c->smax;
+    a.smax = slot >= 0 ? c->smax + 2 * CFL_SPREAD * slot : nullptr;
     a.Lout = Lbuf;
     if (tmark(c)) return FINESS_ERR_HIP;
     const int rc = launch_stage2d(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), 0, a, c->st);
@@ -484,10 +511,12 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         Stage1DStep s;
         memset(&s, 0, sizeof s);
         s.n = p.time_order;
-        s.stage[0] = stage1d_args(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0);
-        if (p.time_order == 2) s.stage[1] = stage1d_args(c, B, B, A, C, 0.5, 0.5, 0.5, dt, 1);
-        if (p.time_order == 3) {
-            s.stage[1] = stage1d_args(c, B, A, B, C, 0.75, 0.25, 0.25, dt, 1);
+        // (only the last stage reduces its wave speeds: 1D GetCFL reads those)
+        const int n = p.time_order;
+        s.stage[0] = stage1d_args(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, n == 1 ? 0 : -1);
+        if (n == 2) s.stage[1] = stage1d_args(c, B, B, A, C, 0.5, 0.5, 0.5, dt, 1);
+        if (n == 3) {
+            s.stage[1] = stage1d_args(c, B, A, B, C, 0.75, 0.25, 0.25, dt, -1);
             s.stage[2] = stage1d_args(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, 2);
         }
         if (c->devctl && c->ctlp) {  // the controller goes with the step
@@ -508,14 +537,14 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         *slot = 0;
         return 0;
     case 2:
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, one_d ? -1 : 0))) return rc;
         if ((rc = stage(c, B, B, A, C, 0.5, 0.5, 0.5, dt, one_d ? 1 : 0))) return rc;
         *res = C;
         *slot = one_d ? 1 : 0;
         return 0;
     case 3:
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
-        if ((rc = stage(c, B, A, B, C, 0.75, 0.25, 0.25, dt, one_d ? 1 : 0))) return rc;
+        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, one_d ? -1 : 0))) return rc;
+        if ((rc = stage(c, B, A, B, C, 0.75, 0.25, 0.25, dt, one_d ? -1 : 0))) return rc;
         if ((rc = stage(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, one_d ? 2 : 0))) return rc;
         *res = B;
         *slot = one_d ? 2 : 0;
@@ -523,17 +552,18 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     case 4: {
         // SSP(10,4) (lib/2d/FinSolveRK.cpp:213-291): stages ping-pong between
         // B and C, q2 lives in buffer 3
+        // (1D: only the last stage's speeds count, in slot 1)
         int x = A, y = B;
         for (int s = 1; s <= 9; s++) {
-            if ((rc = stage(c, x, x, -1, y, 1.0, 0.0, 1.0 / 6.0, dt, 0))) return rc;
+            if ((rc = stage(c, x, x, -1, y, 1.0, 0.0, 1.0 / 6.0, dt, one_d ? -1 : 0))) return rc;
             x = y;
             y = (y == B) ? C : B;
             if (s == 5 && launch_rk4_combine(c->buf[A], c->buf[3], c->buf[x], (long long)c->M * c->plane, c->st))
                 return fail(c, FINESS_ERR_HIP, "RK4 combination launch failed");
         }
-        if ((rc = stage(c, x, 3, x, y, 1.0, 3.0 / 5.0, 1.0 / 10.0, dt, 0))) return rc;
+        if ((rc = stage(c, x, 3, x, y, 1.0, 3.0 / 5.0, 1.0 / 10.0, dt, one_d ? 1 : 0))) return rc;
         *res = y;
-        *slot = 0;
+        *slot = one_d ? 1 : 0;
         return 0;
     }
     case 5: {
@@ -542,7 +572,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         const long long n = (long long)c->M * c->plane;
         int x = A, y = B;
         for (int s = 0; s < 8; s++) {
-            if ((rc = stage_l(c, x, c->buf[4]))) return rc;
+            if ((rc = stage_l(c, x, c->buf[4], one_d ? (s == 7 ? 1 : -1) : 0))) return rc;
             if (launch_rk5_update(c->buf[x], c->buf[3], c->buf[A], c->buf[4], c->buf[y], n, RK5_GAMMA[0][s],
                                   RK5_GAMMA[1][s], RK5_GAMMA[2][s], RK5_DELTA[s], RK5_BETA[s] * dt, s == 0, c->st))
                 return fail(c, FINESS_ERR_HIP, "RK5 update launch failed");
@@ -550,7 +580,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
             y = (y == B) ? C : B;
         }
         *res = x;
-        *slot = 0;
+        *slot = one_d ? 1 : 0;
         return 0;
     }
     default:
@@ -626,6 +656,77 @@ int upload_ghosted(finess_ctx* c, const double* q)
     return 0;
 }
 
+// The 2D aux planes the source term reads: the interior of the caller's aux
+// (reference layout with ghosts, maux fields), once per call that hands it over.
+int upload_aux2d(finess_ctx* c, const double* aux)
+{
+    if (!c->aux2d || !aux) return 0;
+    const finess_params& p = c->p;
+    const int g = p.mbc, na = p.maux;
+    std::vector<double> tmp((size_t)na * c->plane);
+    for (int m = 0; m < na; m++)
+        for (int i = 0; i < p.mx; i++)
+            for (int j = 0; j < p.my; j++)
+                tmp[(size_t)m * c->plane + (size_t)i * p.my + j] = aux[((size_t)(i + g) * (p.my + 2 * g) + (j + g)) * na + m];
+    HIPCHK(c, hipMemcpy(c->aux2d, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+} // namespace
+
+namespace {
+// Rotated frame of direction ixy (1: x, 2: y): the two in-plane momenta swap,
+// as the reference's (mu, mv) pairs do.
+template <class P>
+void rotate(const double* q, int dir, double* r)
+{
+    for (int k = 0; k < P::MEQN; k++) r[P::rot_q(k, dir)] = q[k];
+}
+
+template <class P>
+int eig_matrices(const finess_params& p, int dir, const double* qavg, double* Lm, double* Rm)
+{
+    constexpr int M = P::MEQN;
+    PhysParams pp{p.gamma, p.sw_compat_hl};
+    double qa[M];
+    rotate<P>(qavg, dir, qa);
+    typename P::template Eig<double> e;
+    P::eig(qa, pp, e);
+    // The kernels number the characteristic fields in the rotated frame.  The
+    // Euler callbacks number the entropy and shear fields of ixy = 2 the other
+    // way round (their rows follow (mu, mv)), so Euler's field k is the
+    // callbacks' field rot_q(k, dir); shallow water keeps one order.  The
+    // order does not change the flux: every field is reconstructed on its own.
+    auto fld = [dir](int k) { return std::is_same<P, Euler2D>::value ? P::rot_q(k, dir) : k; };
+    for (int j = 0; j < M; j++) {
+        // column j of L: L applied to the unit vector e_j of the unrotated frame
+        double v[M] = {}, vr[M], w[M];
+        v[j] = 1.0;
+        rotate<P>(v, dir, vr);
+        P::left(e, vr, w);
+        for (int k = 0; k < M; k++) Lm[fld(k) * M + j] = w[k];
+        // column j of R: the unrotated image of the characteristic unit vector
+        double u[M] = {}, qr[M], qo[M];
+        u[j] = 1.0;
+        P::right(e, u, qr);
+        rotate<P>(qr, dir, qo);  // the swap is its own inverse
+        for (int k = 0; k < M; k++) Rm[k * M + fld(j)] = qo[k];
+    }
+    return FINESS_OK;
+}
+
+template <class P>
+double speed(const finess_params& p, int dir, const double* ql, const double* qr)
+{
+    constexpr int M = P::MEQN;
+    PhysParams pp{p.gamma, p.sw_compat_hl};
+    double cl[P::NV], cr[P::NV], rl[M], rr[M];
+    P::cell_record(ql, pp, cl);
+    P::cell_record(qr, pp, cr);
+    P::state(cl, dir, rl);
+    P::state(cr, dir, rr);
+    return P::wave_speed(rl, rr, cl, cr, pp);
+}
 } // namespace
 
 // ======================================================================
@@ -695,7 +796,15 @@ int finess_params_finalize(finess_params* p)
             p->precision != FINESS_PREC_FP64 || p->mbc < r || p->mx < r || (p->ndims == 2 && p->my < r))
             return FINESS_ERR_CONFIG;
     }
-    if (p->source_term) return FINESS_ERR_CONFIG;
+    // source terms (SampleFunction + SourceTermFunc in ConstructL): Runge-Kutta
+    // and ConstructL; ConstructLxWL exits on them (lib/2d/ConstructLxWL.cpp:376-380)
+    if (p->source_term) {
+        if (p->time_stepping_method != FINESS_TS_RK || p->routing != FINESS_ROUTE_NONE) return FINESS_ERR_CONFIG;
+        if (p->source_kind < FINESS_SRC_ZERO || p->source_kind > FINESS_SRC_DAMPING) return FINESS_ERR_CONFIG;
+        if (p->source_kind == FINESS_SRC_SW_BATHYMETRY &&
+            (p->ndims != 2 || p->physics != FINESS_PHYS_SHALLOW_WATER || p->maux < 3))
+            return FINESS_ERR_CONFIG;
+    }
     // global_alpha (GlobalWaveSpd before each operator call) runs in the 1D
     // and 2D stage kernels: Runge-Kutta, Lax-Wendroff (limited or not), RK4-PIF's
     // final stage (ConstructL_NOC computes the speeds but does not use them,
@@ -703,7 +812,9 @@ int finess_params_finalize(finess_params* p)
     if (p->global_alpha && p->routing != FINESS_ROUTE_NONE) return FINESS_ERR_CONFIG;
     if (p->ndims == 1) {
         if (p->physics != FINESS_PHYS_ADVECTION || p->meqn != 1) return FINESS_ERR_CONFIG;
-        if (p->time_stepping_method != FINESS_TS_RK) return FINESS_ERR_CONFIG;
+        // Runge-Kutta, or Lax-Wendroff without the limiter (lib/1d/FinSolveLxW.cpp)
+        if (p->time_stepping_method == FINESS_TS_PIF || (p->time_stepping_method == FINESS_TS_LXW && p->mpp_limiter))
+            return FINESS_ERR_CONFIG;
         if (p->mx < 3) return FINESS_ERR_CONFIG;
     } else if (p->ndims == 2) {
         if (p->physics == FINESS_PHYS_EULER && p->meqn != 5) return FINESS_ERR_CONFIG;
@@ -718,7 +829,9 @@ int finess_params_finalize(finess_params* p)
     if (p->mbc > p->mx || (p->ndims == 2 && p->mbc > p->my)) return FINESS_ERR_CONFIG;
     if (p->time_stepping_method < FINESS_TS_RK || p->time_stepping_method > FINESS_TS_PIF) return FINESS_ERR_CONFIG;
     if (p->time_stepping_method == FINESS_TS_LXW) {
-        if (p->ndims != 2 || p->mbc < 5 || p->physics == FINESS_PHYS_ADVECTION) return FINESS_ERR_CONFIG;
+        // the 5-point differences of the integrated fluxes need mbc >= 5
+        // (lib/2d/ConstructIntegratedR.cpp:87-90, lib/1d/ConstructIntegratedF.cpp)
+        if (p->mbc < 5 || (p->ndims == 2 && p->physics == FINESS_PHYS_ADVECTION)) return FINESS_ERR_CONFIG;
         // the limiter (ApplyMPPLimiter2D) and the FP32 stencil form of this
         // step exist for Euler only
         if (p->physics != FINESS_PHYS_EULER && (p->mpp_limiter || p->precision != FINESS_PREC_FP64))
@@ -728,7 +841,7 @@ int finess_params_finalize(finess_params* p)
         // RK4-PIF exists for time_order 4 only (lib/2d/FinSolveRK_PIF.cpp:151-206)
         if (p->ndims != 2 || p->physics == FINESS_PHYS_ADVECTION || p->time_order != 4) return FINESS_ERR_CONFIG;
         if (p->precision != FINESS_PREC_FP64) return FINESS_ERR_CONFIG;
-    } else if (p->time_order < 1 || p->time_order > (p->ndims == 2 ? 5 : 3)) {
+    } else if (p->time_order < 1 || p->time_order > 5) {
         return FINESS_ERR_CONFIG;
     }
     // the routed stages follow the three-stage schedule
@@ -746,6 +859,52 @@ int finess_params_finalize(finess_params* p)
         return FINESS_ERR_CONFIG;
     for (int k = 0; k < 4; k++)
         if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_REFLECT) return FINESS_ERR_CONFIG;
+    return FINESS_OK;
+}
+
+// ---- the device physics on the host (callback conformance probe) --------
+
+
+int finess_physics_eig(const finess_params* p, int ixy, const double* qavg, double* Lmat, double* Rmat)
+{
+    if (!p || !qavg || !Lmat || !Rmat || (ixy != 1 && ixy != 2)) return FINESS_ERR_ARG;
+    if (p->physics == FINESS_PHYS_EULER) return eig_matrices<Euler2D>(*p, ixy - 1, qavg, Lmat, Rmat);
+    if (p->physics == FINESS_PHYS_SHALLOW_WATER) return eig_matrices<ShallowWater2D>(*p, ixy - 1, qavg, Lmat, Rmat);
+    if (p->physics == FINESS_PHYS_ADVECTION) {  // identity projections (apps/1d/advection/lib)
+        Lmat[0] = Rmat[0] = 1.0;
+        return FINESS_OK;
+    }
+    return FINESS_ERR_CONFIG;
+}
+
+int finess_physics_wave_speed(const finess_params* p, int ixy, const double* ql, const double* qr,
+                              const double* auxl, const double* auxr, double* speed_out)
+{
+    if (!p || !ql || !qr || !speed_out || (ixy != 1 && ixy != 2)) return FINESS_ERR_ARG;
+    if (p->physics == FINESS_PHYS_EULER) *speed_out = speed<Euler2D>(*p, ixy - 1, ql, qr);
+    else if (p->physics == FINESS_PHYS_SHALLOW_WATER) *speed_out = speed<ShallowWater2D>(*p, ixy - 1, ql, qr);
+    else if (p->physics == FINESS_PHYS_ADVECTION) {
+        if (!auxl || !auxr) return FINESS_ERR_ARG;
+        // s1 = min(a_l, mean), s2 = max(a_r, mean) (apps/1d/advection/lib/SetWaveSpd.cpp)
+        const double al = auxl[0], ar = auxr[0];
+        const double s1 = rmin(al, 0.5 * (al + ar)), s2 = rmax(ar, 0.5 * (al + ar));
+        *speed_out = rmax(fabs(s1), fabs(s2));
+    } else {
+        return FINESS_ERR_CONFIG;
+    }
+    return FINESS_OK;
+}
+
+int finess_physics_source(const finess_params* p, const double* q, const double* aux, double* psi)
+{
+    if (!p || !q || !psi) return FINESS_ERR_ARG;
+    if (p->source_kind == FINESS_SRC_SW_BATHYMETRY && (p->meqn != 3 || p->maux < 3 || !aux)) return FINESS_ERR_CONFIG;
+    const int M = p->meqn;
+    const int kind = source_arg(1, p->source_kind);
+    if (M == 5) source_psi<5>(kind, p->source_coef, q, aux, 1, psi);
+    else if (M == 3) source_psi<3>(kind, p->source_coef, q, aux, 1, psi);
+    else if (M == 1) source_psi<1>(kind, p->source_coef, q, aux, 1, psi);
+    else return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -792,6 +951,12 @@ int finess_create(const finess_params* pin, finess_ctx** out)
             if (hipMemcpy(c->aux, one.data(), p.mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
                 rc = FINESS_ERR_HIP;
         }
+    }
+    if (!rc && p.ndims == 2 && p.source_term && p.source_kind == FINESS_SRC_SW_BATHYMETRY) {
+        // aux = 0 until the caller provides it (AuxFunc)
+        const size_t ab = (size_t)p.maux * c->plane * sizeof(double);
+        rc = alloc(c, (void**)&c->aux2d, ab);
+        if (!rc && hipMemset(c->aux2d, 0, ab) != hipSuccess) rc = FINESS_ERR_HIP;
     }
     if (!rc && p.time_stepping_method == FINESS_TS_PIF) {
         rc = alloc(c, (void**)&c->pif_fx, (size_t)(p.mx + 1) * p.my * c->M * sizeof(double));
@@ -845,6 +1010,7 @@ void finess_destroy(finess_ctx* c)
     if (c->st) hipStreamSynchronize(c->st);
     for (int k = 0; k < 5; k++) hipFree(c->buf[k]);
     hipFree(c->aux);
+    hipFree(c->aux2d);
     hipFree(c->lxwF);
     hipFree(c->lxwG);
     hipFree(c->staging);
@@ -989,6 +1155,8 @@ int finess_set_state(finess_ctx* c, const double* q, const double* aux, double t
         if (launch_ref_to_planes(c->staging, c->buf[0], p.mx, p.my, c->M, p.mbc, 0, c->st))
             return fail(c, FINESS_ERR_HIP, "layout conversion failed");
         HIPCHK(c, hipStreamSynchronize(c->st));
+        int rc;
+        if ((rc = upload_aux2d(c, aux))) return rc;
     }
     c->t = t;
     c->rec_age = -1;  // a new state has no routing record yet
@@ -1057,6 +1225,7 @@ int finess_construct_l(finess_ctx* c, const double* q, const double* aux, double
     }
     if (!c->faces_x && (rc = alloc(c, (void**)&c->faces_x, (size_t)(p.mx + 1) * p.my * sizeof(double)))) return rc;
     if (!c->faces_y && (rc = alloc(c, (void**)&c->faces_y, (size_t)p.mx * (p.my + 1) * sizeof(double)))) return rc;
+    if ((rc = upload_aux2d(c, aux))) return rc;
     StageArgs a = base_stage_args(c);
     a.src = c->ghosted;
     a.src_ghosts = p.mbc;
@@ -1097,12 +1266,37 @@ int finess_construct_l(finess_ctx* c, const double* q, const double* aux, double
 
 int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, const double* aux, double* F, double* G)
 {
-    (void)aux;
-    if (!c || !q || !F || !G) return FINESS_ERR_ARG;
+    if (!c || !q || !F) return FINESS_ERR_ARG;
     const finess_params& p = c->p;
-    if (p.ndims != 2 || p.mbc < 5 || p.physics == FINESS_PHYS_ADVECTION)
-        return fail(c, FINESS_ERR_CONFIG, "LxW needs 2D Euler or shallow water, mbc>=5");
+    if (p.mbc < 5) return fail(c, FINESS_ERR_CONFIG, "the integrated fluxes need mbc >= 5");
     int rc;
+    if (p.ndims == 1) {
+        // ConstructIntegratedF (lib/1d/ConstructIntegratedF.cpp:54-212) on the
+        // caller's ghosted q and aux; F receives cells [1-3, mx+3]
+        if ((rc = upload_ghosted(c, q))) return rc;
+        const int n = p.mx + 2 * p.mbc, na = p.maux > 0 ? p.maux : 1;
+        std::vector<double> at((size_t)n, 1.0), hF((size_t)p.mx + 6);
+        if (aux)
+            for (int i = 0; i < n; i++) at[i] = aux[(size_t)i * na];
+        double *auxg = nullptr, *Fd = nullptr;
+        if ((rc = alloc(c, (void**)&auxg, (size_t)n * sizeof(double)))) return rc;
+        if ((rc = alloc(c, (void**)&Fd, hF.size() * sizeof(double)))) { hipFree(auxg); return rc; }
+        Stage1DArgs a = base_stage1d_args(c);
+        a.src = c->ghosted;
+        a.aux = auxg;
+        a.src_ghosts = p.mbc;
+        hipError_t e = hipMemcpy(auxg, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess && launch_lxw_flux1d(a, dt, p.time_order, Fd, c->st)) e = hipErrorLaunchFailure;
+        if (e == hipSuccess) e = hipMemcpyAsync(hF.data(), Fd, hF.size() * sizeof(double), hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        hipFree(auxg);
+        hipFree(Fd);
+        if (e != hipSuccess) return fail(c, FINESS_ERR_HIP, hipGetErrorString(e));
+        for (int i = -3; i < p.mx + 3; i++) F[(size_t)(i + p.mbc) * c->M] = hF[i + 3];
+        return FINESS_OK;
+    }
+    if (!G) return FINESS_ERR_ARG;
+    if (p.physics == FINESS_PHYS_ADVECTION) return fail(c, FINESS_ERR_CONFIG, "2D LxW needs Euler or shallow water");
     if ((rc = upload_ghosted(c, q))) return rc;
     double *Fd = c->lxwF, *Gd = c->lxwG;
     bool own = false;

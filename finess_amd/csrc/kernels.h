@@ -20,6 +20,8 @@ int launch_stage2d_wide(int physics, int wk, const StageArgs& a, hipStream_t st)
 int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
 // The stages of one 1D time step (one launch up to 8192 cells).
 int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st);
+// 1D Lax-Wendroff time-integrated fluxes of the advection flux on [-3, mx+2].
+int launch_lxw_flux1d(const Stage1DArgs& a, double dt, int time_order, double* F, hipStream_t st);
 // Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].
 int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st);
 

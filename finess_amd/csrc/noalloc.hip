@@ -90,6 +90,8 @@ extern "C" int finess_construct_l_planes_noalloc(const finess_params* pin, const
     // the fused fifth-order stage kernel only: the other orders keep their
     // interface fluxes in grid-sized buffers, and routing keeps a record
     if (p.ndims != 2 || p.space_order != 5 || p.routing != FINESS_ROUTE_NONE) return FINESS_ERR_CONFIG;
+    // the bathymetry source term reads aux planes, which this entry point does not take
+    if (p.source_term && p.source_kind == FINESS_SRC_SW_BATHYMETRY) return FINESS_ERR_CONFIG;
     // face and cell indices of the stage kernel are 32-bit
     if ((long long)(p.mx + 1) * (p.my + 1) >= (1ll << 31)) return FINESS_ERR_CONFIG;
     int dev = 0;
@@ -119,6 +121,8 @@ extern "C" int finess_construct_l_planes_noalloc(const finess_params* pin, const
         a.gspread = reinterpret_cast<unsigned long long*>(s + OFF_GSPREAD);
         a.galpha = reinterpret_cast<double*>(s + OFF_GALPHA);
     }
+    a.src_kind = source_arg(p.source_term, p.source_kind);
+    a.src_coef = p.source_coef;
     a.src = q;
     a.qa = q;
     a.a_is_src = 1;

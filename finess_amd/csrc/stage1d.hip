@@ -143,6 +143,19 @@ static int launch_step_wk(const Stage1DStep& s, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// Time-integrated fluxes of the 1D Lax-Wendroff step on the extended cells.
+__global__ __launch_bounds__(256) void k_lxw_flux1d(Stage1DArgs a, double dt, int time_order, double* F)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.mx + 6) F[e] = integrated_f1d(e - 3, a, dt, time_order);
+}
+
+int launch_lxw_flux1d(const Stage1DArgs& a, double dt, int time_order, double* F, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lxw_flux1d, dim3((a.mx + 6 + 255) / 256), dim3(256), 0, st, a, dt, time_order, F);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 template <class F>
 static int with_kind(int wk, F f)
 {

@@ -1,6 +1,7 @@
 // stage1d_body.h -- per-face and per-cell work of the 1D stage (see
 // stage1d.hip); plain functions so that a CPU harness can run them too.
 #pragma once
+#include "lxw_body.h"
 #include "physics.h"
 #include "stage2d_body.h"
 #include "step_control.h"
@@ -30,6 +31,13 @@ struct Stage1DArgs {
     const double* dtp;
     double beta;
     const unsigned long long* halt;
+    // source term (SRC_OFF: none): psi of the source's cell added to L
+    int src_kind;
+    double src_coef;
+    // Lax-Wendroff (null: off): the time-integrated fluxes of the extended
+    // cells [-3, mx+2], cell i at lxwF[i + 3], read as the stencil fluxes
+    // (lib/1d/ConstructLxWL.cpp:14-237)
+    const double* lxwF;
 };
 
 __host__ __device__ __forceinline__ bool halted1d(const Stage1DArgs& a) { return a.halt && *a.halt != 0ull; }
@@ -69,7 +77,7 @@ __host__ __device__ __forceinline__ double face1d(int f, const Stage1DArgs& a)
     for (int s = 0; s < 6; s++) {
         const long long k = idx1d(a, f - 3 + s);
         q[s] = a.src[k];
-        fl[s] = q[s] * a.aux[k];  // FluxFunc: Q(i,1)*Aux(i,1)
+        fl[s] = a.lxwF ? a.lxwF[f + s] : q[s] * a.aux[k];  // FluxFunc: Q(i,1)*Aux(i,1)
     }
     const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
     const double s1 = rmin(al, 0.5 * (al + ar));
@@ -122,9 +130,35 @@ __host__ __device__ __forceinline__ double face1d_any(int f, const Stage1DArgs& 
     else return face1d_wide<wide_radius(WK), wide_version(WK)>(f, a);
 }
 
+// ConstructIntegratedF (lib/1d/ConstructIntegratedF.cpp:54-212) with the
+// advection callbacks (FluxFunc aux(1) q, DFluxFunc aux(1), D2FluxFunc 0):
+// F = f + dt/2 (f_t + dt/3 f_tt), f_t = -a f_x, f_tt = a (a f_xx) for
+// time_order > 2, with the 5-point differences Diff1 / Diff2; cell i of the
+// extended range [-3, mx+2].
+__host__ __device__ __forceinline__ double integrated_f1d(int i, const Stage1DArgs& a, double dt, int time_order)
+{
+    double fv[5];
+    for (int s = 0; s < 5; s++) {
+        const long long k = idx1d(a, i - 2 + s);
+        fv[s] = a.src[k] * a.aux[k];
+    }
+    const double fx = d1(a.dx, fv[0], fv[1], fv[3], fv[4]);
+    const double fxx = d2(a.dx, fv[0], fv[1], fv[2], fv[3], fv[4]);
+    const double A = a.aux[idx1d(a, i)];
+    const double ft = -A * fx;
+    const double ftt = time_order > 2 ? A * (A * fxx) : 0.0;
+    return fv[2] + 0.5 * dt * (ft + dt / 3.0 * ftt);
+}
+
 __host__ __device__ __forceinline__ void update1d(int i, const Stage1DArgs& a)
 {
-    const double L = -(a.fhat[i + 1] - a.fhat[i]) / a.dx;
+    double L = -(a.fhat[i + 1] - a.fhat[i]) / a.dx;
+    if (a.src_kind != SRC_OFF) {  // Lstar = psi + tmp (lib/1d/ConstructL.cpp:194-205)
+        const double q = a.src[idx1d(a, i)];
+        double psi;
+        source_psi<1>(a.src_kind, a.src_coef, &q, nullptr, 0, &psi);
+        L = psi + L;
+    }
     if (a.Lout) a.Lout[i] = L;
     if (!a.out) return;
     double v = a.a1 * a.qa[i];

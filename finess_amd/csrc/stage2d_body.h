@@ -25,6 +25,7 @@
 // can run the very same code one thread at a time.
 #pragma once
 #include "physics.h"
+#include "source.h"
 #include "weno.h"
 
 namespace finess {
@@ -148,6 +149,11 @@ struct StageArgs {
     // launch_stage2d through the [2][CFL_SPREAD] words of gspread
     double* galpha;
     unsigned long long* gspread;
+    // source term (src_kind SRC_OFF: none): psi of the stage source's cell
+    // is added to L; aux planes (interior cells, `plane` apart per field)
+    int src_kind;
+    double src_coef;
+    const double* aux;
 };
 
 // beta * dt of the stage update (dt from the device controller when set)
@@ -447,8 +453,21 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
     const bool exact_lds = sizeof(R) == sizeof(double) && !a.src_ghosts;
     const double idy_ = a.idy;
     const double bdt = stage_bdt(a);
+    double psi[M];
+    if (a.src_kind != SRC_OFF) {
+        // the stage source's state of the cell: exact in an FP64 tile, else
+        // the FP64 operand when the source is one (a narrowed record otherwise)
+        double qs[M];
+        for (int m = 0; m < M; m++) {
+            qs[m] = double(sm.cv[m][c]);
+            if (sizeof(R) != sizeof(double) && a.out && a.a_is_src) qs[m] = pa[m];
+            else if (sizeof(R) != sizeof(double) && a.out && a.use_b && a.b_is_src) qs[m] = pb[m];
+        }
+        source_psi<M>(a.src_kind, a.src_coef, qs, a.aux ? a.aux + idx : nullptr, a.plane, psi);
+    }
     for (int m = 0; m < M; m++) {
-        const double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
+        double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
+        if (a.src_kind != SRC_OFF) L = psi[m] + L;  // Lstar = psi + tmp (lib/2d/ConstructL.cpp:378-386)
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
         if (!a.out) continue;
         const double qa = (exact_lds && a.a_is_src) ? double(sm.cv[m][c]) : pa[m];

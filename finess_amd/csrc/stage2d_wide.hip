@@ -164,9 +164,10 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
 
 // L = -(F(i+1) - F(i))/dx - (G(j+1) - G(j))/dy (lib/2d/ConstructL.cpp:
 // 388-399) and the stage update out = a1 qa + a2 qb + beta dt L.
-template <int M>
+template <class P>
 __global__ __launch_bounds__(256) void k_wide_update(StageArgs a)
 {
+    constexpr int M = P::MEQN;
     if (stage_halted(a)) return;
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= a.plane) return;
@@ -174,9 +175,16 @@ __global__ __launch_bounds__(256) void k_wide_update(StageArgs a)
     const long long nfx = (long long)(a.mx + 1) * a.my, nfy = (long long)a.mx * (a.my + 1);
     const long long fx0 = (long long)i * a.my + j, fy0 = (long long)i * (a.my + 1) + j;
     const double bdt = a.out ? stage_bdt(a) : 0.0;
+    double psi[M];
+    if (a.src_kind != SRC_OFF) {
+        double qs[M];
+        load_state<P>(a, i, j, qs);
+        source_psi<M>(a.src_kind, a.src_coef, qs, a.aux ? a.aux + c : nullptr, a.plane, psi);
+    }
     for (int m = 0; m < M; m++) {
         const double Lx = -(a.face_x[m * nfx + fx0 + a.my] - a.face_x[m * nfx + fx0]) * a.idx;
-        const double L = Lx - (a.face_y[m * nfy + fy0 + 1] - a.face_y[m * nfy + fy0]) * a.idy;
+        double L = Lx - (a.face_y[m * nfy + fy0 + 1] - a.face_y[m * nfy + fy0]) * a.idy;
+        if (a.src_kind != SRC_OFF) L = psi[m] + L;  // Lstar = psi + tmp (lib/2d/ConstructL.cpp:378-386)
         if (a.Lout) a.Lout[m * a.plane + c] = L;
         if (!a.out) continue;
         double v = a.a1 * a.qa[m * a.plane + c];
@@ -192,7 +200,7 @@ static int launch_wide_t(const StageArgs& a, hipStream_t st)
     const unsigned by = (unsigned)wide_tiles_i(1, a.mx) * wide_tiles_j(1, a.my);
     hipLaunchKernelGGL((k_wide_faces<P, R, VER, 0>), dim3(bx), dim3(256), 0, st, a);
     hipLaunchKernelGGL((k_wide_faces<P, R, VER, 1>), dim3(by), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((k_wide_update<P::MEQN>), dim3((unsigned)((a.plane + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_wide_update<P>), dim3((unsigned)((a.plane + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

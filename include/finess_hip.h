@@ -60,6 +60,17 @@ extern "C" {
 #define FINESS_BC_PERIODIC 1
 #define FINESS_BC_REFLECT  2
 
+/* device forms of the app's SourceTermFunc (used when source_term = 1):
+ *   ZERO           psi = 0: the SourceTermFunc of every reference app except
+ *                  shallow water's (e.g. apps/2d/euler/mach_reflection/SourceTermFunc.cpp)
+ *   SW_BATHYMETRY  psi = (0, -h aux(2), -h aux(3)), aux = (b, b_x, b_y), maux >= 3:
+ *                  apps/2d/shallow_water/riemann_test/SourceTermFunc.cpp
+ *   DAMPING        psi = -source_coef * q, every field (no reference app uses
+ *                  it: a linear relaxation term with a closed-form solution)     */
+#define FINESS_SRC_ZERO          0
+#define FINESS_SRC_SW_BATHYMETRY 1
+#define FINESS_SRC_DAMPING       2
+
 /* stencil arithmetic of the device path */
 #define FINESS_PREC_FP64 0   /* plain form, FP64 end to end                       */
 #define FINESS_PREC_FP32 1   /* FP32 stencil, FP64 stage accumulation              */
@@ -85,7 +96,7 @@ typedef struct finess_params {
                                            GetWenoReconstruct (lib/WenoReconstruct.cpp:39-73);
                                            orders other than 5: Runge-Kutta and ConstructL in
                                            FP64, mbc >= (space_order + 1) / 2 */
-    int    time_order;                  /* RK: 1..5 in 2D, 1..3 in 1D; LxW: 2..3; PIF: 4 */
+    int    time_order;                  /* RK: 1..5 (1D and 2D); LxW: 2..3; PIF: 4 */
     int    time_stepping_method;        /* FINESS_TS_* */
     double initial_dt, max_dt, desired_cfl, max_cfl;
     int    nv;
@@ -93,7 +104,15 @@ typedef struct finess_params {
                                            alpha = Max(global, local) at every interface
                                            (lib/2d/ConstructL.cpp:48-54,172,323); not with routing
                                            or the batched mode */
-    int    source_term;                 /* only 0 */
+    int    source_term;                 /* 1: the SourceTermFunc values psi(q, aux) of the
+                                           stage's state are added to L
+                                           (lib/2d/ConstructL.cpp:371-387,
+                                           lib/1d/ConstructL.cpp:194-205), psi being the
+                                           device form named by source_kind; Runge-Kutta
+                                           (1D, 2D, every order) and ConstructL.  Lax-Wendroff
+                                           refuses it as the reference does
+                                           (lib/2d/ConstructLxWL.cpp:376-380); RK4-PIF, routing
+                                           and the batched mode refuse it too */
     int    verbosity;
     /* [weno] (python/finess/params/weno.py:15-51) */
     int    weno_version;                /* FINESS_WENO_* */
@@ -121,6 +140,8 @@ typedef struct finess_params {
                                            Lax-Friedrichs fluxes so that density and pressure stay
                                            positive (lib/2d/FinSolveLxW.cpp:126-157); Runge-Kutta
                                            ignores it, as the reference does */
+    int    source_kind;                 /* FINESS_SRC_*: the device form of SourceTermFunc */
+    double source_coef;                 /* k of FINESS_SRC_DAMPING */
 } finess_params;
 
 /* Fills defaults exactly as the reference's IniParams defaults do. */
@@ -160,7 +181,8 @@ int finess_construct_l(finess_ctx* ctx, const double* q, const double* aux,
 int finess_fin_solve_rk(finess_ctx* ctx, double* q, const double* aux,
                         double* t, double tend, double* dtv);
 
-/* FinSolveLxW (lib/2d/FinSolveLxW.cpp:16-245), same conventions. */
+/* FinSolveLxW (lib/2d/FinSolveLxW.cpp:16-245, lib/1d/FinSolveLxW.cpp:20-218),
+ * same conventions. */
 int finess_fin_solve_lxw(finess_ctx* ctx, double* q, const double* aux,
                          double* t, double tend, double* dtv);
 
@@ -170,7 +192,9 @@ int finess_fin_solve_rk_pif(finess_ctx* ctx, double* q, const double* aux,
                             double* t, double tend, double* dtv);
 
 /* ConstructIntegratedR (lib/2d/ConstructIntegratedR.cpp:59-296): time-
- * integrated fluxes F, G (dTensorBC3(mx,my,meqn,mbc)) on cells [1-3, m+3]. */
+ * integrated fluxes F, G (dTensorBC3(mx,my,meqn,mbc)) on cells [1-3, m+3].
+ * 1D: ConstructIntegratedF (lib/1d/ConstructIntegratedF.cpp:54-212), F only
+ * (dTensorBC2(mx,meqn,mbc), cells [1-3, mx+3]); G may be NULL. */
 int finess_construct_integrated_r(finess_ctx* ctx, double dt, const double* q,
                                   const double* aux, double* F, double* G);
 
@@ -299,6 +323,26 @@ int  finess_batch_autotune(finess_batch* b, int trial_launches, int* tile_i, int
 /* Advances every problem by `steps` accepted steps (rejected ones are retried
  * with the controller's smaller dt); *seconds = wall time of the run. */
 int  finess_batch_run(finess_batch* b, int steps, double* seconds);
+
+/* ---- the device physics, evaluated on the host ----------------------------
+ * No counterpart in the reference.  The kernels carry the app callbacks as
+ * compiled device forms; these entry points evaluate the very same code on
+ * the host, so that the drop-in layer can check an app's own callbacks
+ * against them before it runs the app on the device (no GPU needed).
+ *   finess_physics_eig         the matrices of ProjectLeftEig / ProjectRightEig
+ *                              (apps/2d/euler/lib/ProjectLeftEig.cpp:9-67,
+ *                              ProjectRightEig.cpp:9-63; shallow water alike) at the
+ *                              mean state qavg, direction ixy (1: x, 2: y):
+ *                              Lmat, Rmat are meqn x meqn, row-major, column j the
+ *                              projection of the unit vector e_j
+ *   finess_physics_wave_speed  max(|s1|, |s2|) of SetWaveSpd
+ *                              (apps/2d/euler/lib/SetWaveSpd.cpp:11-69) for the
+ *                              left and right states; 1D advection reads aux
+ *   finess_physics_source      psi of one point for p->source_kind (SourceTermFunc) */
+int finess_physics_eig(const finess_params* p, int ixy, const double* qavg, double* Lmat, double* Rmat);
+int finess_physics_wave_speed(const finess_params* p, int ixy, const double* ql, const double* qr,
+                              const double* auxl, const double* auxr, double* speed);
+int finess_physics_source(const finess_params* p, const double* q, const double* aux, double* psi);
 
 /* ---- library info --------------------------------------------------------- */
 const char* finess_version(void);

@@ -663,8 +663,28 @@ static int check_config(const finess_params* p)
     if (!orc_weno_supported(p->weno_version, p->space_order)) return FINESS_ERR_CONFIG;
     if (p->mbc < 3 || p->mbc < (p->space_order + 1) / 2) return FINESS_ERR_CONFIG;
     if (p->meqn > MAXEQ) return FINESS_ERR_CONFIG;
-    if (p->source_term) return FINESS_ERR_CONFIG;
+    if (p->source_term) {
+        if (p->source_kind < FINESS_SRC_ZERO || p->source_kind > FINESS_SRC_DAMPING) return FINESS_ERR_CONFIG;
+        if (p->source_kind == FINESS_SRC_SW_BATHYMETRY && (p->physics != FINESS_PHYS_SHALLOW_WATER || p->maux < 3))
+            return FINESS_ERR_CONFIG;
+    }
     return FINESS_OK;
+}
+
+/* SourceTermFunc at one point, in the forms the apps define: psi = 0 (e.g.
+ * apps/2d/euler/mach_reflection/SourceTermFunc.cpp), the bathymetry term
+ * psi = (0, -h aux(2), -h aux(3)) of apps/2d/shallow_water/riemann_test/
+ * SourceTermFunc.cpp, or the relaxation psi = -k q. */
+static void source_term_func(const finess_params* p, const double* q, const double* aux, double* psi)
+{
+    for (int m = 0; m < p->meqn; m++) psi[m] = 0.0;
+    if (p->source_kind == FINESS_SRC_DAMPING) {
+        for (int m = 0; m < p->meqn; m++) psi[m] = -p->source_coef * q[m];
+    } else if (p->source_kind == FINESS_SRC_SW_BATHYMETRY) {
+        const double h = q[0], bx = aux[1], by = aux[2];
+        psi[1] = -h * bx;
+        psi[2] = -h * by;
+    }
 }
 
 static int check_config5(const finess_params* p)
@@ -786,15 +806,26 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
             for (int m = 1; m <= meqn; m++) Ghat[ix3(i, j, m, my + 1, meqn, mbc)] = gh[m - 1];
         }
 
-    /* Lstar = -(Fhat(i+1)-Fhat(i))/dx - (Ghat(j+1)-Ghat(j))/dy, :388-399 */
+    /* Lstar = -(Fhat(i+1)-Fhat(i))/dx - (Ghat(j+1)-Ghat(j))/dy, :388-399; with
+     * source_term the sampled psi first, then Lstar = psi + tmp (:371-387).
+     * ConstructLxWL exits on a source term (lib/2d/ConstructLxWL.cpp:376-380). */
+    if (p->source_term && (Fsrc || Gsrc)) {
+        free(Fhat);
+        free(Ghat);
+        return FINESS_ERR_CONFIG;
+    }
 #pragma omp parallel for schedule(static)
     for (int i = 1; i <= mx; i++)
-        for (int j = 1; j <= my; j++)
+        for (int j = 1; j <= my; j++) {
+            double psi[MAXEQ];
+            if (p->source_term)
+                source_term_func(p, &q[ix3(i, j, 1, my, meqn, mbc)], &aux[ix3(i, j, 1, my, na, mbc)], psi);
             for (int m = 1; m <= meqn; m++) {
                 double tmp = -(Fhat[ix3(i + 1, j, m, my, meqn, mbc)] - Fhat[ix3(i, j, m, my, meqn, mbc)]) / dx;
                 tmp = tmp - (Ghat[ix3(i, j + 1, m, my + 1, meqn, mbc)] - Ghat[ix3(i, j, m, my + 1, meqn, mbc)]) / dy;
-                Lstar[ix3(i, j, m, my, meqn, mbc)] = tmp;
+                Lstar[ix3(i, j, m, my, meqn, mbc)] = p->source_term ? psi[m - 1] + tmp : tmp;
             }
+        }
     if (Fhat_out) memcpy(Fhat_out, Fhat, (size_t)(mx + 1 + 2 * mbc) * (my + 2 * mbc) * meqn * sizeof(double));
     if (Ghat_out) memcpy(Ghat_out, Ghat, (size_t)(mx + 2 * mbc) * (my + 1 + 2 * mbc) * meqn * sizeof(double));
     free(Fhat);
@@ -802,10 +833,13 @@ static int construct_l_2d_faces(const finess_params* p, const double* q, const d
     return FINESS_OK;
 }
 
-/* 1D ConstructL, lib/1d/ConstructL.cpp:16-224; smax(i) is overwritten (:149). */
-static int construct_l_1d(const finess_params* p, const double* q, const double* aux,
+/* 1D ConstructL, lib/1d/ConstructL.cpp:16-224; smax(i) is overwritten (:149).
+ * With Fsrc non-NULL the stencil fluxes are read from it: the 1D ConstructLxWL
+ * (lib/1d/ConstructLxWL.cpp:14-237), which exits on a source term (:215-219). */
+static int construct_l_1d(const finess_params* p, const double* q, const double* aux, const double* Fsrc,
                           double* Lstar, double* smax)
 {
+    if (Fsrc && p->source_term) return FINESS_ERR_CONFIG;
     const int mx = p->mx, meqn = p->meqn, mbc = p->mbc, na = naux(p);
     const double dx = p->dx;
     double* fhat = (double*)calloc((size_t)(mx + 1 + 2 * mbc) * meqn, sizeof(double));
@@ -827,7 +861,10 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
         for (int s = 1; s <= ns; s++) {
             const int is = i - 1 + s - r;
             for (int m = 1; m <= meqn; m++) qs[s - 1][m - 1] = q[ix2(is, m, meqn, mbc)];
-            orc_flux(p, qs[s - 1], &aux[ix2(is, 1, na, mbc)], fs[s - 1], NULL);
+            if (Fsrc)
+                for (int m = 1; m <= meqn; m++) fs[s - 1][m - 1] = Fsrc[ix2(is, m, meqn, mbc)];
+            else
+                orc_flux(p, qs[s - 1], &aux[ix2(is, 1, na, mbc)], fs[s - 1], NULL);
         }
         double fh[MAXEQ];
         const double alpha = interface_flux(p, 1, qs, fs, &aux[ix2(i - 1, 1, na, mbc)],
@@ -835,9 +872,15 @@ static int construct_l_1d(const finess_params* p, const double* q, const double*
         smax[i + mbc - 1] = alpha;
         for (int m = 1; m <= meqn; m++) fhat[ix2(i, m, meqn, mbc)] = fh[m - 1];
     }
-    for (int i = 1; i <= mx; i++)
-        for (int m = 1; m <= meqn; m++)
-            Lstar[ix2(i, m, meqn, mbc)] = -(fhat[ix2(i + 1, m, meqn, mbc)] - fhat[ix2(i, m, meqn, mbc)]) / dx;
+    /* with source_term: Lstar = psi + tmp (lib/1d/ConstructL.cpp:194-205) */
+    for (int i = 1; i <= mx; i++) {
+        double psi[MAXEQ];
+        if (p->source_term) source_term_func(p, &q[ix2(i, 1, meqn, mbc)], &aux[ix2(i, 1, na, mbc)], psi);
+        for (int m = 1; m <= meqn; m++) {
+            const double tmp = -(fhat[ix2(i + 1, m, meqn, mbc)] - fhat[ix2(i, m, meqn, mbc)]) / dx;
+            Lstar[ix2(i, m, meqn, mbc)] = p->source_term ? psi[m - 1] + tmp : tmp;
+        }
+    }
     free(fhat);
     return FINESS_OK;
 }
@@ -846,7 +889,7 @@ int orc_construct_l(const finess_params* p, const double* q, const double* aux, 
 {
     const int rc = check_config(p);
     if (rc) return rc;
-    if (p->ndims == 1) return construct_l_1d(p, q, aux, Lstar, smax);
+    if (p->ndims == 1) return construct_l_1d(p, q, aux, NULL, Lstar, smax);
     return construct_l_2d(p, q, aux, NULL, NULL, Lstar, smax);
 }
 
@@ -1230,11 +1273,24 @@ int orc_lxw_mpp_faces(const finess_params* p, double dt, const double* q, const 
 
 /* One LxW step, lib/2d/FinSolveLxW.cpp:122-171; with mpp_limiter the
  * limited fluxes update the interior cells, x then y (:126-157). */
+static int construct_integrated_f_1d(const finess_params* p, double dt, const double* q, const double* aux,
+                                     double* F);
+
 static int lxw_step(const finess_params* p, double* qnew, double* aux, double* F, double* G,
                     double* L, double* smax, double dt)
 {
     const size_t numel = state_numel(p, p->meqn);
     int rc;
+    if (p->ndims == 1) {
+        /* lib/1d/FinSolveLxW.cpp:100-142 without the limiter: SetBndValues,
+         * ConstructIntegratedF, ConstructLxWL, q = q + dt L on every entry */
+        if (p->mpp_limiter) return FINESS_ERR_CONFIG;
+        orc_set_bnd(p, qnew, aux);
+        if ((rc = construct_integrated_f_1d(p, dt, qnew, aux, F))) return rc;
+        if ((rc = construct_l_1d(p, qnew, aux, F, L, smax))) return rc;
+        for (long long k = 0; k < (long long)numel; k++) qnew[k] = qnew[k] + dt * L[k];
+        return FINESS_OK;
+    }
     orc_set_bnd(p, qnew, aux);
     if ((rc = orc_construct_integrated_r(p, dt, qnew, aux, F, G))) return rc;
     if (p->mpp_limiter) {
@@ -1420,8 +1476,11 @@ static int any_step(const finess_params* p, int ts, double* q, double* qstar, do
 static int check_stepper(const finess_params* p, int ts)
 {
     if (ts != FINESS_TS_RK && p->space_order != 5) return FINESS_ERR_CONFIG;
-    if (ts == FINESS_TS_LXW && (p->ndims != 2 || p->mbc < 5)) return FINESS_ERR_CONFIG;
+    if (ts == FINESS_TS_LXW && p->mbc < 5) return FINESS_ERR_CONFIG;
     if (ts == FINESS_TS_PIF && (p->ndims != 2 || p->time_order != 4)) return FINESS_ERR_CONFIG;
+    /* source terms are restated for Runge-Kutta and ConstructL only (LxW: the
+     * reference exits, lib/2d/ConstructLxWL.cpp:376-380) */
+    if (ts != FINESS_TS_RK && p->source_term) return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -1551,11 +1610,49 @@ static double diff2(double dx, const double* f)
 
 /* ConstructIntegratedR (one-stage overload), lib/2d/ConstructIntegratedR.cpp:
  * 59-296: F = f + dt/2 (f_t + dt/3 f_tt) on cells [1-3, m+3]^2. */
+/* 1D ConstructIntegratedF (lib/1d/ConstructIntegratedF.cpp:54-212):
+ * F = f + dt/2 (f_t + dt/3 f_tt) on cells [1-3, mx+3] with f_t = -A f_x and,
+ * for time_order > 2, f_tt = H(f_x, f_x) + A (H(q_x, f_x) + A f_xx); A and H
+ * are the derivatives of FluxFunc (the advection flux aux(1) q: A = aux(1),
+ * H = 0, apps/1d/advection/lib/{DFluxFunc,D2FluxFunc}.cpp). */
+static int construct_integrated_f_1d(const finess_params* p, double dt, const double* q, const double* aux,
+                                     double* F)
+{
+    const int mx = p->mx, meqn = p->meqn, mbc = p->mbc, na = naux(p);
+    const double dx = p->dx;
+    if (mbc < 5 || meqn != 1 || p->physics != FINESS_PHYS_ADVECTION) return FINESS_ERR_CONFIG;
+    const int mbc_small = 3, half = 3, npts = 5;
+    for (int i = 1 - mbc_small; i <= mx + mbc_small; i++) {
+        double Fv[5], qv[5];
+        int s = -half + 1;
+        for (int r = 0; r < npts; r++, s++) {
+            orc_flux(p, &q[ix2(i + s, 1, meqn, mbc)], &aux[ix2(i + s, 1, na, mbc)], &Fv[r], NULL);
+            qv[r] = q[ix2(i + s, 1, meqn, mbc)];
+        }
+        const double fx = diff1(dx, Fv), qx = diff1(dx, qv), fxx = diff2(dx, Fv);
+        (void)qx;  /* enters only through H, which is zero */
+        const double A = aux[ix2(i, 1, na, mbc)];
+        double ft = 0.;
+        ft += -A * fx;
+        double ftt = 0.;
+        if (p->time_order > 2) {
+            double tmp_vec = 0.;   /* sum of H q_x f_x terms: H = 0 */
+            tmp_vec = tmp_vec + (0. + A * fxx);
+            ftt = ftt + (0. + A * tmp_vec);
+        }
+        double f;
+        orc_flux(p, &q[ix2(i, 1, meqn, mbc)], &aux[ix2(i, 1, na, mbc)], &f, NULL);
+        F[ix2(i, 1, meqn, mbc)] = f + 0.5 * dt * (ft + dt / 3.0 * ftt);
+    }
+    return FINESS_OK;
+}
+
 int orc_construct_integrated_r(const finess_params* p, double dt, const double* q, const double* aux,
                                double* F, double* G)
 {
     const int mx = p->mx, my = p->my, meqn = p->meqn, mbc = p->mbc, na = naux(p);
     const double dx = p->dx, dy = p->dy;
+    if (p->ndims == 1) return construct_integrated_f_1d(p, dt, q, aux, F);
     if (mbc < 5 || p->ndims != 2) return FINESS_ERR_CONFIG;
     /* R = SampleFunction(FluxFunc) on all cells incl. ghosts, :83-84 */
     const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);

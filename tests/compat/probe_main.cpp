@@ -3,7 +3,8 @@
 // oracle's restatement of the reference callbacks, a SetBndValues chosen on
 // the command line, and prints what IniParams parsed and what the probes
 // detected.  No device call is made.
-//   probe_main <parameters.ini> <bc: extrap|periodic|mixed|timedep> <physics: euler|sw|bogus>
+//   probe_main <parameters.ini> <bc: extrap|periodic|mixed|timedep>
+//              <physics: euler|euler_rescaled|sw|sw_hr|bogus>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -20,7 +21,7 @@ static finess_params orc_params()
 {
     finess_params p;
     finess_params_default(&p);
-    p.physics = g_phys == "sw" ? FINESS_PHYS_SHALLOW_WATER : FINESS_PHYS_EULER;
+    p.physics = (g_phys == "sw" || g_phys == "sw_hr") ? FINESS_PHYS_SHALLOW_WATER : FINESS_PHYS_EULER;
     p.meqn = global_ini_params.get_meqn();
     p.gamma = global_ini_params.get_gamma();
     p.sw_compat_hl = 1;
@@ -42,6 +43,9 @@ void FluxFunc(const dTensor2& xpts, const dTensor2& Q, const dTensor2& Aux, dTen
     }
 }
 
+// euler_rescaled: the first characteristic field scaled by 2 (left row) and
+// 1/2 (right column) -- still a pair of inverses, but not the device's
+// eigenvectors, which the probe must refuse
 void ProjectLeftEig(int ixy, const dTensor1&, const dTensor1& Qa, const dTensor2& Qv, dTensor2& Wv)
 {
     finess_params p = orc_params();
@@ -49,6 +53,8 @@ void ProjectLeftEig(int ixy, const dTensor1&, const dTensor1& Qa, const dTensor2
     double qa[5];
     for (int m = 1; m <= M; m++) qa[m - 1] = Qa.get(m);
     orc_project_left(&p, ixy, qa, n, Qv.data(), Wv.data());
+    if (g_phys == "euler_rescaled")
+        for (int k = 1; k <= n; k++) Wv.set(1, k, 2.0 * Wv.get(1, k));
 }
 
 void ProjectRightEig(int ixy, const dTensor1&, const dTensor1& Qa, const dTensor2& Wv, dTensor2& Qv)
@@ -57,7 +63,20 @@ void ProjectRightEig(int ixy, const dTensor1&, const dTensor1& Qa, const dTensor
     const int M = Wv.getsize(1), n = Wv.getsize(2);
     double qa[5];
     for (int m = 1; m <= M; m++) qa[m - 1] = Qa.get(m);
-    orc_project_right(&p, ixy, qa, n, Wv.data(), Qv.data());
+    dTensor2 W(M, n);
+    for (int m = 1; m <= M; m++)
+        for (int k = 1; k <= n; k++) W.set(m, k, (g_phys == "euler_rescaled" && m == 1 ? 0.5 : 1.0) * Wv.get(m, k));
+    orc_project_right(&p, ixy, qa, n, W.data(), Qv.data());
+}
+
+// sw_hr: the shallow-water wave speeds with the right momenta divided by h_r
+void SetWaveSpd(const dTensor1& nvec, const dTensor1&, const dTensor1& Ql, const dTensor1& Qr, const dTensor1& Al,
+                const dTensor1& Ar, double& s1, double& s2)
+{
+    finess_params p = orc_params();
+    p.sw_compat_hl = g_phys == "sw_hr" ? 0 : 1;
+    const int ixy = nvec.get(1) != 0.0 ? 1 : 2;
+    orc_wave_speed(&p, ixy, Ql.data(), Qr.data(), Al.data(), Ar.data(), &s1, &s2);
 }
 
 void SetBndValues(StateVars& Q)
@@ -86,8 +105,8 @@ int main(int argc, char* argv[])
                (int)ini.get_time_stepping_method(), (int)ini.get_weno_version(), ini.get_epsilon(),
                ini.get_alpha_scaling(), ini.get_gamma());
         finess_dropin::ProbeReport r = finess_dropin::probe_app();
-        printf("probe physics=%d checked=%d bc=%d,%d,%d,%d bc_checked=%d\n", r.physics, (int)r.physics_checked,
-               r.bc[0], r.bc[1], r.bc[2], r.bc[3], (int)r.bc_checked);
+        printf("probe physics=%d checked=%d bc=%d,%d,%d,%d bc_checked=%d sw_compat_hl=%d\n", r.physics,
+               (int)r.physics_checked, r.bc[0], r.bc[1], r.bc[2], r.bc[3], (int)r.bc_checked, r.sw_compat_hl);
     } catch (const std::logic_error& e) {
         printf("logic_error: %s\n", e.what());
         return 3;

@@ -72,3 +72,23 @@ def test_ini_grammar(harness, tmp_path):
     f.write_text(open(INI).read().replace("nv          = 500000\n", ""))
     rc, out = run(harness, str(f))
     assert rc == 4 and "finess.nv is required" in out, out
+
+
+def test_probe_rejects_rescaled_eigenvectors(harness):
+    """Eigenvector projections that are inverses of each other but scaled
+    differently from the device's change the WENO result: refused."""
+    rc, out = run(harness, INI, "extrap", "euler_rescaled")
+    assert rc == 3 and "ProjectLeftEig differs from the device's left eigenvectors" in out, out
+
+
+@pytest.mark.parametrize("phys,hl", [("sw", 1), ("sw_hr", 0)])
+def test_probe_detects_the_shallow_water_wave_speed_form(harness, tmp_path, phys, hl):
+    """SetWaveSpd dividing the right momenta by h_l (the reference's own,
+    apps/2d/shallow_water/lib/SetWaveSpd.cpp:27-28) or by h_r: the probe
+    finds which one the app has."""
+    f = tmp_path / "sw.ini"
+    f.write_text(open(INI).read().replace("meqn        = 5", "meqn        = 3").replace("physics   = euler",
+                                                                                        "physics   = shallow_water"))
+    rc, out = run(harness, str(f), "periodic", phys)
+    assert rc == 0, out
+    assert "probe physics=2 checked=1 bc=1,1,1,1 bc_checked=1 sw_compat_hl=%d" % hl in out, out

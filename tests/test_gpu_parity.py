@@ -213,24 +213,33 @@ def test_fin_solve_rk_reject_path():
     assert np.all(err <= 1e-10), err
 
 
-def test_fin_solve_rk_advection_full_run():
-    """Workload 1: mx=200, t_final=1, the full run (500 steps at CFL 0.4)."""
-    p, q, aux = problems.advection_1d(mx=200)
+@pytest.mark.parametrize("time_order", [1, 2, 3])
+def test_fin_solve_rk_advection_full_run(time_order):
+    """Workload 1: mx=200, t_final=1, the full run (500 steps at CFL 0.4),
+    with each SSP scheme the fused 1D step kernel carries (time_order 3 is
+    the workload's; 1 and 2 pin that kernel's other coefficient rows).
+    Forward Euler (time_order 1) with WENO5 amplifies rounding differences
+    exponentially: the oracle itself moves by 1e-13 at t = 0.1 and by 5e-5
+    at t = 0.5 under 1e-15 relative input noise, so that scheme is compared
+    over its first 50 steps (t = 0.1)."""
+    p, q, aux = problems.advection_1d(mx=200, time_order=time_order)
+    tend = 0.1 if time_order == 1 else 1.0
     q0 = q.copy()
     dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
-    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, 1.0, dtv0, aux=aux)
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, tend, dtv0, aux=aux)
     assert rc == 0
     q1 = q.copy()
     dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
     with Solver(p) as s:
-        t1 = s.fin_solve_rk(q1, 0.0, 1.0, dtv1, aux=aux)
+        t1 = s.fin_solve_rk(q1, 0.0, tend, dtv1, aux=aux)
     assert abs(t1 - t0) <= 1e-14
     err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
-    exact = np.sin(2 * np.pi * (p.xlow + (np.arange(p.mx) + 0.5) * p.dx - 1.0))
+    exact = np.sin(2 * np.pi * (p.xlow + (np.arange(p.mx) + 0.5) * p.dx - tend))
     l1 = np.mean(np.abs(orc.interior(p, q1)[:, 0] - exact))
-    print("advection: steps", st0.steps_accepted, "rel err vs oracle", err, "L1 vs exact", l1)
+    print("advection: order", time_order, "steps", st0.steps_accepted, "rel err vs oracle", err, "L1 vs exact", l1)
     assert np.all(err <= 1e-10)
-    assert l1 < 2e-6
+    # forward Euler and SSP-RK2 at CFL 0.4 lose accuracy (order 1 and 2 in time)
+    assert l1 < {1: 5e-3, 2: 2e-4, 3: 2e-6}[time_order]
 
 
 def test_nv_limit_raises():

@@ -70,27 +70,90 @@ T_TIMED = 0.15
 ROUTE_INTERVAL = 4
 
 
-def cpu_baseline(mx, steps, threads):
-    """Times the oracle's fixed-step loop (no controller) on `steps` steps of
-    the plain FP64 form of the workload at full size."""
+def host_cpu():
+    """The host's CPU as lscpu reports it: model, sockets, physical cores, and
+    the cores this process may run on."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    sockets = int(info.get("Socket(s)", "1") or 1)
+    per = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "cores_per_socket": per,
+            "physical_cores": sockets * per if per else None,
+            "threads_per_core": int(info.get("Thread(s) per core", "1") or 1),
+            "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def _oracle(threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc  # the oracle loader lives with the tests
-    from finess_amd import problems
-    lib = orc.oracle()
-    lib.orc_set_num_threads(threads)
-    p, q, aux = problems.riemann4(mx=mx)
-    dt = 0.4 / mx / 2.1
+    orc.oracle().orc_set_num_threads(threads)
+    return orc
+
+
+def _time_fixed(orc, p, q, dt, steps, aux=None):
     t0 = time.perf_counter()
-    orc.fixed_steps(p, q, dt, steps, aux=np.zeros(p.shape_aux()))
-    el = time.perf_counter() - t0
-    return {
+    orc.fixed_steps(p, q, dt, steps, aux=aux)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(mx, steps, threads, others=True):
+    """Times the oracle's fixed-step loop (no controller) on `steps` steps of
+    the plain FP64 form of the headline workload at full size with `threads`
+    OpenMP threads, and on a one-thread sample; with `others`, bounded samples
+    of the other workloads beside it."""
+    from finess_amd import TS_LXW, problems
+    orc = _oracle(threads)
+    p, q, _ = problems.riemann4(mx=mx)
+    dt = 0.4 / mx / 2.1
+    el = _time_fixed(orc, p, q, dt, steps, aux=np.zeros(p.shape_aux()))
+    out = {
         "value": round(mx * mx * steps / el / 1e6, 3),
         "unit": "Mcell-updates/s",
-        "cores": int(lib.orc_num_threads()),
+        "cores": int(orc.oracle().orc_num_threads()),
         "kind": "port",
-        "sample": "%d SSP-RK3 step(s) of the %dx%d four-quadrant problem, plain FP64, fixed dt, %.1f s"
-                  % (steps, mx, mx, el),
+        "sample": "%d SSP-RK3 step(s) of the %dx%d four-quadrant problem, plain FP64, fixed dt, %d OpenMP "
+                  "threads, %.1f s" % (steps, mx, mx, threads, el),
+        "host": host_cpu(),
     }
+    orc.oracle().orc_set_num_threads(1)
+    p, q, _ = problems.riemann4(mx=mx)
+    el1 = _time_fixed(orc, p, q, dt, 2, aux=np.zeros(p.shape_aux()))
+    out["one_thread"] = {"value": round(mx * mx * 2 / el1 / 1e6, 3), "sample": "2 steps, 1 thread, %.1f s" % el1}
+    if others:
+        o = {}
+        # workload 1: the whole 500-step run (one thread: 200 cells)
+        p, q, aux = problems.advection_1d(mx=200)
+        t0 = time.perf_counter()
+        orc.fin_solve(p, q, 0.0, 1.0, np.array([0.0, p.initial_dt, p.max_dt]), aux=aux)
+        e = time.perf_counter() - t0
+        o["advection_1d_200"] = {"value": round(200 * 500 / e / 1e6, 3), "threads": 1,
+                                 "sample": "the full run, 500 steps, %.3f s" % e}
+        orc.oracle().orc_set_num_threads(threads)
+        p, q, _ = problems.kelvin_helmholtz(mx=2048)
+        e = _time_fixed(orc, p, q, 0.3 / 2048 / 2.37, 2)
+        o["kelvin_helmholtz_lxw_2048"] = {"value": round(2048 * 2048 * 2 / e / 1e6, 3), "threads": threads,
+                                          "sample": "2 Lax-Wendroff steps at 2048^2, %.1f s" % e}
+        draws = problems.sw_batch_members(n=512)[:16]
+        cells, e = 0, 0.0
+        for nx, ny, A, cx, cy, sg in draws:
+            p, q, _ = problems.sw_bump(nx=nx, ny=ny, A=A, cx=cx, cy=cy, s=sg)
+            e += _time_fixed(orc, p, q, p.initial_dt, 20)
+            cells += nx * ny
+        o["batched_shallow_water"] = {"value": round(cells * 20 / e / 1e6, 3), "threads": threads,
+                                      "sample": "16 of the 512 members, 20 steps each, one after another, %.1f s" % e}
+        p, q, _ = problems.shock_bubble(mx=4096, my=1024)
+        e = _time_fixed(orc, p, q, 0.4 / 1024 / 4.6, 2)
+        o["shock_bubble_4096x1024"] = {"value": round(4096 * 1024 * 2 / e / 1e6, 3), "threads": threads,
+                                       "sample": "2 steps, plain FP64, %.1f s" % e}
+        out["other_workloads"] = o
+    return out
 
 
 def make_solver(form, mx, indicator=None):
@@ -165,7 +228,7 @@ def _interior_density(p, q):
     return q[g:g + p.mx, g:g + p.my, 0].copy()
 
 
-def kelvin_helmholtz_lxw(mx=2048, steps=20, warmup=3):
+def kelvin_helmholtz_lxw(mx=2048, steps=20, warmup=3, t_final=1.0):
     """Third-order Lax-Wendroff, periodic Kelvin-Helmholtz at mx^2: the FP64
     and the FP32-stencil forms (FP64 state and accumulation) timed over
     `steps` accepted steps after `warmup`, from t = 0, and the relative L1
@@ -192,10 +255,30 @@ def kelvin_helmholtz_lxw(mx=2048, steps=20, warmup=3):
     out["t"] = {k: round(v, 6) for k, v in tt.items()}
     out["workload"] = "2D Euler Kelvin-Helmholtz %dx%d periodic, LxW order 3, WENO5-JS, CFL 0.4, %d steps after %d" % (
         mx, mx, steps, warmup)
+    if t_final:
+        # both forms from t = 0 to t_final (the roll-up): wall time and the
+        # density difference between them
+        fin = {}
+        for name, kw in (("fp64", {}), ("fp32", dict(precision=PREC_FP32))):
+            p, q, _ = problems.kelvin_helmholtz(mx=mx, **kw)
+            with Solver(p) as s:
+                s.set_state(q)
+                s.sync()
+                t0 = time.perf_counter()
+                st = s.advance(t_final, np.array([0.0, p.initial_dt, p.max_dt]))
+                s.sync()
+                el = time.perf_counter() - t0
+                qf, _ = s.get_state()
+            rho[name] = _interior_density(p, qf)
+            fin[name] = {"seconds": round(el, 2), "steps": int(st.steps_accepted),
+                         "value": round(mx * mx * st.steps_accepted / el / 1e6, 1)}
+        fin["l1_rel_density_fp32_vs_fp64"] = float(
+            np.mean(np.abs(rho["fp32"] - rho["fp64"])) / np.mean(np.abs(rho["fp64"])))
+        out["to_t_final"] = dict(t_final=t_final, **fin)
     return out
 
 
-def shock_bubble_learned(ind, mx=4096, my=1024, steps=40, warmup=5):
+def shock_bubble_learned(ind, mx=4096, my=1024, steps=40, warmup=5, t_final=0.2):
     """Mach-3 shock-bubble at mx x my: learned+FP32 (routing record every
     ROUTE_INTERVAL steps) against plain FP64, timed over `steps` accepted
     steps after `warmup`, from t = 0; the relative L1 density difference
@@ -226,24 +309,66 @@ def shock_bubble_learned(ind, mx=4096, my=1024, steps=40, warmup=5):
     out["speedup_vs_plain_fp64"] = round(out["learned_fp32"]["value"] / out["plain_fp64"]["value"], 4)
     out["l1_rel_density_learned_vs_plain"] = float(
         np.mean(np.abs(rho["learned_fp32"] - rho["plain_fp64"])) / np.mean(np.abs(rho["plain_fp64"])))
+    if t_final:
+        # both forms from t = 0 to t_final: wall time, the density difference
+        # of the learned form from the plain FP64 one, the routed share
+        fin = {}
+        for name, kw in (("plain_fp64", {}),
+                         ("learned_fp32", dict(precision=PREC_FP32, routing=ROUTE_LEARNED,
+                                               route_per_step=ROUTE_INTERVAL))):
+            p, q, _ = problems.shock_bubble(mx=mx, my=my, **kw)
+            with Solver(p) as s:
+                s.set_state(q)
+                if name == "learned_fp32":
+                    ind.install(s)
+                s.sync()
+                t0 = time.perf_counter()
+                st = s.advance(t_final, np.array([0.0, p.initial_dt, p.max_dt]))
+                s.sync()
+                el = time.perf_counter() - t0
+                route = s.route_stats() if name == "learned_fp32" else None
+                qf, _ = s.get_state()
+            rho[name] = _interior_density(p, qf)
+            fin[name] = {"seconds": round(el, 2), "steps": int(st.steps_accepted),
+                         "value": round(mx * my * st.steps_accepted / el / 1e6, 1)}
+            if route is not None:
+                fin[name]["troubled_fraction"] = round(route[0] / max(route[1], 1), 4)
+        fin["l1_rel_density_learned_vs_plain"] = float(
+            np.mean(np.abs(rho["learned_fp32"] - rho["plain_fp64"])) / np.mean(np.abs(rho["plain_fp64"])))
+        out["to_t_final"] = dict(t_final=t_final, **fin)
     out["workload"] = ("2D Euler Mach-3 shock-bubble %dx%d, WENO5-JS + LLF, SSP-RK3, CFL 0.4, %d steps after %d; "
                        "indicator trained on the four-quadrant problem and a density wave" % (mx, my, steps, warmup))
     return out
 
 
-def plain_fp64_2048(steps=20, warmup=3):
-    """The plain FP64 form on the four-quadrant problem at 2048^2 (timed from
-    t = 0.15 like the headline): Mcell-updates/s and the fused stage kernel's
-    fraction of the HBM roofline (algorithmic bytes over its event-timed
-    average launch)."""
+def at_2048(ind, steps=20, warmup=3, accuracy=True):
+    """Both forms on the four-quadrant problem at 2048^2 (timed from t = 0.15
+    like the headline): Mcell-updates/s and each stage's fraction of the HBM
+    roofline (algorithmic bytes over its event-timed average launch); with
+    `accuracy`, the L1 density errors at t_final against a 4096^2 plain FP64
+    run averaged 2 x 2."""
     mx = 2048
-    el, rej, launch = timed_steps("plain", mx, steps, warmup)
     cells = mx * mx
     bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / STAGES_PER_STEP
-    return {"value": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 4),
-            "steps": steps, "steps_rejected": rej, "avg_stage_launch_us": round(launch * 1e6, 2),
-            "hbm_frac": round(bytes_per_launch / launch / 1e9 / HBM_PEAK_GBS, 4),
-            "workload": "2D Euler four-quadrant Riemann 2048x2048, char-WENO5-JS + LLF, SSP-RK3, plain FP64"}
+    out = {}
+    for form in ("plain", "learned"):
+        el, rej, launch = timed_steps(form, mx, steps, warmup, ind if form == "learned" else None)
+        out[form] = {"value": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 4),
+                     "steps": steps, "steps_rejected": rej, "avg_stage_launch_us": round(launch * 1e6, 2),
+                     "hbm_frac": round(bytes_per_launch / launch / 1e9 / HBM_PEAK_GBS, 4)}
+    res = {"plain_fp64": out["plain"], "learned_fp32": out["learned"],
+           "speedup_vs_plain_fp64": round(out["learned"]["value"] / out["plain"]["value"], 4),
+           "workload": "2D Euler four-quadrant Riemann 2048x2048, char-WENO5-JS + LLF, SSP-RK3"}
+    if accuracy:
+        ref, _, _, _ = full_run("plain", 2 * mx)
+        ref = 0.25 * (ref[0::2, 0::2] + ref[1::2, 0::2] + ref[0::2, 1::2] + ref[1::2, 1::2])
+        rp, _, _, _ = full_run("plain", mx)
+        rl, _, _, route = full_run("learned", mx, ind)
+        l1p, l1l = float(np.mean(np.abs(rp - ref))), float(np.mean(np.abs(rl - ref)))
+        res["accuracy"] = {"reference": "plain FP64 at 4096x4096 to t=%.2f, 2x2-averaged" % T_FINAL,
+                           "l1_density_plain_fp64": l1p, "l1_density_learned_fp32": l1l,
+                           "l1_ratio": round(l1l / l1p, 5), "troubled_fraction": round(route[0] / max(route[1], 1), 4)}
+    return res
 
 
 def advection_1d_full(mx=200):
@@ -416,13 +541,18 @@ def main():
     if not args.no_batch:
         out["batched_shallow_water"] = batched_shallow_water()
     if not args.no_extra:
-        out["plain_fp64_2048"] = plain_fp64_2048()
+        w = at_2048(ind, accuracy=not args.no_accuracy)
+        out["plain_fp64_2048"] = dict(w["plain_fp64"], workload=w["workload"] + ", plain FP64")
+        out["learned_fp32_2048"] = dict(w["learned_fp32"], speedup_vs_plain_fp64=w["speedup_vs_plain_fp64"],
+                                        accuracy=w.get("accuracy"), workload=w["workload"] + ", learned+FP32")
         out["advection_1d_200"] = advection_1d_full()
-        out["kelvin_helmholtz_lxw_2048"] = kelvin_helmholtz_lxw()
-        out["shock_bubble_4096x1024"] = shock_bubble_learned(ind)
+        out["kelvin_helmholtz_lxw_2048"] = kelvin_helmholtz_lxw(t_final=0.0 if args.no_accuracy else 1.0)
+        out["shock_bubble_4096x1024"] = shock_bubble_learned(ind, t_final=0.0 if args.no_accuracy else 0.2)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(mx, args.cpu_steps, threads)
+        # the host's physical cores that this process may use
+        h = host_cpu()
+        threads = min(h["physical_cores"] or h["usable_cpus"], h["usable_cpus"])
+        out["cpu_baseline"] = cpu_baseline(mx, args.cpu_steps, threads, others=not args.no_extra)
     print(json.dumps(out), flush=True)
 
 

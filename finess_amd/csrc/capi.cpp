@@ -343,7 +343,7 @@ int lxw_fluxes(finess_ctx* c, int src, double dt)
     a.dy = p.dy;
     a.pp.gamma = p.gamma;
     a.pp.sw_compat_hl = p.sw_compat_hl;
-    const int rc = launch_lxw_flux(p.physics, a, c->st);
+    const int rc = launch_lxw_flux(p.physics, p.precision == FINESS_PREC_FP32, a, c->st);
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "LxW flux launch failed") : 0;
 }
 
@@ -513,11 +513,11 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         s.n = p.time_order;
         // (only the last stage reduces its wave speeds: 1D GetCFL reads those)
         const int n = p.time_order;
-        s.stage[0] = stage1d_args(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, n == 1 ? 0 : -1);
-        if (n == 2) s.stage[1] = stage1d_args(c, B, B, A, C, 0.5, 0.5, 0.5, dt, 1);
-        if (n == 3) {
-            s.stage[1] = stage1d_args(c, B, A, B, C, 0.75, 0.25, 0.25, dt, -1);
-            s.stage[2] = stage1d_args(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, 2);
+        const int role[3] = {A, B, C};
+        for (int k = 0; k < n; k++) {
+            const SspStage g = ssp_stage(n, k);
+            s.stage[k] = stage1d_args(c, role[g.src], role[g.qa], g.qb >= 0 ? role[g.qb] : -1, role[g.out], g.a1, g.a2,
+                                      g.beta, dt, k == n - 1 ? n - 1 : -1);
         }
         if (c->devctl && c->ctlp) {  // the controller goes with the step
             s.ctl = c->ctl;
@@ -526,29 +526,28 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
         }
         rc = launch_step1d(weno_kind(p), s, c->st);
         if (rc) return fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "1D step launch failed");
-        *res = p.time_order == 2 ? C : B;
-        *slot = p.time_order - 1;
+        *res = role[ssp_result(n)];
+        *slot = n - 1;
         return 0;
     }
     switch (p.time_order) {
     case 1:
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, 0))) return rc;
-        *res = B;
-        *slot = 0;
-        return 0;
     case 2:
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, one_d ? -1 : 0))) return rc;
-        if ((rc = stage(c, B, B, A, C, 0.5, 0.5, 0.5, dt, one_d ? 1 : 0))) return rc;
-        *res = C;
-        *slot = one_d ? 1 : 0;
+    case 3: {
+        // 2D: every stage max-accumulates into slot 0; 1D: only the last
+        // stage's speeds count (slot n - 1)
+        const int n = p.time_order;
+        const int role[3] = {A, B, C};
+        for (int k = 0; k < n; k++) {
+            const SspStage g = ssp_stage(n, k);
+            if ((rc = stage(c, role[g.src], role[g.qa], g.qb >= 0 ? role[g.qb] : -1, role[g.out], g.a1, g.a2, g.beta,
+                            dt, one_d ? (k == n - 1 ? n - 1 : -1) : 0)))
+                return rc;
+        }
+        *res = role[ssp_result(n)];
+        *slot = one_d ? n - 1 : 0;
         return 0;
-    case 3:
-        if ((rc = stage(c, A, A, -1, B, 1.0, 0.0, 1.0, dt, one_d ? -1 : 0))) return rc;
-        if ((rc = stage(c, B, A, B, C, 0.75, 0.25, 0.25, dt, one_d ? -1 : 0))) return rc;
-        if ((rc = stage(c, C, C, A, B, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0, dt, one_d ? 2 : 0))) return rc;
-        *res = B;
-        *slot = one_d ? 2 : 0;
-        return 0;
+    }
     case 4: {
         // SSP(10,4) (lib/2d/FinSolveRK.cpp:213-291): stages ping-pong between
         // B and C, q2 lives in buffer 3
@@ -1322,7 +1321,7 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     a.dy = p.dy;
     a.pp.gamma = p.gamma;
     std::vector<double> hF(lb), hG(lb);
-    hipError_t e = launch_lxw_flux(p.physics, a, c->st) ? hipErrorLaunchFailure : hipSuccess;
+    hipError_t e = launch_lxw_flux(p.physics, p.precision == FINESS_PREC_FP32, a, c->st) ? hipErrorLaunchFailure : hipSuccess;
     if (e == hipSuccess) e = hipMemcpyAsync(hF.data(), Fd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(hG.data(), Gd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -1415,6 +1414,36 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         int in, out, rec_age, rec_par;
         bool fresh;
     };
+    // 1D grids of one workgroup without per-step records: the whole run in
+    // launches of up to 65536 attempts each, the controller deciding on the
+    // device and the kernel rotating the plane sets itself
+    if (run && !rc && p.ndims == 1 && !c->steplog && !c->timing && p.time_order <= 3 && run1d_fits(p.mx)) {
+        Stage1DRun r;
+        memset(&r, 0, sizeof r);
+        r.base = base_stage1d_args(c);
+        r.base.dtp = &c->ctl->dt;
+        r.base.halt = &c->ctl->halt;
+        for (int k = 0; k < 3; k++) r.buf[k] = c->buf[k];
+        r.n = p.time_order;
+        r.smax_slot = c->smax;
+        r.ctl = c->ctl;
+        r.cp = cp;
+        r.max_attempts = 65536;
+        while (run) {
+            r.cur = c->cur;
+            if (launch_run1d(weno_kind(p), r, c->st)) { rc = fail(c, FINESS_ERR_HIP, "1D run launch failed"); break; }
+            if (hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+                hipStreamSynchronize(c->st) != hipSuccess) {
+                rc = fail(c, FINESS_ERR_HIP, "device controller: stream failure");
+                break;
+            }
+            c->cur = (int)h.buf;
+            if (h.halt == HALT_NONE) continue;  // the launch's attempts ran out
+            if (h.halt != HALT_DONE) rc = fail(c, (int)h.err, ctl_error_message(h.err));
+            break;
+        }
+        run = false;
+    }
     std::vector<Snap> snap;
     long long seen = 0;   // attempts processed before the current chunk
     long long logged = 0; // conservation records read back

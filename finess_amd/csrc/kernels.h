@@ -20,10 +20,16 @@ int launch_stage2d_wide(int physics, int wk, const StageArgs& a, hipStream_t st)
 int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st);
 // The stages of one 1D time step (one launch up to 8192 cells).
 int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st);
+// A run of 1D SSP steps with the device controller in one single-workgroup
+// launch (grids that fit it: run1d_fits).
+struct Stage1DRun;
+int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st);
+bool run1d_fits(int mx);
 // 1D Lax-Wendroff time-integrated fluxes of the advection flux on [-3, mx+2].
 int launch_lxw_flux1d(const Stage1DArgs& a, double dt, int time_order, double* F, hipStream_t st);
-// Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2].
-int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st);
+// Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2]; fp32:
+// computed in FP32 (the FP32 stencil form, Euler).
+int launch_lxw_flux(int physics, int fp32, const LxwArgs& a, hipStream_t st);
 
 int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st);
 

@@ -7,25 +7,30 @@
 
 namespace finess {
 
-template <class P, int TI, int TJ>
+template <class P, typename R, int TI, int TJ>
 __global__ __launch_bounds__(TI* TJ) void k_lxw_flux(LxwArgs a, int ntj)
 {
-    __shared__ LxwSmem<TI, TJ, P::MEQN> sm;
+    __shared__ LxwSmem<TI, TJ, P::MEQN, R> sm;
     const int ti = blockIdx.x / ntj, tj = blockIdx.x - ti * ntj;
     const int e0i = ti * TI, e0j = tj * TJ;
-    lxw_phase_load<P, TI, TJ>(threadIdx.x, TI * TJ, e0i, e0j, a, sm);
+    lxw_phase_load<P, TI, TJ, R>(threadIdx.x, TI * TJ, e0i, e0j, a, sm);
     __syncthreads();
-    lxw_phase_cell<P, TI, TJ>(threadIdx.x, e0i, e0j, a, sm);
+    lxw_phase_cell<P, TI, TJ, R>(threadIdx.x, e0i, e0j, a, sm);
 }
 
-int launch_lxw_flux(int physics, const LxwArgs& a, hipStream_t st)
+// fp32: the FP32 stencil form (Euler), whose Lax-Wendroff stage reads the
+// fluxes as FP32: differences, hyper-dual products and Taylor terms in FP32.
+int launch_lxw_flux(int physics, int fp32, const LxwArgs& a, hipStream_t st)
 {
     constexpr int TI = 8, TJ = 32;
     const int nti = (a.mx + 6 + TI - 1) / TI, ntj = (a.my + 6 + TJ - 1) / TJ;
-    if (physics == PHYS_EULER)
-        hipLaunchKernelGGL((k_lxw_flux<Euler2D, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
-    else if (physics == PHYS_SHALLOW_WATER)
-        hipLaunchKernelGGL((k_lxw_flux<ShallowWater2D, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    if (physics == PHYS_EULER && fp32)
+        hipLaunchKernelGGL((k_lxw_flux<Euler2D, float, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    else if (physics == PHYS_EULER)
+        hipLaunchKernelGGL((k_lxw_flux<Euler2D, double, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a, ntj);
+    else if (physics == PHYS_SHALLOW_WATER && !fp32)
+        hipLaunchKernelGGL((k_lxw_flux<ShallowWater2D, double, TI, TJ>), dim3(nti * ntj), dim3(TI * TJ), 0, st, a,
+                           ntj);
     else
         return -1;
     return hipGetLastError() == hipSuccess ? 0 : -4;

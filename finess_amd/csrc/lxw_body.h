@@ -69,17 +69,26 @@ __host__ __device__ __forceinline__ void sw_flux(const T q[3], T F[3], T G[3])
     G[0] = h * u2; G[1] = h * u2 * u1; G[2] = h * u2 * u2 + hh;
 }
 
-struct HDd : HD<double> {
-    using scalar = double;
-    __host__ __device__ HDd() {}
-    __host__ __device__ HDd(const HD<double>& x) : HD<double>(x) {}
-    __host__ __device__ HDd(double v_, double a_, double b_, double ab_) : HD<double>{v_, a_, b_, ab_} {}
+// Hyper-dual number of scalar type T (double for the plain form, float for
+// the FP32 stencil form).
+template <typename T>
+struct HDs : HD<T> {
+    using scalar = T;
+    __host__ __device__ HDs() {}
+    __host__ __device__ HDs(const HD<T>& x) : HD<T>(x) {}
+    __host__ __device__ HDs(T v_, T a_, T b_, T ab_) : HD<T>{v_, a_, b_, ab_} {}
 };
-__host__ __device__ __forceinline__ HDd operator+(HDd x, HDd y) { return HDd(HD<double>(x) + HD<double>(y)); }
-__host__ __device__ __forceinline__ HDd operator-(HDd x, HDd y) { return HDd(HD<double>(x) - HD<double>(y)); }
-__host__ __device__ __forceinline__ HDd operator*(HDd x, HDd y) { return HDd(HD<double>(x) * HD<double>(y)); }
-__host__ __device__ __forceinline__ HDd operator*(double s, HDd x) { return HDd(s * HD<double>(x)); }
-__host__ __device__ __forceinline__ HDd hinv(HDd x) { return HDd(hinv(HD<double>(x))); }
+template <typename T>
+__host__ __device__ __forceinline__ HDs<T> operator+(HDs<T> x, HDs<T> y) { return HDs<T>(HD<T>(x) + HD<T>(y)); }
+template <typename T>
+__host__ __device__ __forceinline__ HDs<T> operator-(HDs<T> x, HDs<T> y) { return HDs<T>(HD<T>(x) - HD<T>(y)); }
+template <typename T>
+__host__ __device__ __forceinline__ HDs<T> operator*(HDs<T> x, HDs<T> y) { return HDs<T>(HD<T>(x) * HD<T>(y)); }
+template <typename T>
+__host__ __device__ __forceinline__ HDs<T> operator*(T s, HDs<T> x) { return HDs<T>(s * HD<T>(x)); }
+template <typename T>
+__host__ __device__ __forceinline__ HDs<T> hinv(HDs<T> x) { return HDs<T>(hinv(HD<T>(x))); }
+using HDd = HDs<double>;
 
 template <class P>
 struct LxwPhys;
@@ -87,22 +96,24 @@ struct LxwPhys;
 template <>
 struct LxwPhys<Euler2D> {
     static constexpr int M = 5;
-    __host__ __device__ __forceinline__ static void flux(const double q[5], const PhysParams& pp, double F[5], double G[5])
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux(const R q[5], const PhysParams& pp, R F[5], R G[5])
     {
-        const double rho = q[0];
-        const double u1 = q[1] / rho, u2 = q[2] / rho, u3 = q[3] / rho;
-        const double e = q[4];
-        const double press = (pp.gamma - 1.0) * (e - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
+        const R rho = q[0];
+        const R u1 = div_lp(q[1], rho), u2 = div_lp(q[2], rho), u3 = div_lp(q[3], rho);
+        const R e = q[4];
+        const R press = R(pp.gamma - 1.0) * (e - R(0.5) * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         F[0] = rho * u1; F[1] = rho * u1 * u1 + press; F[2] = rho * u1 * u2; F[3] = rho * u1 * u3; F[4] = u1 * (e + press);
         G[0] = rho * u2; G[1] = rho * u2 * u1; G[2] = rho * u2 * u2 + press; G[3] = rho * u2 * u3; G[4] = u2 * (e + press);
     }
     // hyper-dual flux at q + a e1 + b e2
-    __host__ __device__ __forceinline__ static void flux_hd(const double q[5], const double a[5], const double b[5],
-                                                            const PhysParams& pp, HDd F[5], HDd G[5])
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux_hd(const R q[5], const R a[5], const R b[5],
+                                                            const PhysParams& pp, HDs<R> F[5], HDs<R> G[5])
     {
-        HDd x[5];
-        for (int m = 0; m < 5; m++) x[m] = HDd(q[m], a[m], b[m], 0.0);
-        euler_flux<HDd>(x, HDd(pp.gamma - 1.0, 0, 0, 0), F, G);
+        HDs<R> x[5];
+        for (int m = 0; m < 5; m++) x[m] = HDs<R>(q[m], a[m], b[m], R(0));
+        euler_flux<HDs<R>>(x, HDs<R>(R(pp.gamma - 1.0), R(0), R(0), R(0)), F, G);
     }
 };
 
@@ -112,19 +123,21 @@ struct LxwPhys<Euler2D> {
 template <>
 struct LxwPhys<ShallowWater2D> {
     static constexpr int M = 3;
-    __host__ __device__ __forceinline__ static void flux(const double q[3], const PhysParams&, double F[3], double G[3])
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux(const R q[3], const PhysParams&, R F[3], R G[3])
     {
-        const double h = q[0];
-        const double u1 = q[1] / h, u2 = q[2] / h;
-        F[0] = h * u1; F[1] = h * u1 * u1 + 0.5 * h * h; F[2] = h * u1 * u2;
-        G[0] = h * u2; G[1] = h * u2 * u1; G[2] = h * u2 * u2 + 0.5 * h * h;
+        const R h = q[0];
+        const R u1 = div_lp(q[1], h), u2 = div_lp(q[2], h);
+        F[0] = h * u1; F[1] = h * u1 * u1 + R(0.5) * h * h; F[2] = h * u1 * u2;
+        G[0] = h * u2; G[1] = h * u2 * u1; G[2] = h * u2 * u2 + R(0.5) * h * h;
     }
-    __host__ __device__ __forceinline__ static void flux_hd(const double q[3], const double a[3], const double b[3],
-                                                            const PhysParams&, HDd F[3], HDd G[3])
+    template <typename R>
+    __host__ __device__ __forceinline__ static void flux_hd(const R q[3], const R a[3], const R b[3],
+                                                            const PhysParams&, HDs<R> F[3], HDs<R> G[3])
     {
-        HDd x[3];
-        for (int m = 0; m < 3; m++) x[m] = HDd(q[m], a[m], b[m], 0.0);
-        sw_flux<HDd>(x, F, G);
+        HDs<R> x[3];
+        for (int m = 0; m < 3; m++) x[m] = HDs<R>(q[m], a[m], b[m], R(0));
+        sw_flux<HDs<R>>(x, F, G);
     }
 };
 
@@ -141,12 +154,14 @@ struct LxwArgs {
     int src_ghosts;      // >0: q planes include that many ghosts per side (drop-in mode)
 };
 
-template <int TI, int TJ, int M>
+// R: the arithmetic of the differences and Taylor terms (double in the plain
+// form, float in the FP32 stencil form, whose stage reads these fluxes as FP32).
+template <int TI, int TJ, int M, typename R = double>
 struct LxwSmem {
     static constexpr int HI = TI + 4, HJ = TJ + 4, NC = HI * HJ;
-    double q[M][NC];
-    double f[M][NC];
-    double g[M][NC];
+    R q[M][NC];
+    R f[M][NC];
+    R g[M][NC];
 };
 
 template <class P>
@@ -171,81 +186,86 @@ __host__ __device__ __forceinline__ void lxw_load_state(const LxwArgs& a, int gi
 
 // Phase 1: states and point fluxes of the tile plus a 2-cell halo (corners
 // included: the cross derivatives read the diagonal neighbours).
-template <class P, int TI, int TJ>
+template <class P, int TI, int TJ, typename R = double>
 __host__ __device__ __forceinline__ void lxw_phase_load(int tid, int nthreads, int e0i, int e0j, const LxwArgs& a,
-                                                        LxwSmem<TI, TJ, P::MEQN>& sm)
+                                                        LxwSmem<TI, TJ, P::MEQN, R>& sm)
 {
-    using S = LxwSmem<TI, TJ, P::MEQN>;
+    using S = LxwSmem<TI, TJ, P::MEQN, R>;
     for (int c = tid; c < S::NC; c += nthreads) {
         const int ci = c / S::HJ, cj = c - ci * S::HJ;
         const int gi = e0i + ci - 2 - 3, gj = e0j + cj - 2 - 3;
         if (gi > a.mx + 4 || gj > a.my + 4) continue;
-        double q[P::MEQN], f[P::MEQN], g[P::MEQN];
-        lxw_load_state<P>(a, gi, gj, q);
+        double qd[P::MEQN];
+        R q[P::MEQN], f[P::MEQN], g[P::MEQN];
+        lxw_load_state<P>(a, gi, gj, qd);
+        for (int m = 0; m < P::MEQN; m++) q[m] = R(qd[m]);
         LxwPhys<P>::flux(q, a.pp, f, g);
         for (int m = 0; m < P::MEQN; m++) { sm.q[m][c] = q[m]; sm.f[m][c] = f[m]; sm.g[m][c] = g[m]; }
     }
 }
 
-__host__ __device__ __forceinline__ double d1(double h, double f0, double f1, double f3, double f4)
+template <typename R>
+__host__ __device__ __forceinline__ R d1(R h, R f0, R f1, R f3, R f4)
 {
-    double t = (f0 - f4) * (1.0 / 12.0);
-    t += (-f1 + f3) * (2.0 / 3.0);
-    return t / h;
+    R t = (f0 - f4) * R(1.0 / 12.0);
+    t += (-f1 + f3) * R(2.0 / 3.0);
+    return div_lp(t, h);
 }
-__host__ __device__ __forceinline__ double d2(double h, double f0, double f1, double f2, double f3, double f4)
+template <typename R>
+__host__ __device__ __forceinline__ R d2(R h, R f0, R f1, R f2, R f3, R f4)
 {
-    double t = (-f0 - f4) * (1.0 / 12.0);
-    t += (f1 + f3) * (4.0 / 3.0);
-    t += (-f2) * (5.0 / 2.0);
-    return t / (h * h);
+    R t = (-f0 - f4) * R(1.0 / 12.0);
+    t += (f1 + f3) * R(4.0 / 3.0);
+    t += (-f2) * R(5.0 / 2.0);
+    return div_lp(t, h * h);
 }
 
 // Phase 2: one output cell per thread.
-template <class P, int TI, int TJ>
+template <class P, int TI, int TJ, typename R = double>
 __host__ __device__ __forceinline__ void lxw_phase_cell(int tid, int e0i, int e0j, const LxwArgs& a,
-                                                        const LxwSmem<TI, TJ, P::MEQN>& sm)
+                                                        const LxwSmem<TI, TJ, P::MEQN, R>& sm)
 {
-    using S = LxwSmem<TI, TJ, P::MEQN>;
+    using S = LxwSmem<TI, TJ, P::MEQN, R>;
     constexpr int M = P::MEQN;
     const int ci = tid / TJ, cj = tid - ci * TJ;
     const int ei = e0i + ci, ej = e0j + cj;              // extended index, gi = ei - 3
     if (ci >= TI || ei >= a.mx + 6 || ej >= a.my + 6) return;
     const int c = (ci + 2) * S::HJ + (cj + 2);
     const int X = S::HJ, Y = 1;
-    double q[M], fx[M], gy[M], qx[M], qy[M], S_[M];
+    const R dx = R(a.dx), dy = R(a.dy), dt = R(a.dt);
+    R q[M], fx[M], gy[M], qx[M], qy[M], S_[M];
     for (int m = 0; m < M; m++) {
         q[m] = sm.q[m][c];
-        fx[m] = d1(a.dx, sm.f[m][c - 2 * X], sm.f[m][c - X], sm.f[m][c + X], sm.f[m][c + 2 * X]);
-        qx[m] = d1(a.dx, sm.q[m][c - 2 * X], sm.q[m][c - X], sm.q[m][c + X], sm.q[m][c + 2 * X]);
-        gy[m] = d1(a.dy, sm.g[m][c - 2 * Y], sm.g[m][c - Y], sm.g[m][c + Y], sm.g[m][c + 2 * Y]);
-        qy[m] = d1(a.dy, sm.q[m][c - 2 * Y], sm.q[m][c - Y], sm.q[m][c + Y], sm.q[m][c + 2 * Y]);
+        fx[m] = d1(dx, sm.f[m][c - 2 * X], sm.f[m][c - X], sm.f[m][c + X], sm.f[m][c + 2 * X]);
+        qx[m] = d1(dx, sm.q[m][c - 2 * X], sm.q[m][c - X], sm.q[m][c + X], sm.q[m][c + 2 * X]);
+        gy[m] = d1(dy, sm.g[m][c - 2 * Y], sm.g[m][c - Y], sm.g[m][c + Y], sm.g[m][c + 2 * Y]);
+        qy[m] = d1(dy, sm.q[m][c - 2 * Y], sm.q[m][c - Y], sm.q[m][c + Y], sm.q[m][c + 2 * Y]);
         S_[m] = fx[m] + gy[m];
     }
-    HDd F1[M], G1[M];
+    HDs<R> F1[M], G1[M];
     LxwPhys<P>::flux_hd(q, qx, S_, a.pp, F1, G1);      // A S (e2), H1(qx, S) (e1e2)
-    double f_t[M], g_t[M], f_tt[M], g_tt[M];
-    for (int m = 0; m < M; m++) { f_t[m] = -F1[m].b; g_t[m] = -G1[m].b; f_tt[m] = 0.0; g_tt[m] = 0.0; }
+    R f_t[M], g_t[M], f_tt[M], g_tt[M];
+    for (int m = 0; m < M; m++) { f_t[m] = -F1[m].b; g_t[m] = -G1[m].b; f_tt[m] = R(0); g_tt[m] = R(0); }
     if (a.time_order > 2) {
-        HDd F2[M], G2[M], F3[M], G3[M];
+        HDs<R> F2[M], G2[M], F3[M], G3[M];
         LxwPhys<P>::flux_hd(q, qy, S_, a.pp, F2, G2);  // H2(qy, S)
-        double sx[M], sy[M];
+        R sx[M], sy[M];
         for (int m = 0; m < M; m++) {
-            const double fxx = d2(a.dx, sm.f[m][c - 2 * X], sm.f[m][c - X], sm.f[m][c], sm.f[m][c + X], sm.f[m][c + 2 * X]);
-            const double gyy = d2(a.dy, sm.g[m][c - 2 * Y], sm.g[m][c - Y], sm.g[m][c], sm.g[m][c + Y], sm.g[m][c + 2 * Y]);
-            double t1 = 0.5 * (sm.f[m][c + X + Y] - sm.f[m][c - X + Y]) / a.dx;
-            t1 -= 0.5 * (sm.f[m][c + X - Y] - sm.f[m][c - X - Y]) / a.dx;
-            t1 *= 0.5 / a.dy;
-            double t2 = 0.5 * (sm.g[m][c + X + Y] - sm.g[m][c - X + Y]) / a.dx;
-            t2 -= 0.5 * (sm.g[m][c + X - Y] - sm.g[m][c - X - Y]) / a.dx;
-            t2 *= 0.5 / a.dy;
+            const R fxx = d2(dx, sm.f[m][c - 2 * X], sm.f[m][c - X], sm.f[m][c], sm.f[m][c + X], sm.f[m][c + 2 * X]);
+            const R gyy = d2(dy, sm.g[m][c - 2 * Y], sm.g[m][c - Y], sm.g[m][c], sm.g[m][c + Y], sm.g[m][c + 2 * Y]);
+            R t1 = div_lp(R(0.5) * (sm.f[m][c + X + Y] - sm.f[m][c - X + Y]), dx);
+            t1 -= div_lp(R(0.5) * (sm.f[m][c + X - Y] - sm.f[m][c - X - Y]), dx);
+            t1 *= div_lp(R(0.5), dy);
+            R t2 = div_lp(R(0.5) * (sm.g[m][c + X + Y] - sm.g[m][c - X + Y]), dx);
+            t2 -= div_lp(R(0.5) * (sm.g[m][c + X - Y] - sm.g[m][c - X - Y]), dx);
+            t2 *= div_lp(R(0.5), dy);
             sx[m] = fxx + t2;  // f_xx + g_xy
             sy[m] = t1 + gyy;  // f_xy + g_yy
         }
         LxwPhys<P>::flux_hd(q, sx, sy, a.pp, F3, G3);  // A1 sx (e1 of F), A2 sy (e2 of G)
-        double qtt[M];
+        R qtt[M];
         for (int m = 0; m < M; m++) qtt[m] = F1[m].ab + G2[m].ab + F3[m].a + G3[m].b;
-        HDd F4[M], G4[M], F5[M], G5[M];
+        HDs<R> F4[M], G4[M], F5[M], G5[M];
         LxwPhys<P>::flux_hd(q, qtt, S_, a.pp, F4, G4); // A q_tt (e1)
         LxwPhys<P>::flux_hd(q, S_, S_, a.pp, F5, G5);  // H(S, S) (e1e2)
         for (int m = 0; m < M; m++) {
@@ -256,8 +276,8 @@ __host__ __device__ __forceinline__ void lxw_phase_cell(int tid, int e0i, int e0
     const long long lld = a.my + 6, lpl = (long long)(a.mx + 6) * lld;
     const long long o = (long long)ei * lld + ej;
     for (int m = 0; m < M; m++) {
-        a.F[m * lpl + o] = sm.f[m][c] + 0.5 * a.dt * (f_t[m] + a.dt / 3.0 * f_tt[m]);
-        a.G[m * lpl + o] = sm.g[m][c] + 0.5 * a.dt * (g_t[m] + a.dt / 3.0 * g_tt[m]);
+        a.F[m * lpl + o] = double(sm.f[m][c] + R(0.5) * dt * (f_t[m] + div_lp(dt, R(3)) * f_tt[m]));
+        a.G[m * lpl + o] = double(sm.g[m][c] + R(0.5) * dt * (g_t[m] + div_lp(dt, R(3)) * g_tt[m]));
     }
 }
 

@@ -45,26 +45,38 @@ struct Euler2D {
 
     // cell record: q0..q4, 1/rho, p, c  with c = sqrt(|gamma p / rho|), the
     // cell sound speed SetWaveSpd uses for the left and right states
-    // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Always formed in FP64 (the
-    // pressure recovery keeps FP64 in the FP32 form as well).
-    // LP: the record is rounded to FP32 afterwards, so 1/rho comes from the
-    // refined reciprocal estimate (within an FP64 ulp), the pressure from the
-    // momenta directly (still in FP64) and c from the FP32 square root of the
-    // FP64 c^2.
+    // (apps/2d/euler/lib/SetWaveSpd.cpp:47-49).  Formed in FP64.
+    // LP (the FP32 stencil form, whose records are rounded to FP32): 1/rho,
+    // the kinetic energy |m|^2 / (2 rho) from the momenta, p and c in FP32,
+    // except in near-vacuum cells, where E - |m|^2/(2 rho) cancels more than
+    // two bits (p < E/4): there the pressure is recovered in FP64.
     template <bool LP = false>
     __host__ __device__ __forceinline__ static void cell_record(const double q[5], const PhysParams& pp, double v[8])
     {
         const double gamma = pp.gamma;
-        const double rho = q[0];
-        const double irho = LP ? rcp_nr(rho) : 1.0 / rho;
-        const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
-        // LP: the kinetic energy from the momenta, |m|^2 / (2 rho), without the velocities
-        const double press = LP ? (gamma - 1.0) * (q[4] - 0.5 * ((q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * irho))
-                                : (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2]; v[3] = q[3]; v[4] = q[4];
+        if constexpr (LP) {
+            const float r = float(q[0]), m1 = float(q[1]), m2 = float(q[2]), m3 = float(q[3]), e = float(q[4]);
+            const float ir = frcp(r);
+            const float ke = 0.5f * ((m1 * m1 + m2 * m2 + m3 * m3) * ir);
+            const float gm1 = float(gamma - 1.0);
+            float press;
+            if (e - ke >= 0.25f * e) {
+                press = gm1 * (e - ke);
+            } else {  // near vacuum (or a non-finite state): FP64
+                const double irho = rcp_nr(q[0]);
+                press = float((gamma - 1.0) * (q[4] - 0.5 * ((q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * irho)));
+            }
+            v[5] = ir; v[6] = press;
+            v[7] = sqrt_lp(fabsf(float(gamma) * press * ir));
+            return;
+        }
+        const double rho = q[0];
+        const double irho = 1.0 / rho;
+        const double u1 = q[1] * irho, u2 = q[2] * irho, u3 = q[3] * irho;
+        const double press = (gamma - 1.0) * (q[4] - 0.5 * rho * (u1 * u1 + u2 * u2 + u3 * u3));
         v[5] = irho; v[6] = press;
-        const double c2 = fabs(gamma * press * irho);
-        v[7] = LP ? double(sqrt_lp(float(c2))) : sqrt(c2);
+        v[7] = sqrt(fabs(gamma * press * irho));
     }
 
     // LDS variable index of rotated component k (dir 0: x, 1: y)
@@ -201,11 +213,17 @@ struct ShallowWater2D {
     static constexpr int NORMAL_X = 1;
     static constexpr int NORMAL_Y = 2;
 
+    // LP (FP32 stencil form): the velocities in FP32.
     template <bool LP = false>
     __host__ __device__ __forceinline__ static void cell_record(const double q[3], const PhysParams&, double v[5])
     {
-        const double ih = LP ? rcp_nr(q[0]) : 1.0 / q[0];
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
+        if constexpr (LP) {
+            const float ih = frcp(float(q[0]));
+            v[3] = float(q[1]) * ih; v[4] = float(q[2]) * ih;
+            return;
+        }
+        const double ih = 1.0 / q[0];
         v[3] = q[1] * ih; v[4] = q[2] * ih;
     }
 

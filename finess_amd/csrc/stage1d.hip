@@ -115,6 +115,50 @@ __global__ __launch_bounds__(SMALL1D_NT) void k_step1d_small(Stage1DStep s)
     if (s.ctl && threadIdx.x < 64) step_control_wave(s.ctl, s.stage[s.n - 1].smax_slot, s.cp, threadIdx.x);
 }
 
+// A run of steps (Stage1DRun).  Every thread takes the controller's decision
+// from the block after the barrier that follows it: a step that raised the
+// accepted count rotates the plane sets (an nv stop happens after its step
+// is accepted, as on the host); a rejection is cleared here and retried.
+template <int WK>
+__global__ __launch_bounds__(SMALL1D_NT) void k_run1d_small(Stage1DRun r)
+{
+    __shared__ double red[SMALL1D_NT / 64];
+    int cur = r.cur;
+    for (int att = 0; att < r.max_attempts; att++) {
+        if (__hip_atomic_load(&r.ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != HALT_NONE) break;
+        const long long acc0 = __hip_atomic_load(&r.ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < r.n; k++) {
+            const SspStage g = ssp_stage(r.n, k);
+            Stage1DArgs a = r.base;
+            a.src = r.buf[(cur + g.src) % 3];
+            a.qa = r.buf[(cur + g.qa) % 3];
+            a.qb = g.qb >= 0 ? r.buf[(cur + g.qb) % 3] : nullptr;
+            a.use_b = g.qb >= 0 && g.a2 != 0.0;
+            a.out = r.buf[(cur + g.out) % 3];
+            a.a1 = g.a1;
+            a.a2 = g.a2;
+            a.beta = g.beta;
+            a.smax_slot = k == r.n - 1 ? r.smax_slot : nullptr;
+            stage1d_small_body<WK>(a, red);
+            __syncthreads();
+        }
+        if (threadIdx.x < 64) step_control_wave(r.ctl, r.smax_slot, r.cp, threadIdx.x);
+        __syncthreads();
+        const unsigned long long h = __hip_atomic_load(&r.ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_load(&r.ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != acc0)
+            cur = (cur + ssp_result(r.n)) % 3;
+        if (h == HALT_REJECT) {
+            __syncthreads();  // every thread has read the halt word before it is cleared
+            if (threadIdx.x == 0) __hip_atomic_store(&r.ctl->halt, (unsigned long long)HALT_NONE, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            continue;
+        }
+        if (h != HALT_NONE) break;
+    }
+    if (threadIdx.x == 0) r.ctl->buf = cur;
+}
+
 template <int WK>
 static int launch_wk(const Stage1DArgs& a, hipStream_t st)
 {
@@ -180,6 +224,18 @@ int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
 {
     return with_kind(wk, [&](auto k) { return launch_wk<decltype(k)::value>(a, st); });
 }
+
+int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st)
+{
+    if (r.n < 1 || r.n > 3 || r.base.mx > SMALL1D) return -1;
+    if (r.base.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, r.base);
+    return with_kind(wk, [&](auto k) {
+        hipLaunchKernelGGL(k_run1d_small<decltype(k)::value>, dim3(1), dim3(SMALL1D_NT), 0, st, r);
+        return hipGetLastError() == hipSuccess ? 0 : -4;
+    });
+}
+
+bool run1d_fits(int mx) { return mx <= SMALL1D; }
 
 int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st)
 {

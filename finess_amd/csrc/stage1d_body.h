@@ -53,6 +53,22 @@ struct Stage1DStep {
     StepCtlParams cp;
 };
 
+// A run of 1D time steps in one launch (grids up to the single-workgroup
+// size): the stages of each attempt over the three plane sets, the device
+// controller after it, and the next attempt -- from the accepted state, or
+// from the same state with the controller's smaller dt after a rejection --
+// until the controller halts the run (the end, an error) or `max_attempts`.
+struct Stage1DRun {
+    Stage1DArgs base;   // the common fields; dt, halt from the controller block
+    double* buf[3];
+    int cur;            // plane set holding the state at entry
+    int n;              // SSP order 1..3
+    unsigned long long* smax_slot;
+    StepCtl* ctl;
+    StepCtlParams cp;
+    int max_attempts;
+};
+
 // SetWaveSpd of the advection callbacks with both sides equal (|a|) and the
 // Max(alpha1, local) of lib/1d/ConstructL.cpp:148 when global_alpha is set.
 __host__ __device__ __forceinline__ double alpha1d(const Stage1DArgs& a, double local)

@@ -204,6 +204,9 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
                                                     StageSmem<P, R, TI, TJ>& sm)
 {
     using S = StageSmem<P, R, TI, TJ>;
+    // A tile whose halo lies inside the grid (most of them) reads its cells
+    // directly, without the boundary map; the others map every index.
+    const bool inner = !a.src_ghosts && i0 >= 3 && j0 >= 3 && i0 + TI + 3 <= a.mx && j0 + TJ + 3 <= a.my;
     // two cells per thread and round: both cells' loads are issued before
     // either record is formed, so their memory latencies overlap
     constexpr int K = 2;
@@ -218,7 +221,14 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
             const int gi = i0 + ci - 3, gj = j0 + cj - 3;
             // corners are never read; nor is anything beyond every face's stencil
             ok[k] = c < S::NC && !(ri && rj) && gi <= a.mx + 2 && gj <= a.my + 2;
-            if (ok[k]) load_state<P>(a, gi, gj, q[k]);
+            if (ok[k]) {
+                if (inner) {
+                    const long long idx = (long long)gi * a.my + gj;
+                    for (int m = 0; m < P::MEQN; m++) q[k][m] = a.src[m * a.plane + idx];
+                } else {
+                    load_state<P>(a, gi, gj, q[k]);
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; k++) {

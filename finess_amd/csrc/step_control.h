@@ -3,6 +3,7 @@
 // a host round trip per step.  The host reads the block back once per chunk
 // of steps.
 #pragma once
+#include <hip/hip_runtime.h>
 
 namespace finess {
 
@@ -20,7 +21,30 @@ struct StepCtl {
     unsigned long long halt;
     long long err;      // FINESS_ERR_* when halt == HALT_ERROR
     long long nrec;     // accepted steps whose conservation record is written
+    long long buf;      // 1D run kernel: the plane set holding the last accepted state
 };
+
+// The SSP Runge-Kutta schemes of orders 1-3 as stages over three plane sets
+// (0 = A = q^n, never written during a step; 1 = B; 2 = C; -1 = none):
+// stage k of order n forms out = a1 qa + a2 qb + beta dt L(src), with the
+// coefficients of lib/2d/SetRKinfo.cpp:50-64 and the operand order of
+// lib/2d/FinSolveRK.cpp:158-211 (lib/1d/FinSolveRK.cpp:129-200).  One table
+// for every path that runs these stages.
+struct SspStage {
+    int src, qa, qb, out;
+    double a1, a2, beta;
+};
+
+__host__ __device__ __forceinline__ SspStage ssp_stage(int n, int k)
+{
+    if (k == 0) return {0, 0, -1, 1, 1.0, 0.0, 1.0};
+    if (n == 2) return {1, 1, 0, 2, 0.5, 0.5, 0.5};
+    if (k == 1) return {1, 0, 1, 2, 0.75, 0.25, 0.25};
+    return {2, 2, 0, 1, 2.0e0 / 3.0e0, 1.0e0 / 3.0e0, 2.0e0 / 3.0e0};
+}
+
+// the plane set that holds q^{n+1} after the last stage
+__host__ __device__ __forceinline__ int ssp_result(int n) { return n == 2 ? 2 : 1; }
 
 struct StepCtlParams {
     double tend, max_dt, cfl_max, cfl_target, dx, dy;

@@ -451,9 +451,17 @@ def main():
     ap.add_argument("--wide-version", default="js", choices=["js", "z", "fd"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (profiler counter calibration)")
+    ap.add_argument("--only", choices=["kh_lxw", "batch", "advection_1d"],
+                    help="only run one of the other workloads (for profiling its kernels)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.only:
+        fn = {"kh_lxw": lambda: kelvin_helmholtz_lxw(steps=args.steps, warmup=args.warmup, t_final=0.0),
+              "batch": lambda: batched_shallow_water(steps=args.steps),
+              "advection_1d": advection_1d_full}[args.only]
+        print(json.dumps(fn()), flush=True)
         return
     if args.wide:
         print(json.dumps(wide_order(args.wide, args.wide_version)), flush=True)

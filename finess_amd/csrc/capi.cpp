@@ -58,12 +58,13 @@ struct finess_ctx {
     unsigned long long* smax_h = nullptr;  // pinned copy
     MlpFrags* frags = nullptr;                    // device copy of the installed network
     unsigned long long* route_counts = nullptr;   // device [2][CFL_SPREAD]
-    // routing record of the current step: per-face slot (-1 smooth), troubled
-    // list, its length, and the troubled faces' fluxes of the current stage
-    int* route_idx = nullptr;
+    // routing record of the current step: per-tile troubled-face masks,
+    // troubled list, its length, and the troubled faces' fluxes of the
+    // current stage (in the stencil precision)
+    RouteTileRec* route_tile = nullptr;
     int* route_list = nullptr;
     unsigned long long* route_n = nullptr;
-    double* route_flux = nullptr;
+    void* route_flux = nullptr;
     // limited Lax-Wendroff: interface fluxes (high-order, then limited in
     // place; Lax-Friedrichs), per-cell fractions, global wave-speed words
     double *mpp_fh = nullptr, *mpp_gh = nullptr, *mpp_fl = nullptr, *mpp_gl = nullptr, *mpp_amin = nullptr;
@@ -225,8 +226,8 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
     a.route_exact = p.routing == FINESS_ROUTE_ORACLE;
     a.route_tau = p.route_tau;
     a.route_counts = c->route_counts;
-    if (stepping && p.route_per_step && c->route_idx) {
-        a.route_idx = c->route_idx;
+    if (stepping && p.route_per_step && c->route_tile) {
+        a.route_tile = c->route_tile;
         a.route_list = c->route_list;
         a.route_flux = c->route_flux;
         bool decide = c->route_fresh && (c->rec_age < 0 || c->rec_age >= p.route_per_step);
@@ -987,11 +988,11 @@ int finess_create(const finess_params* pin, finess_ctx** out)
     if (!rc) rc = alloc(c, (void**)&c->ctl, sizeof(StepCtl));
     if (!rc && p.routing != FINESS_ROUTE_NONE && p.ndims == 2) {
         const size_t nf = (size_t)(p.mx + 1) * p.my + (size_t)p.mx * (p.my + 1);
-        rc = alloc(c, (void**)&c->route_idx, nf * sizeof(int));
+        rc = alloc(c, (void**)&c->route_tile, (size_t)routed_tiles(p.mx, p.my) * sizeof(RouteTileRec));
         if (!rc) rc = alloc(c, (void**)&c->route_list, nf * sizeof(int));
         if (!rc) rc = alloc(c, (void**)&c->route_n, 4 * sizeof(unsigned long long));
         if (!rc) rc = launch_fill_u64(c->route_n, 4, 0ull, c->st) ? FINESS_ERR_HIP : 0;
-        if (!rc) rc = alloc(c, (void**)&c->route_flux, nf * 6 * sizeof(double));
+        if (!rc) rc = alloc(c, &c->route_flux, nf * 6 * (p.precision == FINESS_PREC_FP32 ? sizeof(float) : sizeof(double)));
     }
     if (!rc) rc = launch_fill_u64(c->route_counts, 2 * CFL_SPREAD, 0ull, c->st) ? FINESS_ERR_HIP : 0;
     if (!rc) rc = launch_fill_u64(c->smax, 2 * CFL_SPREAD * NSLOT, 0ull, c->st) ? FINESS_ERR_HIP : 0;
@@ -1020,7 +1021,7 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->smax);
     hipFree(c->frags);
     hipFree(c->route_counts);
-    hipFree(c->route_idx);
+    hipFree(c->route_tile);
     hipFree(c->route_list);
     hipFree(c->route_n);
     hipFree(c->ctl);

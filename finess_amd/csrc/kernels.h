@@ -32,6 +32,8 @@ int launch_lxw_flux1d(const Stage1DArgs& a, double dt, int time_order, double* F
 int launch_lxw_flux(int physics, int fp32, const LxwArgs& a, hipStream_t st);
 
 int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st);
+// tiles of the routed kernels (records of the per-step routing record)
+long long routed_tiles(int mx, int my);
 
 struct HarvestArgs;
 struct MlpFrags;

@@ -33,10 +33,9 @@ __device__ __forceinline__ void block_max2(double ax, double ay, unsigned long l
     }
 }
 
-// Waves per SIMD asked of the register allocator.  The LDS footprint of a
-// 7 x 32 tile admits three workgroups per CU, but squeezing the FP64 face
-// computation into 168 VGPRs (3 waves) spills and measured 4% slower on
-// MI355X than 182 VGPRs at 2 waves.
+// The minimum waves per SIMD asked of the register allocator.  The FP64
+// kernel compiles to 156-166 VGPRs without spills, so 3 waves per SIMD run,
+// as many as the tile's LDS (about 40 KB, 3 workgroups per CU) allows.
 #ifndef FINESS_WPE
 #define FINESS_WPE 2
 #endif
@@ -84,14 +83,24 @@ static int launch_stage(const StageArgs& a0, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// The tile of the fused stage kernel (cells in i, cells in j; at most 256
+// faces of each direction).  15 x 16 computes 511 faces and 426 cell
+// records for 240 cells, 7 x 32 487 faces and 458 records for 224: 4%
+// less time per FP64 stage on MI355X (1024^2 Euler).
+#ifndef FINESS_STAGE_TI
+#define FINESS_STAGE_TI 15
+#define FINESS_STAGE_TJ 16
+#endif
+constexpr int STI = FINESS_STAGE_TI, STJ = FINESS_STAGE_TJ;
+
 template <class P, typename R, bool LXW>
 static int dispatch_weno(int wk, const StageArgs& a, hipStream_t st)
 {
     switch (wk) {
-    case WENO_JS5_P2: return launch_stage<P, R, WENO_JS5_P2, LXW, 7, 32>(a, st);
-    case WENO_FD5: return launch_stage<P, R, WENO_FD5, LXW, 7, 32>(a, st);
-    case WENO_JS5_POW: return launch_stage<P, R, WENO_JS5_POW, LXW, 7, 32>(a, st);
-    case WENO_Z5: return launch_stage<P, R, WENO_Z5, LXW, 7, 32>(a, st);
+    case WENO_JS5_P2: return launch_stage<P, R, WENO_JS5_P2, LXW, STI, STJ>(a, st);
+    case WENO_FD5: return launch_stage<P, R, WENO_FD5, LXW, STI, STJ>(a, st);
+    case WENO_JS5_POW: return launch_stage<P, R, WENO_JS5_POW, LXW, STI, STJ>(a, st);
+    case WENO_Z5: return launch_stage<P, R, WENO_Z5, LXW, STI, STJ>(a, st);
     default: return -1;
     }
 }

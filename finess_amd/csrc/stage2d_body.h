@@ -78,6 +78,18 @@ __device__ __forceinline__ int xcd_remap(int b, int n)
     return (b % NXCD) * (n / NXCD) + b / NXCD;
 }
 
+// Routing record of one tile: the troubled faces the tile owns, as one
+// 64-bit ballot per wave of 64 faces and direction (bit = lane = the face's
+// thread within the wave), and the troubled-list slot of its first troubled
+// face of each direction (offset of the direction's list not included).
+// A face on the tile's far edge is owned by the next tile, where it is that
+// tile's first row (x) or column (y) of faces.
+struct RouteTileRec {
+    unsigned long long mask[2][4];
+    int base[2];
+    int pad[2];
+};
+
 struct StageArgs {
     // state planes, field-major: q[m][i][j] at m*plane + i*ld + j (ld = my)
     const double* src;     // stencil operand
@@ -117,22 +129,23 @@ struct StageArgs {
     int route_exact;
     double route_tau;
     unsigned long long* route_counts;  // [2][CFL_SPREAD]: troubled faces, faces
-    // per-step routing record (null: indicator only, nothing stored).  Face
-    // ids: x faces i*my + j, then y faces (mx+1)*my + i*(my+1) + j.
-    //   route_idx[face]  -1 for a smooth face, else its slot in the troubled list
+    // per-step routing record (null: indicator only, nothing stored).
+    //   route_tile[tile] the tile's troubled owned faces (RouteTileRec; tile
+    //                    = ti * ntj + tj of the routed kernels' 7 x 32 tiles)
     //   route_list[slot] the face's (i, j) packed as i << 16 | j; troubled x
     //                    faces take slots 0.., y faces (mx+1)*my..
     //   route_n[2]       number of troubled x faces, y faces (zero when the
     //                    indicator starts a record)
     //   route_n_clear[2] set to zero by the indicator stage: the counter pair
     //                    of the previous record, which the next record uses
-    //   route_flux[slot] 5 flux components + alpha of the troubled face
+    //   route_flux[slot] 5 flux components + alpha of the troubled face, in the
+    //                    stencil precision R (6 floats in the FP32 form)
     // route_mask_read: 0 = the indicator decides and records, 1 = read the record
-    int* route_idx;
+    struct RouteTileRec* route_tile;
     int* route_list;
     unsigned long long* route_n;
     unsigned long long* route_n_clear;
-    double* route_flux;
+    void* route_flux;
     int route_mask_read;
     // optional face-flux / L output for the drop-in ConstructL (may be null):
     double* Lout;          // L planes (mx*my)

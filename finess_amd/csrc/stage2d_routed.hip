@@ -31,7 +31,6 @@ struct RouteTile {
     R fy[P::MEQN][NFY];         // y-face fluxes
     int list[NF];               // face ids: x face f -> f, y face f -> NFX + f
     int nt, ns;                 // troubled / smooth faces of the tile
-    int rcnt[2][4], rbase[2];   // recorded troubled faces per direction and wave, the tile's first slots
     double red[2][4];
 };
 
@@ -40,14 +39,6 @@ struct MlpTile {
     uint16_t feat[4][64][MLP_FEAT_LD];
     float lo[4][64];
 };
-
-// Index of a face in the per-step routing record.
-__device__ __forceinline__ long long mask_index(int dir, const FaceGeom& g, int i0, int j0, const StageArgs& a)
-{
-    const long long nfx = (long long)(a.mx + 1) * a.my;
-    return dir == 0 ? (long long)(i0 + g.fi) * a.my + (j0 + g.fj)
-                    : nfx + (long long)(i0 + g.fi) * (a.my + 1) + (j0 + g.fj);
-}
 
 template <class P, typename R, int TI, int TJ>
 __device__ __forceinline__ void stencil_records(const StageSmem<P, R, TI, TJ>& s, const FaceGeom& g,
@@ -68,17 +59,49 @@ __device__ __forceinline__ bool face_owned(int dir, const FaceGeom& g, int i0, i
 // Stores a troubled face in list slot `slot` of the step's record: the list
 // entry is the face's cell-index pair (gi, gj) packed in 16 bits each (the
 // host limits routed grids to 65535 cells per side).
-__device__ __forceinline__ void record_troubled(long long mi, int gi, int gj, int slot, const StageArgs& a)
+__device__ __forceinline__ void record_troubled(int gi, int gj, long long slot, const StageArgs& a)
 {
     a.route_list[slot] = (int)(((unsigned)gi << 16) | (unsigned)gj);
-    a.route_idx[mi] = slot;
 }
 
-// EXACT: route by the weight-departure label itself, else by the MLP.  With
-// a.route_idx set, the decisions are recorded for the step's later stages.
+// The list slot of a face from a tile record: thread tt of the tile, -1 when
+// the face is smooth.  The slot counts the troubled faces of lower waves and
+// lanes of the same direction.
+__device__ __forceinline__ int record_slot(const unsigned long long* mask, int base, int tt)
+{
+    const int w = tt >> 6, lane = tt & 63;
+    const unsigned long long m = mask[w];
+    if (!((m >> lane) & 1ull)) return -1;
+    int s = base + __popcll(m & ((1ull << lane) - 1ull));
+    for (int k = 0; k < w; k++) s += __popcll(mask[k]);
+    return s;
+}
+
+// Loads the routing records of a tile and of its next tiles in i and j
+// (rt[0], rt[1], rt[2]; zero where there is none) into LDS: threads below
+// 3 x 10 read one 8-byte word each before the tile load (record_fetch) and
+// store it after the load (record_store), ahead of the caller's barrier.
+template <int TI>
+__device__ __forceinline__ unsigned long long record_fetch(const StageArgs& a, int t, int tile, int ti, int tj)
+{
+    constexpr int RW = sizeof(RouteTileRec) / sizeof(unsigned long long);
+    if (t >= 3 * RW) return 0ull;
+    const int rk = t / RW, rwi = t - rk * RW;
+    const int nti = (a.mx + TI - 1) / TI;
+    const int other = rk == 0 ? tile : rk == 1 ? (ti + 1 < nti ? tile + a.ntj : -1) : (tj + 1 < a.ntj ? tile + 1 : -1);
+    return other >= 0 ? reinterpret_cast<const unsigned long long*>(a.route_tile + other)[rwi] : 0ull;
+}
+__device__ __forceinline__ void record_store(RouteTileRec* rt, int t, unsigned long long rword)
+{
+    constexpr int RW = sizeof(RouteTileRec) / sizeof(unsigned long long);
+    if (t < 3 * RW) reinterpret_cast<unsigned long long*>(&rt[t / RW])[t % RW] = rword;
+}
+
+// EXACT: route by the weight-departure label itself, else by the MLP; the
+// decisions hold for this stage only (a record is made by k_classify).
 // The learned FP32 kernel is held to 3 waves per SIMD (168 registers
 // including the MFMA accumulators; its LDS allows 3 workgroups per CU).
-template <class P, typename R, int WK, int TI, int TJ, bool EXACT, bool EVAL>
+template <class P, typename R, int WK, int TI, int TJ, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : (sizeof(R) == 4 ? 3 : 2)))) void k_stage2d_routed(StageArgs a)
 {
     constexpr int NT = 256, M = P::MEQN, NV = P::NV;
@@ -94,12 +117,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
 
     const FaceGeom gx = face_geom<TI, TJ>(t, 0, i0, j0, a);
     const FaceGeom gy = face_geom<TI, TJ>(t, 1, i0, j0, a);
-    static_assert(EVAL, "the reusing stages run k_route_full + k_stage2d_lookup");
     bool tx = false, ty = false;
     double pa[M], pb[M];
     phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
     if (t == 0) { sm.nt = 0; sm.ns = 0; }
-    if constexpr (EVAL && !EXACT) {
+    if constexpr (!EXACT) {
         __shared__ MlpTile ml;
         mlp_stage_frags(a.frags, &ml.fr, t, NT);
         phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm.s);
@@ -119,30 +141,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     } else {
         phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm.s);
         __syncthreads();
-        if (EVAL) {  // EXACT
-            if (gx.valid) {
-                R rec[6][NV];
-                stencil_records<P, R, TI, TJ>(sm.s, gx, rec);
-                tx = face_departure<P, R>(rec, 0, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
-            }
-            if (gy.valid) {
-                R rec[6][NV];
-                stencil_records<P, R, TI, TJ>(sm.s, gy, rec);
-                ty = face_departure<P, R>(rec, 1, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
-            }
+        if (gx.valid) {
+            R rec[6][NV];
+            stencil_records<P, R, TI, TJ>(sm.s, gx, rec);
+            tx = face_departure<P, R>(rec, 0, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
         }
-    }
-    // the record: smooth owned faces are marked at once; the troubled ones
-    // are counted per wave (ballots) and get their list slots from one
-    // global atomic per tile and direction, taken below (x faces fill the
-    // list from slot 0, y faces from slot (mx+1) my)
-    const bool ox = a.route_idx && gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a);
-    const bool oy = a.route_idx && gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a);
-    const unsigned long long bx = __ballot(ox && tx), by = __ballot(oy && ty);
-    if (a.route_idx) {
-        if (ox && !tx) a.route_idx[mask_index(0, gx, i0, j0, a)] = -1;
-        if (oy && !ty) a.route_idx[mask_index(1, gy, i0, j0, a)] = -1;
-        if (lane == 0) { sm.rcnt[0][w] = __popcll(bx); sm.rcnt[1][w] = __popcll(by); }
+        if (gy.valid) {
+            R rec[6][NV];
+            stencil_records<P, R, TI, TJ>(sm.s, gy, rec);
+            ty = face_departure<P, R>(rec, 1, a.pp, R(a.eps), R(a.alpha_scaling)) > R(a.route_tau);
+        }
     }
     // compaction: troubled faces to the front of the list, smooth to the back
     if (gx.valid) {
@@ -155,12 +163,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
     }
     __syncthreads();
     const int nt = sm.nt, n = nt + sm.ns;
-    if (t < 2 && a.route_n_clear && blockIdx.x == 0) a.route_n_clear[t] = 0ull;
-    if (t < 2 && a.route_idx) {  // thread d takes the slots of direction d
-        int c = 0;
-        for (int k = 0; k < NT / 64; k++) c += sm.rcnt[t][k];
-        sm.rbase[t] = c ? (int)atomicAdd(&a.route_n[t], (unsigned long long)c) : 0;
-    }
     if (t == 0 && a.route_counts) {
         const int b = blockIdx.x % CFL_SPREAD;
         atomicAdd(&a.route_counts[b], (unsigned long long)nt);
@@ -194,19 +196,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 :
         }
     }
     __syncthreads();
-    if (bx | by) {  // the wave's troubled owned faces (the lane's own bits of the ballots)
-        int bsx = sm.rbase[0], bsy = (a.mx + 1) * a.my + sm.rbase[1];
-        for (int k = 0; k < w; k++) { bsx += sm.rcnt[0][k]; bsy += sm.rcnt[1][k]; }
-        const unsigned long long below = (1ull << lane) - 1ull;
-        if ((bx >> lane) & 1ull) {
-            const FaceGeom h = face_geom<TI, TJ>(t, 0, i0, j0, a);
-            record_troubled(mask_index(0, h, i0, j0, a), i0 + h.fi, j0 + h.fj, bsx + __popcll(bx & below), a);
-        }
-        if ((by >> lane) & 1ull) {
-            const FaceGeom h = face_geom<TI, TJ>(t, 1, i0, j0, a);
-            record_troubled(mask_index(1, h, i0, j0, a), i0 + h.fi, j0 + h.fj, bsy + __popcll(by & below), a);
-        }
-    }
     // L = -(Fhat(i+1) - Fhat(i))/dx - (Ghat(j+1) - Ghat(j))/dy and the stage
     // update, in the operation order of the plain kernel
     {
@@ -267,6 +256,7 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
     using S = StageSmem<P, R, TI, TJ>;
     __shared__ S ss;
     __shared__ int cnt[4][4];  // per wave: recorded troubled x, y; troubled faces, faces
+    __shared__ unsigned long long bal[2][4];
     __shared__ int rbase[2];
     if (stage_halted(a)) return;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -309,14 +299,12 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
     }
     tx = tx && gx.valid;
     ty = ty && gy.valid;
-    // the record, as in k_stage2d_routed: smooth owned faces marked at once,
-    // troubled owned faces counted per wave and given slots from one atomic
-    // per tile and direction
+    // the record: the troubled owned faces as one ballot per wave and
+    // direction; their list slots from one global atomic per tile and
+    // direction (x faces fill the list from slot 0, y faces from (mx+1) my)
     const bool ox = gx.valid && face_owned(0, gx, i0, j0, TI, TJ, a);
     const bool oy = gy.valid && face_owned(1, gy, i0, j0, TI, TJ, a);
     const unsigned long long bx = __ballot(ox && tx), by = __ballot(oy && ty);
-    if (ox && !tx) a.route_idx[mask_index(0, gx, i0, j0, a)] = -1;
-    if (oy && !ty) a.route_idx[mask_index(1, gy, i0, j0, a)] = -1;
     const int nt = __popcll(__ballot(tx)) + __popcll(__ballot(ty));
     const int nf = __popcll(__ballot(gx.valid)) + __popcll(__ballot(gy.valid));
     if (lane == 0) {
@@ -324,6 +312,8 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
         cnt[1][w] = __popcll(by);
         cnt[2][w] = nt;
         cnt[3][w] = nf;
+        bal[0][w] = bx;
+        bal[1][w] = by;
     }
     __syncthreads();
     if (t < 2) {  // thread d takes the slots of direction d
@@ -337,13 +327,15 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
         atomicAdd(&a.route_counts[(t - 2) * CFL_SPREAD + blockIdx.x % CFL_SPREAD], (unsigned long long)c);
     }
     __syncthreads();
+    RouteTileRec* rec = a.route_tile + tile;
+    if (t < 8) rec->mask[t >> 2][t & 3] = bal[t >> 2][t & 3];
+    else if (t < 10) rec->base[t - 8] = rbase[t - 8];
     if (bx | by) {
-        int bsx = rbase[0], bsy = (a.mx + 1) * a.my + rbase[1];
+        long long bsx = rbase[0], bsy = (long long)(a.mx + 1) * a.my + rbase[1];
         for (int k = 0; k < w; k++) { bsx += cnt[0][k]; bsy += cnt[1][k]; }
         const unsigned long long below = (1ull << lane) - 1ull;
-        if (ox && tx) record_troubled(mask_index(0, gx, i0, j0, a), i0 + gx.fi, j0 + gx.fj, bsx + __popcll(bx & below), a);
-        if (oy && ty)
-            record_troubled(mask_index(1, gy, i0, j0, a), i0 + gy.fi, j0 + gy.fj, bsy + __popcll(by & below), a);
+        if (ox && tx) record_troubled(i0 + gx.fi, j0 + gx.fj, bsx + __popcll(bx & below), a);
+        if (oy && ty) record_troubled(i0 + gy.fi, j0 + gy.fj, bsy + __popcll(by & below), a);
     }
 }
 
@@ -382,9 +374,9 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
     const R alpha = face_flux<P, R, WK, false, 6>(cv, 0, 1, DIR, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh);
-    double* out = a.route_flux + 6 * slot;
-    for (int m = 0; m < M; m++) out[m] = double(fh[m]);
-    out[M] = double(alpha);
+    R* out = static_cast<R*>(a.route_flux) + 6 * slot;
+    for (int m = 0; m < M; m++) out[m] = fh[m];
+    out[M] = alpha;
 }
 
 // Characteristic WENO5 fluxes of the listed troubled faces of this stage's
@@ -416,6 +408,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     __shared__ S sm;
     __shared__ R fy[M][TI * (TJ + 1)];
     __shared__ double red[2][NT / 64];
+    // the routing records of this tile (0), of the next tile in i (1, whose
+    // first x faces are this tile's far x faces) and in j (2, likewise for y)
+    __shared__ RouteTileRec rt[3];
     if (stage_halted(a)) return;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / a.ntj, tj = tile - ti * a.ntj;
@@ -423,13 +418,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const FaceGeom gx = face_geom<TI, TJ>(t, 0, i0, j0, a);
     const FaceGeom gy = face_geom<TI, TJ>(t, 1, i0, j0, a);
-    // the step's routing and the update operands, read before the tile load
-    const int sx = gx.valid ? a.route_idx[mask_index(0, gx, i0, j0, a)] : -1;
-    const int sy = gy.valid ? a.route_idx[mask_index(1, gy, i0, j0, a)] : -1;
+    // the step's routing records and the update operands, read before the
+    // tile load
+    const unsigned long long rword = record_fetch<TI>(a, t, tile, ti, tj);
     double pa[M], pb[M];
     phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
     phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
+    record_store(rt, t, rword);
     __syncthreads();
+    // list slots of the thread's faces (-1: smooth): an owned face from this
+    // tile's record, a far-edge face from the next tile's, where it is face
+    // fj (x) or fi (TJ + 1) (y) of that tile
+    int sx = -1, sy = -1;
+    if (gx.valid) {
+        sx = face_owned(0, gx, i0, j0, TI, TJ, a) ? record_slot(rt[0].mask[0], rt[0].base[0], t)
+                                                   : record_slot(rt[1].mask[0], rt[1].base[0], gx.fj);
+    }
+    if (gy.valid) {
+        sy = face_owned(1, gy, i0, j0, TI, TJ, a) ? record_slot(rt[0].mask[1], rt[0].base[1], t)
+                                                   : record_slot(rt[2].mask[1], rt[2].base[1], gy.fi * (TJ + 1));
+        if (sy >= 0) sy += (a.mx + 1) * a.my;
+    }
     double ax = 0.0, ay = 0.0;
     for (int dir = 0; dir < 2; dir++) {
         const FaceGeom& h = dir ? gy : gx;
@@ -438,9 +447,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
         R fh[M];
         R alpha;
         if (slot >= 0) {
-            const double* in = a.route_flux + 6 * (long long)slot;
-            for (int m = 0; m < M; m++) fh[m] = R(in[m]);
-            alpha = R(in[M]);
+            const R* in = static_cast<const R*>(a.route_flux) + 6 * (long long)slot;
+            for (int m = 0; m < M; m++) fh[m] = in[m];
+            alpha = in[M];
         } else {
             alpha = face_flux_linear<P, R, S::NC>(sm.cv, h.base, h.stride, dir, a.pp, R(a.alpha_scaling), fh);
         }
@@ -504,15 +513,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     }
 }
 
+// the tile of the routed kernels (at most 256 faces of each direction)
+#ifndef FINESS_ROUTE_TI
+#define FINESS_ROUTE_TI 7
+#define FINESS_ROUTE_TJ 32
+#endif
+constexpr int ROUTE_TI = FINESS_ROUTE_TI, ROUTE_TJ = FINESS_ROUTE_TJ;
+
 template <typename R>
 static int launch_routed(const StageArgs& a0, hipStream_t st)
 {
-    constexpr int TI = 7, TJ = 32;
+    constexpr int TI = ROUTE_TI, TJ = ROUTE_TJ;
     StageArgs a = a0;
     const int nti = (a.mx + TI - 1) / TI, ntj = (a.my + TJ - 1) / TJ;
     a.ntj = ntj;
     const dim3 grid(nti * ntj), block(256);
-    if (a.route_mask_read || a.route_idx) {
+    if (a.route_mask_read || a.route_tile) {
         // a new record: the indicator first, then the stage from the record
         if (!a.route_mask_read) {
             if (a.route_exact)
@@ -523,16 +539,21 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
         hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(1024), block, 0, st, a);
         hipLaunchKernelGGL((k_stage2d_lookup<Euler2D, R, TI, TJ>), grid, block, 0, st, a);
     } else if (a.route_exact)
-        hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, true, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, true>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, false, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, false>), grid, block, 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+long long routed_tiles(int mx, int my)
+{
+    return (long long)((mx + ROUTE_TI - 1) / ROUTE_TI) * ((my + ROUTE_TJ - 1) / ROUTE_TJ);
 }
 
 int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hipStream_t st)
 {
     if (physics != PHYS_EULER || wk != WENO_JS5_P2) return -1;
-    if ((a.route_mask_read || a.route_idx) && !(a.route_idx && a.route_list && a.route_n && a.route_flux && a.out))
+    if ((a.route_mask_read || a.route_tile) && !(a.route_tile && a.route_list && a.route_n && a.route_flux && a.out))
         return -1;
     return fp32 ? launch_routed<float>(a, st) : launch_routed<double>(a, st);
 }

@@ -320,6 +320,13 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "2D stage launch failed") : 0;
 }
 
+// Bytes per stored time-integrated flux value: the 2D kernels keep them in
+// their arithmetic (FP32 in the FP32 form); the 1D path in FP64.
+static size_t lxw_flux_bytes(const finess_params& p)
+{
+    return p.ndims == 2 && p.precision == FINESS_PREC_FP32 ? sizeof(float) : sizeof(double);
+}
+
 int lxw_fluxes(finess_ctx* c, int src, double dt)
 {
     const finess_params& p = c->p;
@@ -963,7 +970,8 @@ int finess_create(const finess_params* pin, finess_ctx** out)
         if (!rc) rc = alloc(c, (void**)&c->pif_fy, (size_t)p.mx * (p.my + 1) * c->M * sizeof(double));
     }
     if (!rc && p.time_stepping_method != FINESS_TS_RK) {
-        const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6) * sizeof(double);
+        // (the FP32 form stores the time-integrated fluxes in FP32)
+        const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6) * lxw_flux_bytes(p);
         rc = alloc(c, (void**)&c->lxwF, lb);
         if (!rc) rc = alloc(c, (void**)&c->lxwG, lb);
         if (!rc && p.mpp_limiter) {
@@ -1301,10 +1309,11 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     double *Fd = c->lxwF, *Gd = c->lxwG;
     bool own = false;
     const size_t lb = (size_t)c->M * (p.mx + 6) * (p.my + 6);
+    const size_t vb = lxw_flux_bytes(p);  // the kernel's precision
     if (!Fd) {
         own = true;
-        if ((rc = alloc(c, (void**)&Fd, lb * sizeof(double)))) return rc;
-        if ((rc = alloc(c, (void**)&Gd, lb * sizeof(double)))) { hipFree(Fd); return rc; }
+        if ((rc = alloc(c, (void**)&Fd, lb * vb))) return rc;
+        if ((rc = alloc(c, (void**)&Gd, lb * vb))) { hipFree(Fd); return rc; }
     }
     LxwArgs a;
     memset(&a, 0, sizeof a);
@@ -1322,12 +1331,16 @@ int finess_construct_integrated_r(finess_ctx* c, double dt, const double* q, con
     a.dy = p.dy;
     a.pp.gamma = p.gamma;
     std::vector<double> hF(lb), hG(lb);
+    std::vector<float> hF32(vb == sizeof(float) ? lb : 0), hG32(hF32.size());
+    void* dstF = vb == sizeof(float) ? (void*)hF32.data() : (void*)hF.data();
+    void* dstG = vb == sizeof(float) ? (void*)hG32.data() : (void*)hG.data();
     hipError_t e = launch_lxw_flux(p.physics, p.precision == FINESS_PREC_FP32, a, c->st) ? hipErrorLaunchFailure : hipSuccess;
-    if (e == hipSuccess) e = hipMemcpyAsync(hF.data(), Fd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(hG.data(), Gd, lb * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dstF, Fd, lb * vb, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dstG, Gd, lb * vb, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     if (own) { hipFree(Fd); hipFree(Gd); }
     if (e != hipSuccess) return fail(c, FINESS_ERR_HIP, hipGetErrorString(e));
+    for (size_t k = 0; k < hF32.size(); k++) { hF[k] = hF32[k]; hG[k] = hG32[k]; }
     const int g = p.mbc, M = c->M;
     const long long ldq = p.my + 2 * g, le = p.my + 6, lpl = (long long)(p.mx + 6) * le;
     for (int i = -3; i < p.mx + 3; i++)

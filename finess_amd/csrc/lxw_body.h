@@ -143,8 +143,8 @@ struct LxwPhys<ShallowWater2D> {
 
 struct LxwArgs {
     const double* q;     // state planes (mx*my), boundary-mapped on load
-    double* F;           // out: planes (mx+6)*(my+6), extended index (i+3)*(my+6)+(j+3)
-    double* G;
+    void* F;             // out: planes (mx+6)*(my+6), extended index (i+3)*(my+6)+(j+3),
+    void* G;             //      of R (the kernel's arithmetic: FP64, or FP32 for the FP32 form)
     int mx, my;
     long long plane;
     int bc[4];
@@ -276,8 +276,8 @@ __host__ __device__ __forceinline__ void lxw_phase_cell(int tid, int e0i, int e0
     const long long lld = a.my + 6, lpl = (long long)(a.mx + 6) * lld;
     const long long o = (long long)ei * lld + ej;
     for (int m = 0; m < M; m++) {
-        a.F[m * lpl + o] = double(sm.f[m][c] + R(0.5) * dt * (f_t[m] + div_lp(dt, R(3)) * f_tt[m]));
-        a.G[m * lpl + o] = double(sm.g[m][c] + R(0.5) * dt * (g_t[m] + div_lp(dt, R(3)) * g_tt[m]));
+        static_cast<R*>(a.F)[m * lpl + o] = sm.f[m][c] + R(0.5) * dt * (f_t[m] + div_lp(dt, R(3)) * f_tt[m]);
+        static_cast<R*>(a.G)[m * lpl + o] = sm.g[m][c] + R(0.5) * dt * (g_t[m] + div_lp(dt, R(3)) * g_tt[m]);
     }
 }
 

@@ -97,9 +97,10 @@ struct StageArgs {
     const double* qb;      // UpdateSoln's "qnew" operand (a2)
     double* out;
     // Lax-Wendroff: time-integrated fluxes on the extended grid
-    // [-3, mx+2] x [-3, my+2], plane (mx+6)*(my+6), F at m*lplane + (i+3)*(my+6) + (j+3)
-    const double* lxw_F;
-    const double* lxw_G;
+    // [-3, mx+2] x [-3, my+2], plane (mx+6)*(my+6), F at m*lplane + (i+3)*(my+6) + (j+3),
+    // stored in the stencil precision R (FP32 in the FP32 form)
+    const void* lxw_F;
+    const void* lxw_G;
     // ghosts stored in src instead of mapped (drop-in ConstructL mode):
     // src planes are (mx+2*gh) x (my+2*gh)
     int src_ghosts;
@@ -308,7 +309,7 @@ __host__ __device__ __forceinline__ R face_setup(const R (*cv)[NC], int base, in
 // twice the reconstructed characteristic flux of field k.
 template <class P, typename R, int WK, bool LXW, int NC>
 __host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, int stride, int dir,
-                                                   const double* lflux, long long lbase, long long lstride,
+                                                   const R* lflux, long long lbase, long long lstride,
                                                    long long lplane, const typename P::template Eig<R>& e,
                                                    R la, const WenoParams<R>& wp4, int part, R w[P::MEQN])
 {
@@ -322,7 +323,7 @@ __host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, 
         for (int v = 0; v < NV; v++) rec[v] = cv[v][base + s * stride];
         P::state(rec, dir, q);
         if (LXW) {
-            for (int k = 0; k < M; k++) f[k] = R(lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride]);
+            for (int k = 0; k < M; k++) f[k] = lflux[P::rot_q(k, dir) * lplane + lbase + s * lstride];
         } else {
             P::flux(q, rec, f);
         }
@@ -338,7 +339,7 @@ __host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, 
 // x, :215-358 for y).  base/stride locate the 6 stencil records i-3..i+2.
 template <class P, typename R, int WK, bool LXW, int NC>
 __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int stride, int dir,
-                                                const double* lflux, long long lbase, long long lstride,
+                                                const R* lflux, long long lbase, long long lstride,
                                                 long long lplane, const PhysParams& pp,
                                                 const WenoParams<R>& wp, R ascale, R fh[P::MEQN],
                                                 R amin = R(0))
@@ -424,7 +425,8 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
     const R amin = a.galpha ? R(a.galpha[dir]) : R(0);
-    const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, dir ? a.lxw_G : a.lxw_F, lbase,
+    const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir,
+                                                    static_cast<const R*>(dir ? a.lxw_G : a.lxw_F), lbase,
                                                     lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh, amin);
     (void)sizeof(S);
     for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];

@@ -60,8 +60,17 @@ STAGES_PER_STEP = 3
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
 # counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 153.6e6   # mean over the 12 stages of a 4-step indicator interval
-TRAFFIC_PLAIN_1024 = 113.8e6
+TRAFFIC_ROUTED_1024 = 138.9e6   # mean over the 12 stages of a 4-step indicator interval
+TRAFFIC_PLAIN_1024 = 113.9e6
+# VALU instructions per launch at 1024x1024 (rocprofv3 SQ_INSTS_VALU and the
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters, same profile files): a wave64
+# VALU instruction holds a SIMD for 4 cycles, so at 100% issue a launch takes
+# N x 4 / (1024 SIMDs x 2.4 GHz).  The bench divides that by the measured
+# launch time.
+VALU_PLAIN_1024 = 43.79e6        # k_stage2d, FP64
+VALU_F64_PLAIN_1024 = 35.26e6    # of which FP64 arithmetic
+VALU_LOOKUP_1024 = 12.44e6       # k_stage2d_lookup, FP32 stencil
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 T_FINAL = 0.3
 T_TIMED = 0.15
 # the indicator runs at the first stage of every ROUTE_INTERVAL-th step; the
@@ -529,7 +538,13 @@ def main():
         "plain_fp64": {"value": round(v_plain, 3), "ms_per_step": round(el_p / args.steps * 1e3, 4),
                        "avg_stage_launch_us": round(launch_p * 1e6, 2),
                        "hbm_frac": round(bytes_per_launch / launch_p / 1e9 / HBM_PEAK_GBS, 4),
-                       "traffic": TRAFFIC_PLAIN_1024 if mx == 1024 else None},
+                       "traffic": TRAFFIC_PLAIN_1024 if mx == 1024 else None,
+                       # the stage is bound by VALU issue, FP64 arithmetic above all
+                       "valu": None if mx != 1024 else {
+                           "issue_frac": round(VALU_PLAIN_1024 * 4 / SIMDS / CLOCK_HZ / launch_p, 4),
+                           "f64_issue_frac": round(VALU_F64_PLAIN_1024 * 4 / SIMDS / CLOCK_HZ / launch_p, 4),
+                           "instructions_per_launch": VALU_PLAIN_1024,
+                           "f64_instructions_per_launch": VALU_F64_PLAIN_1024}},
         "speedup_vs_plain_fp64": round(v_learned / v_plain, 4),
         "indicator": ind.info,
         "accuracy": accuracy,

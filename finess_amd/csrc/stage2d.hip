@@ -54,15 +54,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int t = threadIdx.x;
     double pa[P::MEQN], pb[P::MEQN];
     phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
-    phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
-    __syncthreads();
+    // a tile whose cells all have rho w = 0 (every 2D Euler problem that
+    // starts without it) skips that field's reconstructions
+    const bool nz3 = __syncthreads_or(phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm));
+    const bool skip3 = !LXW && !nz3;
     double ax = 0.0, ay = 0.0;
     double Lx[P::MEQN];
-    phase_faces<P, R, WK, LXW, TI, TJ>(t, 0, i0, j0, a, sm, ax);
+    phase_faces<P, R, WK, LXW, TI, TJ>(t, 0, i0, j0, a, sm, ax, skip3);
     __syncthreads();
     phase_lx<P, R, TI, TJ>(t, a, sm, Lx);
     __syncthreads();
-    phase_faces<P, R, WK, LXW, TI, TJ>(t, 1, i0, j0, a, sm, ay);
+    phase_faces<P, R, WK, LXW, TI, TJ>(t, 1, i0, j0, a, sm, ay, skip3);
     __syncthreads();
     phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
     if (a.smax) {
@@ -92,6 +94,13 @@ static int launch_stage(const StageArgs& a0, hipStream_t st)
 #define FINESS_STAGE_TJ 16
 #endif
 constexpr int STI = FINESS_STAGE_TI, STJ = FINESS_STAGE_TJ;
+// the tile of the FP32 stencil form (15 x 16: 2% less time per stage than
+// 7 x 32 for the Runge-Kutta form; the same for Lax-Wendroff)
+#ifndef FINESS_STAGE32_TI
+#define FINESS_STAGE32_TI 15
+#define FINESS_STAGE32_TJ 16
+#endif
+constexpr int STI32 = FINESS_STAGE32_TI, STJ32 = FINESS_STAGE32_TJ;
 
 template <class P, typename R, bool LXW>
 static int dispatch_weno(int wk, const StageArgs& a, hipStream_t st)
@@ -168,8 +177,8 @@ int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, h
         if (!fp32) return lxw ? dispatch_weno<Euler2D, double, true>(wk, a, st)
                               : dispatch_weno<Euler2D, double, false>(wk, a, st);
         if (wk != WENO_JS5_P2) return -1;
-        return lxw ? launch_stage<Euler2D, float, WENO_JS5_P2, true, 7, 32>(a, st)
-                   : launch_stage<Euler2D, float, WENO_JS5_P2, false, 7, 32>(a, st);
+        return lxw ? launch_stage<Euler2D, float, WENO_JS5_P2, true, STI32, STJ32>(a, st)
+                   : launch_stage<Euler2D, float, WENO_JS5_P2, false, STI32, STJ32>(a, st);
     }
     if (physics == PHYS_SHALLOW_WATER) {
         // the Lax-Wendroff form of the stage serves RK4-PIF's final stage
@@ -177,7 +186,7 @@ int launch_stage2d(int physics, int fp32, int wk, int lxw, const StageArgs& a, h
                               : dispatch_weno<ShallowWater2D, double, false>(wk, a, st);
         if (lxw) return -1;
         if (wk != WENO_JS5_P2) return -1;
-        return launch_stage<ShallowWater2D, float, WENO_JS5_P2, false, 7, 32>(a, st);
+        return launch_stage<ShallowWater2D, float, WENO_JS5_P2, false, STI32, STJ32>(a, st);
     }
     return -1;
 }

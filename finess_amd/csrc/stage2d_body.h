@@ -213,8 +213,10 @@ __host__ __device__ __forceinline__ void load_state(const StageArgs& a, int gi, 
     if (fj) q[P::NORMAL_Y] = -q[P::NORMAL_Y];
 }
 
+// Returns whether a cell this thread loaded has a nonzero third momentum
+// (rho w, 5-field systems; see face_part).
 template <class P, typename R, int TI, int TJ>
-__host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i0, int j0, const StageArgs& a,
+__host__ __device__ __forceinline__ bool phase_load(int tid, int nthreads, int i0, int j0, const StageArgs& a,
                                                     StageSmem<P, R, TI, TJ>& sm)
 {
     using S = StageSmem<P, R, TI, TJ>;
@@ -224,6 +226,7 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
     // two cells per thread and round: both cells' loads are issued before
     // either record is formed, so their memory latencies overlap
     constexpr int K = 2;
+    bool nz3 = false;
     for (int c0 = tid; c0 < S::NC; c0 += K * nthreads) {
         double q[K][P::MEQN];
         bool ok[K];
@@ -247,11 +250,13 @@ __host__ __device__ __forceinline__ void phase_load(int tid, int nthreads, int i
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (!ok[k]) continue;
+            if constexpr (P::MEQN == 5) nz3 = nz3 || q[k][3] != 0.0;
             double rec[P::NV];
             P::template cell_record<sizeof(R) == 4>(q[k], a.pp, rec);
             for (int v = 0; v < P::NV; v++) sm.cv[v][c0 + k * nthreads] = R(rec[v]);
         }
     }
+    return nz3;
 }
 
 // The old-state operands of the thread's cell for the stage update, read at
@@ -307,7 +312,11 @@ __host__ __device__ __forceinline__ R face_setup(const R (*cv)[NC], int base, in
 // the 1/2 is applied after the reconstruction: every WENO5 variant satisfies
 // W(2g; 4 eps) = 2 W(g; eps) exactly in binary arithmetic.  w[k] receives
 // twice the reconstructed characteristic flux of field k.
-template <class P, typename R, int WK, bool LXW, int NC>
+// SKIP3 (5-field systems; the caller sets it only when every stencil cell
+// has rho w = 0): field 3's flux, the mean state's w and so field 3's
+// characteristic values are all zero, its reconstruction is zero and is
+// not computed (2 of the 10 reconstructions of a face).
+template <class P, typename R, int WK, bool LXW, int NC, bool SKIP3 = false>
 __host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, int stride, int dir,
                                                    const R* lflux, long long lbase, long long lstride,
                                                    long long lplane, const typename P::template Eig<R>& e,
@@ -332,12 +341,15 @@ __host__ __device__ __forceinline__ void face_part(const R (*cv)[NC], int base, 
         P::left(e, vs, t);
         for (int k = 0; k < M; k++) g[k][n] = t[k];
     }
-    for (int k = 0; k < M; k++) w[k] = weno5<WK>(g[k], wp4);
+    for (int k = 0; k < M; k++) {
+        if (SKIP3 && M == 5 && k == 3) w[k] = R(0);
+        else w[k] = weno5<WK>(g[k], wp4);
+    }
 }
 
 // One face: characteristic WENO-LLF flux (lib/2d/ConstructL.cpp:63-208 for
 // x, :215-358 for y).  base/stride locate the 6 stencil records i-3..i+2.
-template <class P, typename R, int WK, bool LXW, int NC>
+template <class P, typename R, int WK, bool LXW, int NC, bool SKIP3 = false>
 __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int stride, int dir,
                                                 const R* lflux, long long lbase, long long lstride,
                                                 long long lplane, const PhysParams& pp,
@@ -355,7 +367,7 @@ __host__ __device__ __forceinline__ R face_flux(const R (*cv)[NC], int base, int
     R gh[M];
     auto part_flux = [&](int part) {
         R w[M];
-        face_part<P, R, WK, LXW, NC>(cv, base, stride, dir, lflux, lbase, lstride, lplane, e, la, wp4, part, w);
+        face_part<P, R, WK, LXW, NC, SKIP3>(cv, base, stride, dir, lflux, lbase, lstride, lplane, e, la, wp4, part, w);
         for (int k = 0; k < M; k++) gh[k] = part ? R(0.5) * (gh[k] + w[k]) : w[k];
     };
     // (5-field systems reading a tile: a rolled loop keeps the compiler from
@@ -412,7 +424,7 @@ __host__ __device__ __forceinline__ FaceGeom face_geom(int f, int dir, int i0, i
 
 template <class P, typename R, int WK, bool LXW, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, int j0, const StageArgs& a,
-                                                     StageSmem<P, R, TI, TJ>& sm, double& amax)
+                                                     StageSmem<P, R, TI, TJ>& sm, double& amax, bool skip3 = false)
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
@@ -425,9 +437,12 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
     const R amin = a.galpha ? R(a.galpha[dir]) : R(0);
-    const R alpha = face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir,
-                                                    static_cast<const R*>(dir ? a.lxw_G : a.lxw_F), lbase,
-                                                    lstride, lpl, a.pp, wp, R(a.alpha_scaling), fh, amin);
+    const R* lflux = static_cast<const R*>(dir ? a.lxw_G : a.lxw_F);
+    // (skip3: a tile-uniform choice between two forms of the face code)
+    const R alpha = skip3 ? face_flux<P, R, WK, LXW, S::NC, true>(sm.cv, base, stride, dir, lflux, lbase, lstride, lpl,
+                                                                  a.pp, wp, R(a.alpha_scaling), fh, amin)
+                          : face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, lflux, lbase, lstride, lpl,
+                                                            a.pp, wp, R(a.alpha_scaling), fh, amin);
     (void)sizeof(S);
     for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
     if constexpr (LXW) {

@@ -352,6 +352,7 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
     const unsigned id = (unsigned)a.route_list[slot];
     const int gi = (int)(id >> 16), gj = (int)(id & 0xffffu);
     R cv[NV][6];
+    bool nz3 = false;  // a stencil cell with rho w != 0 (face_part's skip3)
     const bool inner = !a.src_ghosts && (DIR == 0 ? (gi >= 3 && gi + 2 < a.mx) : (gj >= 3 && gj + 2 < a.my));
     if (inner) {
         const int stride = DIR == 0 ? a.my : 1;
@@ -359,6 +360,7 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
         for (int s = 0; s < 6; s++) {
             double q[M], rec[NV];
             for (int m = 0; m < M; m++) q[m] = (a.src + m * a.plane)[c0 + s * stride];
+            if constexpr (M == 5) nz3 = nz3 || q[3] != 0.0;
             P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
             for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
         }
@@ -367,13 +369,16 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
             double q[M], rec[NV];
             if (DIR == 0) load_state<P>(a, gi - 3 + s, gj, q);
             else load_state<P>(a, gi, gj - 3 + s, q);
+            if constexpr (M == 5) nz3 = nz3 || q[3] != 0.0;
             P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
             for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
         }
     }
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];
-    const R alpha = face_flux<P, R, WK, false, 6>(cv, 0, 1, DIR, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh);
+    const R alpha = nz3 ? face_flux<P, R, WK, false, 6>(cv, 0, 1, DIR, nullptr, 0, 0, 0, a.pp, wp, R(a.alpha_scaling), fh)
+                        : face_flux<P, R, WK, false, 6, true>(cv, 0, 1, DIR, nullptr, 0, 0, 0, a.pp, wp,
+                                                              R(a.alpha_scaling), fh);
     R* out = static_cast<R*>(a.route_flux) + 6 * slot;
     for (int m = 0; m < M; m++) out[m] = fh[m];
     out[M] = alpha;

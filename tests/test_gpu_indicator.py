@@ -127,12 +127,17 @@ def test_routed_exact_matches_oracle():
     assert np.all(err <= 1e-12)
 
 
-@pytest.mark.parametrize("per_step", [0, 1, 2])
-def test_routed_stepping_matches_oracle(per_step):
+@pytest.mark.parametrize("per_step,rho_w", [(0, False), (1, False), (2, False), (2, True)])
+def test_routed_stepping_matches_oracle(per_step, rho_w):
     """SSP-RK3 steps with exact-departure routing decided at every stage, once
     per step (recorded at the first stage, reused by the later ones), or once
-    every two steps."""
+    every two steps; rho_w: a nonzero third momentum in part of the grid, so
+    that the troubled-face pass runs every characteristic field there."""
     p, q, _ = problems.riemann4(mx=64, my=72, routing=ROUTE_ORACLE, route_tau=0.05, route_per_step=per_step)
+    if rho_w:
+        w = 0.3 * np.sin(np.linspace(0.0, 3.0, q.shape[1]))[None, :] * np.ones((q.shape[0] // 2, 1))
+        q[: q.shape[0] // 2, :, 3] = q[: q.shape[0] // 2, :, 0] * w
+        q[: q.shape[0] // 2, :, 4] += 0.5 * q[: q.shape[0] // 2, :, 0] * w * w
     dt = 0.3 / 72 / 2.5
     q_ref = q.copy()
     orc.fixed_steps_routed(p, q_ref, dt, 5, 0.05, per_step)

@@ -39,6 +39,28 @@ def test_construct_l_euler(shape):
 
 
 @pytest.mark.parametrize("weno", [WENO_Z, WENO_FD])
+def _with_rho_w(q, frac_i=0.5):
+    """rho w = 0.3 rho sin(...) in the first frac_i of the rows (i), zero in
+    the rest, with the energy raised by the added kinetic energy."""
+    q = q.copy()
+    n = int(q.shape[0] * frac_i)
+    w = 0.3 * np.sin(np.linspace(0.0, 3.0, q.shape[1]))[None, :] * np.ones((n, 1))
+    q[:n, :, 3] = q[:n, :, 0] * w
+    q[:n, :, 4] += 0.5 * q[:n, :, 0] * w * w
+    return q
+
+
+def test_construct_l_euler_nonzero_rho_w():
+    """A state whose rho w is nonzero in half of the grid: the stage kernel
+    reconstructs all five characteristic fields in the tiles that have it and
+    skips the identically zero one elsewhere; both against the oracle."""
+    p, q, _ = problems.riemann4(mx=70, my=66)
+    q = _with_rho_w(_perturb(q))
+    assert np.any(q[..., 3] != 0.0) and np.any(q[..., 3] == 0.0)
+    _construct_l_case(p, q)
+
+
+@pytest.mark.parametrize("weno", [WENO_Z, WENO_FD])
 def test_construct_l_euler_weno_variants(weno):
     p, q, _ = problems.riemann4(mx=48, my=40, weno_version=weno)
     _construct_l_case(p, _perturb(q))
@@ -165,6 +187,11 @@ def test_steps_riemann4_128():
     _fixed_steps_case(p, q, 1.5e-3, 20)
 
 
+def test_steps_euler_nonzero_rho_w():
+    p, q, _ = problems.riemann4(mx=96, my=64)
+    _fixed_steps_case(p, _with_rho_w(q, 0.4), 1.5e-3, 10)
+
+
 def test_steps_mixed_boundaries():
     p, q, _ = problems.riemann4(mx=64, my=96, bc=[BC_EXTRAP, BC_PERIODIC, BC_REFLECT, BC_REFLECT])
     _fixed_steps_case(p, q, 2e-3, 10)
@@ -187,7 +214,7 @@ def test_steps_lxw_kelvin_helmholtz():
 
 def test_steps_lxw_kelvin_helmholtz_fp32():
     """The FP32-stencil form of the Lax-Wendroff step (time-integrated fluxes
-    in FP64, their characteristic WENO reconstruction in FP32, FP64 stage
+    and their characteristic WENO reconstruction in FP32, FP64 stage
     accumulation) against the FP64 oracle after 10 steps: within 2e-5
     relative per field (FP32 rounding, 6e-8, amplified by the cancellation in
     the flux differences and accumulated over the steps)."""

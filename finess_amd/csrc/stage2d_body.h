@@ -259,6 +259,38 @@ __host__ __device__ __forceinline__ bool phase_load(int tid, int nthreads, int i
     return nz3;
 }
 
+// The time-integrated flux values of field 3 (rho w) that the tile's faces
+// read: F3 along the x-face stencils (extended rows i0 .. i0+TI+5), G3
+// along the y-face stencils (extended columns j0 .. j0+TJ+5), at most three
+// per thread of 256 (zero where there is none).  On the Lax-Wendroff path
+// face_part's SKIP3 needs them zero as well as the state's rho w; they are
+// fetched before the tile load and tested after it, so that the two loads'
+// latencies overlap.
+template <class P, typename R, int TI, int TJ>
+__device__ __forceinline__ void lxw_fetch3(int tid, int i0, int j0, const StageArgs& a, R v[3])
+{
+    constexpr int NX = (TI + 6) * TJ, NY = TI * (TJ + 6);
+    static_assert(NX + NY <= 3 * 256, "three values per thread of 256");
+    for (int k = 0; k < 3; k++) v[k] = R(0);
+    if constexpr (P::MEQN == 5) {
+        const long long lld = a.my + 6, lpl = (long long)(a.mx + 6) * lld;
+        const R* F = static_cast<const R*>(a.lxw_F) + 3 * lpl;
+        const R* G = static_cast<const R*>(a.lxw_G) + 3 * lpl;
+        for (int k = 0; k < 3; k++) {
+            const int c = tid + 256 * k;
+            if (c < NX) {
+                const int r = c / TJ, col = c - r * TJ;
+                const int er = i0 + r;
+                if (er <= a.mx + 5 && j0 + col < a.my) v[k] = F[er * lld + (j0 + col + 3)];
+            } else if (c < NX + NY) {
+                const int e = c - NX, r = e / (TJ + 6), col = e - r * (TJ + 6);
+                const int ec = j0 + col;
+                if (i0 + r < a.mx && ec <= a.my + 5) v[k] = G[(i0 + r + 3) * lld + ec];
+            }
+        }
+    }
+}
+
 // The old-state operands of the thread's cell for the stage update, read at
 // the start of the kernel so that their latency hides behind the tile load
 // (those the FP64 tile already holds exactly are taken from LDS instead).

@@ -212,6 +212,14 @@ def test_steps_lxw_kelvin_helmholtz():
     _fixed_steps_case(p, q, 1e-3, 10)
 
 
+def test_steps_lxw_nonzero_rho_w():
+    """Lax-Wendroff with rho w != 0 in part of the grid: the tiles whose
+    stencils see it (in the state or in the time-integrated fluxes)
+    reconstruct every field, the others skip the zero one."""
+    p, q, _ = problems.kelvin_helmholtz(mx=96, my=64)
+    _fixed_steps_case(p, _with_rho_w(q, 0.4), 1e-3, 10)
+
+
 def test_steps_lxw_kelvin_helmholtz_fp32():
     """The FP32-stencil form of the Lax-Wendroff step (time-integrated fluxes
     and their characteristic WENO reconstruction in FP32, FP64 stage

@@ -60,16 +60,15 @@ STAGES_PER_STEP = 3
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
 # counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 138.9e6   # mean over the 12 stages of a 4-step indicator interval
+TRAFFIC_ROUTED_1024 = 139.0e6   # mean over the 12 stages of a 4-step indicator interval
 TRAFFIC_PLAIN_1024 = 113.9e6
 # VALU instructions per launch at 1024x1024 (rocprofv3 SQ_INSTS_VALU and the
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters, same profile files): a wave64
 # VALU instruction holds a SIMD for 4 cycles, so at 100% issue a launch takes
 # N x 4 / (1024 SIMDs x 2.4 GHz).  The bench divides that by the measured
 # launch time.
-VALU_PLAIN_1024 = 43.79e6        # k_stage2d, FP64
-VALU_F64_PLAIN_1024 = 35.26e6    # of which FP64 arithmetic
-VALU_LOOKUP_1024 = 12.44e6       # k_stage2d_lookup, FP32 stencil
+VALU_PLAIN_1024 = 39.90e6        # k_stage2d, FP64
+VALU_F64_PLAIN_1024 = 31.07e6    # of which FP64 arithmetic
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 T_FINAL = 0.3
 T_TIMED = 0.15

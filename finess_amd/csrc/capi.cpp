@@ -1431,7 +1431,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
     // 1D grids of one workgroup without per-step records: the whole run in
     // launches of up to 65536 attempts each, the controller deciding on the
     // device and the kernel rotating the plane sets itself
-    if (run && !rc && p.ndims == 1 && !c->steplog && !c->timing && p.time_order <= 3 && run1d_fits(p.mx)) {
+    if (run && !rc && p.ndims == 1 && !c->steplog && !c->timing && p.time_order <= 3 && run1d_fits(weno_kind(p), p.mx)) {
         Stage1DRun r;
         memset(&r, 0, sizeof r);
         r.base = base_stage1d_args(c);

@@ -24,7 +24,7 @@ int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st);
 // launch (grids that fit it: run1d_fits).
 struct Stage1DRun;
 int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st);
-bool run1d_fits(int mx);
+bool run1d_fits(int wk, int mx);
 // 1D Lax-Wendroff time-integrated fluxes of the advection flux on [-3, mx+2].
 int launch_lxw_flux1d(const Stage1DArgs& a, double dt, int time_order, double* F, hipStream_t st);
 // Lax-Wendroff time-integrated fluxes on [-3, mx+2] x [-3, my+2]; fp32:

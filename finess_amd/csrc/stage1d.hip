@@ -4,10 +4,11 @@
 // UpdateSoln (lib/1d/UpdateSoln.cpp:17-54) and the periodic halo
 // (apps/1d/advection/smooth_example/SetBndValues.cpp:6-63).
 //
-// 1D grids are small (the advection case has 200 cells): up to SMALL1D cells
-// a stage is one workgroup (faces, barrier, cells), and a time step's stages
-// run in one launch of that workgroup; beyond it each stage is two plain
-// launches, one thread per face, then one thread per cell.
+// 1D grids are small (the advection case has 200 cells): up to
+// small1d_cells<WK>() cells a stage is one workgroup (faces, barrier, cells),
+// and a time step's stages run in one launch of that workgroup; beyond it
+// each stage is two plain launches, one thread per face, then one thread
+// per cell.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "kernels.h"
@@ -59,10 +60,26 @@ __global__ __launch_bounds__(256) void k_global_speed1d(Stage1DArgs a)
     }
 }
 
-// Grids up to SMALL1D cells run the whole stage in one workgroup: the faces,
-// a workgroup barrier (which orders the face fluxes written to global memory
-// before they are read), then the cells.  One launch instead of two.
-constexpr int SMALL1D = 8192;
+// Grids up to small1d_cells<WK>() cells run the whole stage in one
+// workgroup: the faces, a workgroup barrier (which orders the face fluxes
+// written to global memory before they are read), then the cells.  One
+// launch instead of two, but one CU: the limits are the measured crossovers
+// with the two-launch path, which takes 31-34 us per SSP-RK3 step (with the
+// controller) at any size up to 8192 cells.  WENO5: 21.5 against 30.6 us
+// at 1024 cells, 34.6 against 31.0 at 2048; WENO11: 31.6 against 33.2 at
+// 512, 44.2 against 33.6 at 1024.  Orders 7 and 9 lie between.
+template <int WK>
+constexpr int small1d_cells()
+{
+#ifdef FINESS_SMALL1D
+    return FINESS_SMALL1D;
+#else
+    return WK == WENO_JS11                                       ? 512
+           : (WK == WENO_JS9 || WK == WENO_Z9 || WK == WENO_FD9) ? 768
+           : (WK == WENO_JS7 || WK == WENO_Z7 || WK == WENO_FD7) ? 1024
+                                                                 : 1536;
+#endif
+}
 
 constexpr int SMALL1D_NT = 512;  // 2 waves per SIMD: the wide orders fit without spilling
 
@@ -163,7 +180,7 @@ template <int WK>
 static int launch_wk(const Stage1DArgs& a, hipStream_t st)
 {
     if (a.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, a);
-    if (a.mx <= SMALL1D) {
+    if (a.mx <= small1d_cells<WK>()) {
         hipLaunchKernelGGL(k_stage1d_small<WK>, dim3(1), dim3(SMALL1D_NT), 0, st, a);
         return hipGetLastError() == hipSuccess ? 0 : -4;
     }
@@ -177,7 +194,7 @@ static int launch_wk(const Stage1DArgs& a, hipStream_t st)
 template <int WK>
 static int launch_step_wk(const Stage1DStep& s, hipStream_t st)
 {
-    if (s.stage[0].mx > SMALL1D) {
+    if (s.stage[0].mx > small1d_cells<WK>()) {
         for (int k = 0; k < s.n; k++)
             if (const int rc = launch_wk<WK>(s.stage[k], st)) return rc;
         return s.ctl ? launch_step_control(s.ctl, s.stage[s.n - 1].smax_slot, s.cp, st) : 0;
@@ -227,7 +244,7 @@ int launch_stage1d(int wk, const Stage1DArgs& a, hipStream_t st)
 
 int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st)
 {
-    if (r.n < 1 || r.n > 3 || r.base.mx > SMALL1D) return -1;
+    if (r.n < 1 || r.n > 3 || !run1d_fits(wk, r.base.mx)) return -1;
     if (r.base.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, r.base);
     return with_kind(wk, [&](auto k) {
         hipLaunchKernelGGL(k_run1d_small<decltype(k)::value>, dim3(1), dim3(SMALL1D_NT), 0, st, r);
@@ -235,7 +252,10 @@ int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st)
     });
 }
 
-bool run1d_fits(int mx) { return mx <= SMALL1D; }
+bool run1d_fits(int wk, int mx)
+{
+    return with_kind(wk, [&](auto k) { return mx <= small1d_cells<decltype(k)::value>() ? 1 : 0; }) == 1;
+}
 
 int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st)
 {

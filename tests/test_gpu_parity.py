@@ -297,6 +297,29 @@ def test_fin_solve_rk_advection_full_run(time_order):
     assert l1 < {1: 5e-3, 2: 2e-4, 3: 2e-6}[time_order]
 
 
+@pytest.mark.parametrize("mx,order", [(1536, 5), (1537, 5), (2000, 5), (512, 11), (600, 11)])
+def test_fin_solve_rk_advection_launch_paths(mx, order):
+    """The 1D run on either side of the single-workgroup limit (1536 cells
+    for WENO5, 512 for WENO11): the whole run in one workgroup up to it, two
+    launches per stage and the controller kernel per step above it; both
+    against the oracle with the controller, to t = 0.05."""
+    r = (order + 1) // 2
+    p, q, aux = problems.advection_1d(mx=mx, space_order=order, mbc=max(3, r), initial_dt=0.4 / mx)
+    tend = 0.05
+    q0 = q.copy()
+    dtv0 = np.array([0.0, p.initial_dt, p.max_dt])
+    rc, t0, st0 = orc.fin_solve(p, q0, 0.0, tend, dtv0, aux=aux)
+    assert rc == 0
+    q1 = q.copy()
+    dtv1 = np.array([0.0, p.initial_dt, p.max_dt])
+    with Solver(p) as s:
+        t1 = s.fin_solve_rk(q1, 0.0, tend, dtv1, aux=aux)
+    assert abs(t1 - t0) <= 1e-14
+    err = orc.rel_linf(orc.interior(p, q1), orc.interior(p, q0))
+    print("mx", mx, "order", order, "steps", st0.steps_accepted, "rel err", err)
+    assert np.all(err <= 1e-10), err
+
+
 def test_nv_limit_raises():
     p, q, _ = problems.riemann4(mx=32, nv=2)
     dtv = np.array([0.0, p.initial_dt, p.max_dt])

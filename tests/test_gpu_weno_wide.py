@@ -203,8 +203,9 @@ def test_wide_orders_rejected_where_the_device_has_no_path():
 
 @pytest.mark.parametrize("version,order", [(WENO_JS, 5), (WENO_JS, 11), (WENO_Z, 9)])
 def test_construct_l_1d_large_grid(version, order):
-    """Above 8192 cells the 1D stage runs as two launches (one thread per face,
-    then one per cell) instead of one workgroup."""
+    """Above the single-workgroup limit (1536 cells for WENO5, 512 for
+    WENO11) the 1D stage runs as two launches (one thread per face, then one
+    per cell) instead of one workgroup."""
     r = (order + 1) // 2
     p, q, aux = problems.advection_1d(mx=9001, space_order=order, weno_version=version, mbc=max(3, r))
     q = q + 0.3 * RNG.random(q.shape)

@@ -126,9 +126,10 @@ def test_1d_nv_limit():
 
 @pytest.mark.parametrize("order", [5, 9])
 def test_1d_large_grid_two_launch_path(order):
-    """Above 8192 cells the 1D stage is a face launch and a cell launch, each
-    reading dt and the halt word from the control block: a rejected first
-    attempt (CFL 6), global_alpha, three chunks of steps."""
+    """Above the single-workgroup limit (1536 cells for WENO5, 768 for
+    WENO9) the 1D stage is a face launch and a cell launch, each reading dt
+    and the halt word from the control block: a rejected first attempt
+    (CFL 6), global_alpha, three chunks of steps."""
     r = (order + 1) // 2
     p, q, aux = _advection_varying_speed(10000, global_alpha=1, space_order=order, mbc=max(3, r), initial_dt=4e-4)
     st, _ = _same(p, q, lambda s, dtv: [s.advance(t, dtv) for t in (0.002, 0.005)], aux=aux)

@@ -213,22 +213,37 @@ def batched_shallow_water(n=512, steps=200):
     """512 independent periodic shallow-water problems (grids drawn from
     {64,...,384}^2), WENO5 SSP-RK3, FP32 stencil + FP64 state, per-problem
     CFL controller, one persistent launch per stage in a HIP graph, tile
-    shape from the autotuner."""
-    from finess_amd import PREC_FP32, Batch, problems
+    shape from the autotuner.  The same batch in plain FP64 beside it: its
+    throughput and the FP32 form's error against it (relative L1 of the
+    depth per member after the same accepted steps; the members' final
+    times differ by the controllers' dt rounding, also reported)."""
+    from finess_amd import PREC_FP32, PREC_FP64, Batch, problems
     draws = problems.sw_batch_members(n=n)
     sizes = [(d[0], d[1]) for d in draws]
-    mem = [problems.sw_bump(nx=nx, ny=ny, A=A, cx=cx, cy=cy, s=s, precision=PREC_FP32)[:2]
-           for nx, ny, A, cx, cy, s in draws]
-    with Batch(mem[0][0], sizes) as b:
-        for k, (p, q) in enumerate(mem):
-            b.set_state(k, q, 0.0, p.initial_dt)
-        tile = b.autotune(5)
-        sec = b.run(steps)
-        rej = sum(b.get_state(k)[4] for k in range(n))
     cells = sum(a * c for a, c in sizes)
-    return {"value": round(cells * steps / sec / 1e6, 1), "unit": "Mcell-updates/s", "problems": n,
-            "steps_per_problem": steps, "cells": cells, "seconds": round(sec, 4), "tile": list(tile),
-            "steps_rejected": int(rej)}
+    res = {}
+    for name, prec in (("fp32", PREC_FP32), ("fp64", PREC_FP64)):
+        mem = [problems.sw_bump(nx=nx, ny=ny, A=A, cx=cx, cy=cy, s=s, precision=prec)[:2]
+               for nx, ny, A, cx, cy, s in draws]
+        with Batch(mem[0][0], sizes) as b:
+            for k, (p, q) in enumerate(mem):
+                b.set_state(k, q, 0.0, p.initial_dt)
+            tile = b.autotune(5)
+            sec = b.run(steps)
+            fin = [b.get_state(k) for k in range(n)]
+        g = mem[0][0].mbc
+        res[name] = {"sec": sec, "tile": tile, "rej": sum(f[4] for f in fin),
+                     "h": [f[0][g:g + sz[0], g:g + sz[1], 0] for f, sz in zip(fin, sizes)],
+                     "t": np.array([f[1] for f in fin])}
+    rel = [float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))) for a, b in zip(res["fp32"]["h"], res["fp64"]["h"])]
+    r32 = res["fp32"]
+    return {"value": round(cells * steps / r32["sec"] / 1e6, 1), "unit": "Mcell-updates/s", "problems": n,
+            "steps_per_problem": steps, "cells": cells, "seconds": round(r32["sec"], 4), "tile": list(r32["tile"]),
+            "steps_rejected": int(r32["rej"]),
+            "fp64": {"value": round(cells * steps / res["fp64"]["sec"] / 1e6, 1), "tile": list(res["fp64"]["tile"]),
+                     "steps_rejected": int(res["fp64"]["rej"])},
+            "fp32_vs_fp64": {"rel_l1_depth_max": float(np.max(rel)), "rel_l1_depth_mean": float(np.mean(rel)),
+                             "max_final_time_difference": float(np.max(np.abs(r32["t"] - res["fp64"]["t"])))}}
 
 
 def _interior_density(p, q):

@@ -39,9 +39,20 @@ struct WideSmem {
 // keeps), each pass projecting the stencil with one row of the left
 // eigenvectors, so a thread holds the 2 (2R - 1) split values of one field
 // at a time.  R = 1 (first order): the split parts are the upwind values.
+// skip3 (5-field systems; set when every cell of the tile has rho w = 0):
+// field 3's flux, the mean state's w and so its characteristic values are
+// zero, and so is its reconstruction, which is not computed.  Only from
+// stencil radius SKIP3_MIN_R on: for the seventh order the branch in the
+// field loop made the face kernel slower (1,185 against 1,282
+// Mcell-updates/s); the ninth and eleventh orders gain 14% and 15% (1,004
+// against 881, 761 against 661, 1024^2 Euler).
+#ifndef FINESS_SKIP3_MIN_R
+#define FINESS_SKIP3_MIN_R 5
+#endif
+constexpr int SKIP3_MIN_R = FINESS_SKIP3_MIN_R;
 template <class P, int R, int VER, int DIR, class S>
 __device__ __forceinline__ double wide_face_flux(const S& sm, int base, int stride, const StageArgs& a,
-                                                 double fh[P::MEQN])
+                                                 double fh[P::MEQN], bool skip3)
 {
     constexpr int M = P::MEQN, NV = P::NV, WS = 2 * R - 1, NS = 2 * R;
     double cl[NV], cr[NV], ql[M], qr[M];
@@ -62,6 +73,10 @@ __device__ __forceinline__ double wide_face_flux(const S& sm, int base, int stri
     double ghat[M];
 #pragma unroll 1
     for (int m = 0; m < M; m++) {
+        if (M == 5 && R >= SKIP3_MIN_R && m == 3 && skip3) {
+            ghat[3] = 0.0;
+            continue;
+        }
         double gp[WS], gm[WS];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -117,6 +132,7 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
     // LDS cell (ci, cj) is cell (i0 + ci - R, j0 + cj) for x faces and
     // (i0 + ci, j0 + cj - R) for y faces; cells beyond the last face's
     // stencil (index n + R - 1 along the normal) are not loaded
+    bool nz3 = false;  // a loaded cell with rho w != 0
     for (int c = t; c < S::NC; c += 256) {
         const int ci = c / S::HJ, cj = c - ci * S::HJ;
         const int gi = i0 + ci - (DIR == 0 ? R : 0), gj = j0 + cj - (DIR == 1 ? R : 0);
@@ -124,13 +140,16 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
         if (!ok) continue;
         double q[M], rec[NV], qr[M], f[M];
         load_state<P>(a, gi, gj, q);
+        if constexpr (M == 5 && R >= SKIP3_MIN_R) nz3 = nz3 || q[3] != 0.0;
         P::cell_record(q, a.pp, rec);
         P::state(rec, DIR, qr);
         P::flux(qr, rec, f);
         for (int v = 0; v < NV; v++) sm.rec[v][c] = rec[v];
         for (int k = 0; k < M; k++) sm.f[k][c] = f[k];
     }
-    __syncthreads();
+    bool skip3 = false;
+    if constexpr (M == 5 && R >= SKIP3_MIN_R) skip3 = !__syncthreads_or(nz3);
+    else __syncthreads();
     const int fi = t / WT, fj = t - fi * WT;
     const int i = i0 + fi, j = j0 + fj;
     const bool valid = DIR == 0 ? (i <= a.mx && j < a.my) : (i < a.mx && j <= a.my);
@@ -140,7 +159,7 @@ __global__ __launch_bounds__(256) void k_wide_faces(StageArgs a)
         const long long nf = DIR == 0 ? (long long)(a.mx + 1) * a.my : (long long)a.mx * (a.my + 1);
         const long long f = (long long)i * nj + j;
         double fh[M];
-        const double alpha = wide_face_flux<P, R, VER, DIR>(sm, fi * S::HJ + fj, DIR == 0 ? S::HJ : 1, a, fh);
+        const double alpha = wide_face_flux<P, R, VER, DIR>(sm, fi * S::HJ + fj, DIR == 0 ? S::HJ : 1, a, fh, skip3);
         double* dst = DIR == 0 ? a.face_x : a.face_y;
         for (int m = 0; m < M; m++) dst[m * nf + f] = fh[m];
         double* sf = DIR == 0 ? a.smax_faces_x : a.smax_faces_y;

@@ -61,6 +61,26 @@ def test_construct_l_wide_euler(version, order):
     assert np.max(np.abs(s1 - s0)) <= 1e-13 * max(1.0, np.max(s0))
 
 
+@pytest.mark.parametrize("order", [7, 11])
+def test_construct_l_wide_euler_nonzero_rho_w(order):
+    """rho w != 0 in the first half of the rows (36 of 70): the face tiles
+    that see it reconstruct every characteristic field, the others skip the
+    zero one; both against the oracle."""
+    p, q = _perturbed_riemann(70, 48, WENO_JS, order)
+    n = q.shape[0] // 2
+    w = 0.3 * np.sin(np.linspace(0.0, 3.0, q.shape[1]))[None, :] * np.ones((n, 1))
+    q[:n, :, 3] = q[:n, :, 0] * w
+    q[:n, :, 4] += 0.5 * q[:n, :, 0] * w * w
+    orc.set_bnd(p, q)
+    L0, s0 = orc.construct_l(p, q)
+    with Solver(p) as s:
+        L1, s1 = s.construct_l(q)
+    err = orc.rel_linf(orc.interior(p, L1), orc.interior(p, L0))
+    tol, spread = _l_tol(p, q, L0)
+    print("order", order, "construct_l rel err", err, "oracle spread", spread)
+    assert np.all(err <= tol), err
+
+
 @pytest.mark.parametrize("bc", [BC_PERIODIC, BC_REFLECT, BC_EXTRAP])
 def test_construct_l_order11_boundaries(bc):
     """Eleventh order reaches six cells into the ghosts on every side."""

@@ -467,7 +467,10 @@ def main():
                     help="skip the Lax-Wendroff Kelvin-Helmholtz and the shock-bubble workloads")
     ap.add_argument("--route-interval", type=int, default=ROUTE_INTERVAL,
                     help="steps between indicator evaluations of the learned form")
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--train-seeds", type=int, default=3,
+                    help="networks trained from different initialisations; the one flagging the fewest faces of "
+                         "the held-out snapshot at the recall target is kept")
     ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
     ap.add_argument("--wide", type=int, default=0, metavar="ORDER",
                     help="only time the plain FP64 form with WENO order 1, 7, 9 or 11 (face-tile kernels)")
@@ -498,7 +501,8 @@ def main():
     ROUTE_INTERVAL = args.route_interval
     mx = args.mx
     cells = mx * mx
-    ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride)
+    ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride,
+                          seeds=tuple(range(3, 3 + args.train_seeds)))
 
     # throughput over K steps from the mid-run state
     el_p, rej_p, launch_p = timed_steps("plain", mx, args.steps, args.warmup)

@@ -8,8 +8,10 @@
 //   gW3 = H2^T dz,  dH2 = dz w3^T . [H2 > 0],  gW2 = H1^T dH2,
 //   dH1 = dH2 W2^T . [H1 > 0],  gW1 = X^T dH1,  gb = column sums,
 // then Adam on the FP32 master parameters.  Every GEMM runs on
-// v_mfma_f32_16x16x16_bf16 with both operands rounded to BF16 on load; the
-// weight-gradient GEMMs reduce over the batch with split-K and FP32 atomics.
+// v_mfma_f32_16x16x16_bf16 with both operands rounded to BF16 on load.  The
+// reductions over the batch (weight and bias gradients) run in a fixed
+// order, with no atomics, so a training run gives the same network every
+// time.
 #include <hip/hip_runtime.h>
 
 #include "indicator.h"
@@ -34,7 +36,7 @@ struct GemmArgs {
     const float* bias;     // length N or null
     const float* mask; int ldm;  // mask(m, n) = mask[m*ldm + n] or null
     int relu;
-    int atomic;            // accumulate into C with atomics (split-K)
+    int atomic;            // C += the product, K split over the 4 waves of a workgroup (k_gemm_bf16_splitk)
     int kchunk;            // K range per wave (multiple of 16)
     int tiles_m, tiles_n, chunks;
 };
@@ -73,8 +75,42 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g)
         if (g.bias) x += g.bias[n];
         if (g.relu) x = x > 0.f ? x : 0.f;
         if (g.mask && !(g.mask[(long long)m * g.ldm + n] > 0.f)) x = 0.f;
-        if (g.atomic) atomicAdd(&g.C[(long long)m * g.ldc + n], x);
-        else g.C[(long long)m * g.ldc + n] = x;
+        g.C[(long long)m * g.ldc + n] = x;
+    }
+}
+
+// C (M x N) += op(A) op(B) for the weight gradients, whose K is the batch:
+// one workgroup per 16x16 output tile, its 4 waves taking interleaved
+// 16-wide slices of K, the four partial tiles summed in a fixed order by
+// wave 0 (no bias, ReLU or mask).
+__global__ __launch_bounds__(256) void k_gemm_bf16_splitk(GemmArgs g)
+{
+    __shared__ float part[4][64][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tm = blockIdx.x / g.tiles_n, tn = blockIdx.x - tm * g.tiles_n;
+    const int m0 = tm * 16, n0 = tn * 16;
+    const int r = lane & 15, q = lane >> 4;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 16 * w; k0 < g.K; k0 += 64) {
+        float av[4], bv[4];
+        for (int v = 0; v < 4; v++) {
+            const int k = k0 + 4 * q + v;
+            const int m = m0 + r, n = n0 + r;
+            const bool kin = k < g.K;
+            av[v] = (kin && m < g.M) ? (g.ta ? g.A[(long long)k * g.lda + m] : g.A[(long long)m * g.lda + k]) : 0.f;
+            bv[v] = (kin && n < g.N) ? (g.tb ? g.B[(long long)n * g.ldb + k] : g.B[(long long)k * g.ldb + n]) : 0.f;
+        }
+        acc = mfma16(pack4(av[0], av[1], av[2], av[3]), pack4(bv[0], bv[1], bv[2], bv[3]), acc);
+    }
+    for (int v = 0; v < 4; v++) part[w][lane][v] = acc[v];
+    __syncthreads();
+    if (w != 0) return;
+    const int n = n0 + r;
+    if (n >= g.N) return;
+    for (int v = 0; v < 4; v++) {
+        const int m = m0 + 4 * q + v;
+        if (m >= g.M) continue;
+        g.C[(long long)m * g.ldc + n] += ((part[0][lane][v] + part[1][lane][v]) + part[2][lane][v]) + part[3][lane][v];
     }
 }
 
@@ -83,13 +119,12 @@ static int gemm(GemmArgs g, hipStream_t st)
     g.tiles_m = (g.M + 15) / 16;
     g.tiles_n = (g.N + 15) / 16;
     if (g.atomic) {
-        // split the batch reduction over enough waves to fill the chip
-        g.kchunk = 1024;
-        g.chunks = (g.K + g.kchunk - 1) / g.kchunk;
-    } else {
-        g.kchunk = ((g.K + 15) / 16) * 16;
-        g.chunks = 1;
+        if (g.bias || g.relu || g.mask) return -1;
+        hipLaunchKernelGGL(k_gemm_bf16_splitk, dim3(g.tiles_m * g.tiles_n), dim3(256), 0, st, g);
+        return hipGetLastError() == hipSuccess ? 0 : -4;
     }
+    g.kchunk = ((g.K + 15) / 16) * 16;
+    g.chunks = 1;
     const int waves = g.tiles_m * g.tiles_n * g.chunks;
     hipLaunchKernelGGL(k_gemm_bf16, dim3((waves + 3) / 4), dim3(256), 0, st, g);
     return hipGetLastError() == hipSuccess ? 0 : -4;
@@ -110,14 +145,18 @@ __global__ void k_bce(const float* z, const float* y, float* dz, int n, float in
     if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, l);
 }
 
-// column sums of a (n x c) matrix into out[c] (atomic)
-__global__ void k_colsum(const float* a, int n, int c, float* out)
+// out[col] += the column sum of a (n x c) matrix, one workgroup of 256
+// threads per column, summed in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum(const float* a, int n, int c, float* out)
 {
+    __shared__ float red[4];
     const int col = blockIdx.y;
     float s = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[(long long)i * c + col];
+    for (int i = threadIdx.x; i < n; i += 256) s += a[(long long)i * c + col];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&out[col], s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[col] += ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 __global__ void k_adam(float* p, float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
@@ -196,10 +235,9 @@ int mlp_backward(const float* params, const float* X, const float* y, int B, flo
     g.A = X; g.lda = MLP_IN; g.ta = 1; g.B = dH1; g.ldb = MLP_H; g.C = grad + MLP_OFF_W1; g.ldc = MLP_H; g.atomic = 1;
     if ((rc = gemm(g, st))) return rc;
     // bias gradients
-    const int gx = (B + 255) / 256 < 256 ? (B + 255) / 256 : 256;
-    hipLaunchKernelGGL(k_colsum, dim3(gx, 1), dim3(256), 0, st, dz, B, 1, grad + MLP_OFF_B3);
-    hipLaunchKernelGGL(k_colsum, dim3(gx, MLP_H), dim3(256), 0, st, dH2, B, MLP_H, grad + MLP_OFF_B2);
-    hipLaunchKernelGGL(k_colsum, dim3(gx, MLP_H), dim3(256), 0, st, dH1, B, MLP_H, grad + MLP_OFF_B1);
+    hipLaunchKernelGGL(k_colsum, dim3(1, 1), dim3(256), 0, st, dz, B, 1, grad + MLP_OFF_B3);
+    hipLaunchKernelGGL(k_colsum, dim3(1, MLP_H), dim3(256), 0, st, dH2, B, MLP_H, grad + MLP_OFF_B2);
+    hipLaunchKernelGGL(k_colsum, dim3(1, MLP_H), dim3(256), 0, st, dH1, B, MLP_H, grad + MLP_OFF_B1);
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 

@@ -60,6 +60,22 @@ def test_training_reduces_loss():
     assert loss[-1] < 0.5 * loss[0] and acc > 0.95
 
 
+def test_training_is_reproducible():
+    """The batch reductions of the gradients run in a fixed order: two
+    trainings from the same seed and data give the same parameters, bit for
+    bit (the minibatch order comes from a seeded host shuffle)."""
+    X = RNG.normal(size=(1 << 15, 24)).astype(np.float32)
+    y = ((X[:, 1] + 0.7 * X[:, 4] - X[:, 11]) > 0.2).astype(np.float32)
+    params = []
+    for _ in range(2):
+        net = Mlp(seed=11, capacity=1 << 15)
+        net.set_data(X, y)
+        net.train(epochs=2, batch=4096, lr=3e-3)
+        params.append(net.params.copy())
+    print("max |diff|", np.max(np.abs(params[0] - params[1])))
+    assert np.array_equal(params[0], params[1])
+
+
 def test_harvest_matches_oracle():
     p, q, _ = problems.riemann4(mx=48, my=40)
     with Solver(p) as s:

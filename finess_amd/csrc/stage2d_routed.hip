@@ -518,6 +518,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     }
 }
 
+// workgroups of the troubled-face pass (a grid-stride loop over the list)
+#ifndef FINESS_ROUTE_FULL_BLOCKS
+#define FINESS_ROUTE_FULL_BLOCKS 1024
+#endif
+constexpr int ROUTE_FULL_BLOCKS = FINESS_ROUTE_FULL_BLOCKS;
+
 // the tile of the routed kernels (at most 256 faces of each direction)
 #ifndef FINESS_ROUTE_TI
 #define FINESS_ROUTE_TI 7
@@ -541,7 +547,7 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
             else
                 hipLaunchKernelGGL((k_classify<Euler2D, R, TI, TJ, false>), grid, block, 0, st, a);
         }
-        hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(1024), block, 0, st, a);
+        hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(ROUTE_FULL_BLOCKS), block, 0, st, a);
         hipLaunchKernelGGL((k_stage2d_lookup<Euler2D, R, TI, TJ>), grid, block, 0, st, a);
     } else if (a.route_exact)
         hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, true>), grid, block, 0, st, a);

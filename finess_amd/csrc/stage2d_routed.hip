@@ -339,17 +339,14 @@ __global__ __launch_bounds__(256) void k_classify(StageArgs a)
     }
 }
 
-// Characteristic WENO5 flux of troubled face k of direction DIR (list
-// slots from slot0; lanes with k >= n idle).  The stencil of a face whose
-// six cells all lie inside the grid is read directly, others through the
+// Characteristic WENO5 flux of the troubled face of direction DIR in list
+// slot `slot`, whose list entry is `id`.  The stencil of a face whose six
+// cells all lie inside the grid is read directly, others through the
 // boundary map; the records are formed as in the tile load.
 template <class P, typename R, int WK, int DIR>
-__device__ __forceinline__ void route_face(const StageArgs& a, long long k, long long n, long long slot0)
+__device__ __forceinline__ void route_face(const StageArgs& a, unsigned id, long long slot)
 {
     constexpr int M = P::MEQN, NV = P::NV;
-    if (k >= n) return;
-    const long long slot = slot0 + k;
-    const unsigned id = (unsigned)a.route_list[slot];
     const int gi = (int)(id >> 16), gj = (int)(id & 0xffffu);
     R cv[NV][6];
     bool nz3 = false;  // a stencil cell with rho w != 0 (face_part's skip3)
@@ -384,22 +381,48 @@ __device__ __forceinline__ void route_face(const StageArgs& a, long long k, long
     out[M] = alpha;
 }
 
+// workgroups of the troubled-face pass (a grid-stride loop over the list)
+#ifndef FINESS_ROUTE_FULL_BLOCKS
+#define FINESS_ROUTE_FULL_BLOCKS 1024
+#endif
+constexpr int ROUTE_FULL_BLOCKS = FINESS_ROUTE_FULL_BLOCKS;
+static_assert(ROUTE_FULL_BLOCKS % 2 == 0, "half the workgroups per direction");
+
+// The troubled faces of one direction: lane g0 of `step` lanes takes list
+// entries g0, g0 + step, ... below n of the direction's list (slots from
+// off); id0 is the lane's first entry, read ahead by the caller.  The loop
+// runs whole waves (lanes past the end idle).
+template <class P, typename R, int WK, int DIR>
+__device__ __forceinline__ void route_dir(const StageArgs& a, long long g0, long long step, long long n, long long off,
+                                          unsigned id0)
+{
+    const long long lane = g0 & 63;
+    for (long long g = g0; g - lane < n; g += step)
+        if (g < n) route_face<P, R, WK, DIR>(a, g == g0 ? id0 : (unsigned)a.route_list[off + g], off + g);
+}
+
 // Characteristic WENO5 fluxes of the listed troubled faces of this stage's
-// source state, one face per lane.  The x faces are laid out over whole
-// waves first, then the y faces, so that every wave runs one direction
-// (a compile-time constant in route_face) without divergence.
+// source state, one face per lane: the x faces on the first half of the
+// workgroups, the y faces on the second, so that every wave runs one
+// direction (a compile-time constant in route_face) and every lane knows its
+// first list slot before the list lengths arrive: the halt word, the lengths
+// and that slot's entry are read together instead of one after another.
 template <class P, typename R, int WK>
 __global__ __launch_bounds__(256) void k_route_full(StageArgs a)
 {
-    if (stage_halted(a)) return;
-    const long long nx = (long long)a.route_n[0], ny = (long long)a.route_n[1];
-    const long long px = (nx + 63) & ~63ll;
-    const long long nfx = (long long)(a.mx + 1) * a.my;
-    const long long step = (long long)gridDim.x * blockDim.x;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; (g & ~63ll) < px + ny; g += step) {
-        if (g < px) route_face<P, R, WK, 0>(a, g, nx, 0);
-        else route_face<P, R, WK, 1>(a, g - px, ny, nfx);
-    }
+    const int half = (int)gridDim.x / 2;
+    const int dir = (int)blockIdx.x >= half;
+    const long long g0 = (long long)((int)blockIdx.x - dir * half) * blockDim.x + threadIdx.x;
+    const long long step = (long long)half * blockDim.x;
+    const long long nfx = (long long)(a.mx + 1) * a.my, nfy = (long long)a.mx * (a.my + 1);
+    const long long off = dir ? nfx : 0;
+    // the entry is read whether or not the slot is in use (a stale entry of
+    // an unused slot is never used; the list holds nfx + nfy slots)
+    const unsigned id0 = g0 < (dir ? nfy : nfx) ? (unsigned)a.route_list[off + g0] : 0u;
+    const long long n = (long long)a.route_n[dir];
+    if (stage_halted(a) | (n <= 0)) return;
+    if (dir) route_dir<P, R, WK, 1>(a, g0, step, n, off, id0);
+    else route_dir<P, R, WK, 0>(a, g0, step, n, off, id0);
 }
 
 // One stage with the step's recorded routing: the linear flux at smooth
@@ -517,12 +540,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
         }
     }
 }
-
-// workgroups of the troubled-face pass (a grid-stride loop over the list)
-#ifndef FINESS_ROUTE_FULL_BLOCKS
-#define FINESS_ROUTE_FULL_BLOCKS 1024
-#endif
-constexpr int ROUTE_FULL_BLOCKS = FINESS_ROUTE_FULL_BLOCKS;
 
 // the tile of the routed kernels (at most 256 faces of each direction)
 #ifndef FINESS_ROUTE_TI

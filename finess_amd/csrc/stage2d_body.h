@@ -67,6 +67,15 @@ __device__ __forceinline__ double wave_max64(double v)
     }
     return v;
 }
+// the same for FP32 values (half the shuffles and compares)
+__device__ __forceinline__ float wave_max64(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Remapping the
 // launch index gives each XCD a contiguous band of tiles, so the halo rows

@@ -467,7 +467,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
                                                    : record_slot(rt[2].mask[1], rt[2].base[1], gy.fi * (TJ + 1));
         if (sy >= 0) sy += (a.mx + 1) * a.my;
     }
-    double ax = 0.0, ay = 0.0;
+    // wave-speed maxima in the stencil precision (the maximum of values
+    // that are all R is exact in R)
+    R ax = R(0), ay = R(0);
     for (int dir = 0; dir < 2; dir++) {
         const FaceGeom& h = dir ? gy : gx;
         const int slot = dir ? sy : sx;
@@ -483,10 +485,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
         }
         if (dir) {
             for (int m = 0; m < M; m++) fy[m][t] = fh[m];
-            if (h.count) ay = rmax(ay, double(alpha));
+            if (h.count) ay = rmax(ay, alpha);
         } else {
             for (int m = 0; m < M; m++) sm.face[m][t] = fh[m];
-            if (h.count) ax = rmax(ax, double(alpha));
+            if (h.count) ax = rmax(ax, alpha);
         }
     }
     __syncthreads();
@@ -522,11 +524,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
         }
     }
     if (a.smax) {
-        if (ax != ax) ax = __builtin_inf();
-        if (ay != ay) ay = __builtin_inf();
+        if (ax != ax) ax = R(__builtin_inf());
+        if (ay != ay) ay = R(__builtin_inf());
         ax = wave_max64(ax);
         ay = wave_max64(ay);
-        if (lane == 0) { red[0][w] = ax; red[1][w] = ay; }
+        if (lane == 0) { red[0][w] = double(ax); red[1][w] = double(ay); }
         __syncthreads();
         if (t == 0) {
             double mx_ = red[0][0], my_ = red[1][0];

@@ -60,7 +60,7 @@ STAGES_PER_STEP = 3
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
 # counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 139.0e6   # mean over the 12 stages of a 4-step indicator interval
+TRAFFIC_ROUTED_1024 = 139.6e6   # mean over the 12 stages of a 4-step indicator interval
 TRAFFIC_PLAIN_1024 = 113.9e6
 # VALU instructions per launch at 1024x1024 (rocprofv3 SQ_INSTS_VALU and the
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters, same profile files): a wave64

@@ -463,9 +463,12 @@ __host__ __device__ __forceinline__ FaceGeom face_geom(int f, int dir, int i0, i
     return g;
 }
 
+// The face fluxes go to sm.face, or to fout (field m at fout[m * fld + tid])
+// when it is given.
 template <class P, typename R, int WK, bool LXW, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, int j0, const StageArgs& a,
-                                                     StageSmem<P, R, TI, TJ>& sm, double& amax, bool skip3 = false)
+                                                     StageSmem<P, R, TI, TJ>& sm, double& amax, bool skip3 = false,
+                                                     R* fout = nullptr, int fld = 0)
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
@@ -485,7 +488,11 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
                           : face_flux<P, R, WK, LXW, S::NC>(sm.cv, base, stride, dir, lflux, lbase, lstride, lpl,
                                                             a.pp, wp, R(a.alpha_scaling), fh, amin);
     (void)sizeof(S);
-    for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
+    if (fout) {
+        for (int m = 0; m < M; m++) fout[m * fld + tid] = fh[m];
+    } else {
+        for (int m = 0; m < M; m++) sm.face[m][tid] = fh[m];
+    }
     if constexpr (LXW) {
         if (a.face_x) {
             const int gi = i0 + fi, gj = j0 + fj;
@@ -518,11 +525,13 @@ __host__ __device__ __forceinline__ void phase_lx(int tid, const StageArgs& a, c
         Lx[m] = -(double(sm.face[m][(ci + 1) * TJ + cj]) - double(sm.face[m][ci * TJ + cj])) * idx_;
 }
 
-// L = Lx - (Ghat(j+1) - Ghat(j))/dy and the stage update in FP64.
+// L = Lx - (Ghat(j+1) - Ghat(j))/dy and the stage update in FP64 (the y
+// fluxes from sm.face, or from fy with leading dimension fyld).
 template <class P, typename R, int TI, int TJ>
 __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, const StageArgs& a,
                                                       const StageSmem<P, R, TI, TJ>& sm, const double Lx[P::MEQN],
-                                                      const double pa[P::MEQN], const double pb[P::MEQN])
+                                                      const double pa[P::MEQN], const double pb[P::MEQN],
+                                                      const R* fy = nullptr, int fyld = 0)
 {
     using S = StageSmem<P, R, TI, TJ>;
     constexpr int M = P::MEQN;
@@ -547,7 +556,8 @@ __host__ __device__ __forceinline__ void phase_update(int tid, int i0, int j0, c
         source_psi<M>(a.src_kind, a.src_coef, qs, a.aux ? a.aux + idx : nullptr, a.plane, psi);
     }
     for (int m = 0; m < M; m++) {
-        double L = Lx[m] - (double(sm.face[m][ci * (TJ + 1) + cj + 1]) - double(sm.face[m][ci * (TJ + 1) + cj])) * idy_;
+        const R* gm = fy ? fy + m * fyld : sm.face[m];
+        double L = Lx[m] - (double(gm[ci * (TJ + 1) + cj + 1]) - double(gm[ci * (TJ + 1) + cj])) * idy_;
         if (a.src_kind != SRC_OFF) L = psi[m] + L;  // Lstar = psi + tmp (lib/2d/ConstructL.cpp:378-386)
         if (a.Lout) a.Lout[m * a.plane + idx] = L;
         if (!a.out) continue;

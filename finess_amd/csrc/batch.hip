@@ -16,6 +16,10 @@
 
 namespace finess {
 
+#ifndef FINESS_BATCH_FY
+#define FINESS_BATCH_FY 1
+#endif
+
 template <class P, typename R, int TI, int TJ, int STAGE>
 __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
 {
@@ -23,6 +27,10 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
     static_assert((TI + 1) * TJ <= NT && TI * (TJ + 1) <= NT, "one face per thread and direction");
     __shared__ StageSmem<P, R, TI, TJ> sm;
     __shared__ double red[2][NT / 64];
+#if FINESS_BATCH_FY
+    constexpr int NFY = TI * (TJ + 1);
+    __shared__ R fy[M][NFY];
+#endif
     const int t = threadIdx.x;
     for (int tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
         const BatchProb* pr = b.probs + b.tile_prob[tile];
@@ -62,6 +70,14 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
         __syncthreads();
         double ax = 0.0, ay = 0.0;
+#if FINESS_BATCH_FY
+        // the y fluxes in their own array: no barrier between the face passes
+        phase_faces<P, R, WENO_JS5_P2, false, TI, TJ>(t, 0, i0, j0, a, sm, ax);
+        phase_faces<P, R, WENO_JS5_P2, false, TI, TJ>(t, 1, i0, j0, a, sm, ay, false, &fy[0][0], NFY);
+        __syncthreads();
+        phase_lx<P, R, TI, TJ>(t, a, sm, Lx);
+        phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb, &fy[0][0], NFY);
+#else
         phase_faces<P, R, WENO_JS5_P2, false, TI, TJ>(t, 0, i0, j0, a, sm, ax);
         __syncthreads();
         phase_lx<P, R, TI, TJ>(t, a, sm, Lx);
@@ -69,6 +85,7 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         phase_faces<P, R, WENO_JS5_P2, false, TI, TJ>(t, 1, i0, j0, a, sm, ay);
         __syncthreads();
         phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
+#endif
         if (ax != ax) ax = __builtin_inf();
         if (ay != ay) ay = __builtin_inf();
         ax = wave_max64(ax);

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         phase_prefetch<P, R, TI, TJ>(t, i0, j0, a, pa, pb);
         phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
         __syncthreads();
-        double ax = 0.0, ay = 0.0;
+        R ax = R(0), ay = R(0);  // wave-speed maxima in the stencil precision
 #if FINESS_BATCH_FY
         // the y fluxes in their own array: no barrier between the face passes
         phase_faces<P, R, WENO_JS5_P2, false, TI, TJ>(t, 0, i0, j0, a, sm, ax);
@@ -86,11 +86,11 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
         __syncthreads();
         phase_update<P, R, TI, TJ>(t, i0, j0, a, sm, Lx, pa, pb);
 #endif
-        if (ax != ax) ax = __builtin_inf();
-        if (ay != ay) ay = __builtin_inf();
+        if (ax != ax) ax = R(__builtin_inf());
+        if (ay != ay) ay = R(__builtin_inf());
         ax = wave_max64(ax);
         ay = wave_max64(ay);
-        if ((t & 63) == 0) { red[0][t >> 6] = ax; red[1][t >> 6] = ay; }
+        if ((t & 63) == 0) { red[0][t >> 6] = double(ax); red[1][t >> 6] = double(ay); }
         __syncthreads();
         if (t == 0) {
             double mx_ = red[0][0], my_ = red[1][0];

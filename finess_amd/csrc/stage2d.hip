@@ -11,15 +11,16 @@ namespace finess {
 
 // Reduces two per-thread maxima over the workgroup and folds them into
 // slot[b] and slot[CFL_SPREAD + b], b = block index mod CFL_SPREAD
-// (non-negative doubles compare like their bit patterns).
-template <int NT>
-__device__ __forceinline__ void block_max2(double ax, double ay, unsigned long long* slot)
+// (non-negative doubles compare like their bit patterns).  The per-thread
+// maxima are FP64 or FP32 values; the wave reduction runs in their type.
+template <int NT, typename A>
+__device__ __forceinline__ void block_max2(A ax, A ay, unsigned long long* slot)
 {
     __shared__ double red[2][NT / 64];
     ax = wave_max64(ax);
     ay = wave_max64(ay);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) { red[0][wid] = ax; red[1][wid] = ay; }
+    if (lane == 0) { red[0][wid] = double(ax); red[1][wid] = double(ay); }
     __syncthreads();
     if (threadIdx.x == 0) {
         double mx_ = red[0][0], my_ = red[1][0];
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     bool nz = phase_load<P, R, TI, TJ>(t, NT, i0, j0, a, sm);
     if constexpr (LXW) nz = nz || f3[0] != R(0) || f3[1] != R(0) || f3[2] != R(0);
     const bool skip3 = !__syncthreads_or(nz) && SKIPPABLE;
-    double ax = 0.0, ay = 0.0;
+    R ax = R(0), ay = R(0);  // wave-speed maxima in the stencil precision
     double Lx[P::MEQN];
     if constexpr (SEPARATE_FY<R, LXW>) {
         // the y fluxes in an array of their own: both face passes run
@@ -97,8 +98,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     }
     if (a.smax) {
         // a NaN speed must not vanish in the max: report it as +inf
-        if (ax != ax) ax = __builtin_inf();
-        if (ay != ay) ay = __builtin_inf();
+        if (ax != ax) ax = R(__builtin_inf());
+        if (ay != ay) ay = R(__builtin_inf());
         block_max2<NT>(ax, ay, a.smax);
     }
 }

@@ -464,10 +464,11 @@ __host__ __device__ __forceinline__ FaceGeom face_geom(int f, int dir, int i0, i
 }
 
 // The face fluxes go to sm.face, or to fout (field m at fout[m * fld + tid])
-// when it is given.
-template <class P, typename R, int WK, bool LXW, int TI, int TJ>
+// when it is given.  amax: the running maximum of the face wave speeds, in
+// FP64 or in the stencil precision R (a maximum of R values is exact in R).
+template <class P, typename R, int WK, bool LXW, int TI, int TJ, typename A>
 __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, int j0, const StageArgs& a,
-                                                     StageSmem<P, R, TI, TJ>& sm, double& amax, bool skip3 = false,
+                                                     StageSmem<P, R, TI, TJ>& sm, A& amax, bool skip3 = false,
                                                      R* fout = nullptr, int fld = 0)
 {
     using S = StageSmem<P, R, TI, TJ>;
@@ -507,7 +508,7 @@ __host__ __device__ __forceinline__ void phase_faces(int tid, int dir, int i0, i
             }
         }
     }
-    if (count) amax = rmax(amax, double(alpha));
+    if (count) amax = rmax(amax, A(alpha));
     if (dir == 0 && a.smax_faces_x) a.smax_faces_x[(long long)(i0 + fi) * a.my + (j0 + fj)] = double(alpha);
     if (dir == 1 && a.smax_faces_y) a.smax_faces_y[(long long)(i0 + fi) * (a.my + 1) + (j0 + fj)] = double(alpha);
 }

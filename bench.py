@@ -67,7 +67,7 @@ TRAFFIC_PLAIN_1024 = 113.9e6
 # VALU instruction holds a SIMD for 4 cycles, so at 100% issue a launch takes
 # N x 4 / (1024 SIMDs x 2.4 GHz).  The bench divides that by the measured
 # launch time.
-VALU_PLAIN_1024 = 39.90e6        # k_stage2d, FP64
+VALU_PLAIN_1024 = 40.00e6        # k_stage2d, FP64
 VALU_F64_PLAIN_1024 = 31.07e6    # of which FP64 arithmetic
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 T_FINAL = 0.3

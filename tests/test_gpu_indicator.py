@@ -166,3 +166,25 @@ def test_routed_stepping_matches_oracle(per_step, rho_w):
     print("per_step", per_step, "troubled", nt, "of", nf, "err", err)
     assert 0 < nt < nf
     assert np.all(err <= 1e-10), err
+
+
+def test_routed_stepping_all_troubled_large():
+    """Every face troubled (tau < 0) on a 384 x 384 grid, the record made once
+    every two steps and reused by the stages after it: each direction has more
+    troubled faces (147,840 x faces) than the troubled-face pass has lanes for
+    one direction (131,072), so its grid-stride loop takes a second face on
+    part of the lanes.  Against the routed oracle, which with every face
+    troubled is the characteristic WENO5 operator everywhere."""
+    p, q, _ = problems.riemann4(mx=384, my=384, routing=ROUTE_ORACLE, route_tau=-1.0, route_per_step=2)
+    dt = 0.3 / 384 / 2.5
+    q_ref = q.copy()
+    orc.fixed_steps_routed(p, q_ref, dt, 3, -1.0, 2)
+    with Solver(p) as s:
+        s.set_state(q)
+        s.run_fixed_steps(dt, 3)
+        q_gpu, _ = s.get_state()
+        nt, nf = s.route_stats()
+    err = orc.rel_linf(orc.interior(p, q_gpu), orc.interior(p, q_ref))
+    print("troubled", nt, "of", nf, "err", err)
+    assert nt == nf and nf > 2 * 131072
+    assert np.all(err <= 1e-10), err

@@ -60,7 +60,7 @@ STAGES_PER_STEP = 3
 # WRITE_SIZE runs, calibrated against a 256 MiB device copy (commands and raw
 # counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
-TRAFFIC_ROUTED_1024 = 139.6e6   # mean over the 12 stages of a 4-step indicator interval
+TRAFFIC_ROUTED_1024 = 134.0e6   # mean over the 12 stages of a 4-step indicator interval
 TRAFFIC_PLAIN_1024 = 113.9e6
 # VALU instructions per launch at 1024x1024 (rocprofv3 SQ_INSTS_VALU and the
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters, same profile files): a wave64
@@ -472,6 +472,11 @@ def main():
                     help="networks trained from different initialisations; the one flagging the fewest faces of "
                          "the held-out snapshot at the recall target is kept")
     ap.add_argument("--train-stride", type=int, default=2, help="harvest every n-th interface of each snapshot")
+    ap.add_argument("--save-indicator", metavar="PATH",
+                    help="write the trained network (the drop-in layer's indicator file) after training")
+    ap.add_argument("--indicator", metavar="PATH",
+                    help="use this network instead of training one (a file of --save-indicator), so that a "
+                         "profiled run times the default run's network without profiling its training")
     ap.add_argument("--wide", type=int, default=0, metavar="ORDER",
                     help="only time the plain FP64 form with WENO order 1, 7, 9 or 11 (face-tile kernels)")
     ap.add_argument("--wide-version", default="js", choices=["js", "z", "fd"])
@@ -496,13 +501,18 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU slab decomposition is a design, not code)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    from finess_amd.indicator import train_indicator
+    from finess_amd.indicator import Indicator, train_indicator
 
     ROUTE_INTERVAL = args.route_interval
     mx = args.mx
     cells = mx * mx
-    ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride,
-                          seeds=tuple(range(3, 3 + args.train_seeds)))
+    if args.indicator:
+        ind = Indicator.load(args.indicator)
+    else:
+        ind = train_indicator(mx=mx, epochs=args.train_epochs, stride=args.train_stride,
+                              seeds=tuple(range(3, 3 + args.train_seeds)))
+    if args.save_indicator:
+        ind.save(args.save_indicator)
 
     # throughput over K steps from the mid-run state
     el_p, rej_p, launch_p = timed_steps("plain", mx, args.steps, args.warmup)

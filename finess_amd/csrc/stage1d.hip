@@ -83,16 +83,41 @@ constexpr int small1d_cells()
 
 constexpr int SMALL1D_NT = 512;  // 2 waves per SIMD: the wide orders fit without spilling
 
+#ifndef FINESS_PAIR1D
+#define FINESS_PAIR1D 1
+#endif
+
 // One stage on a single workgroup: faces, barrier, cells, the CFL maximum.
 template <int WK>
 __device__ __forceinline__ void stage1d_small_body(const Stage1DArgs& a, double* red)
 {
     double amax = 0.0;
-    for (int f = threadIdx.x; f <= a.mx; f += SMALL1D_NT) {
-        double alpha = face1d_any<WK>(f, a);
-        if (alpha != alpha) alpha = __builtin_inf();
-        // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
-        if (f < a.mx) amax = alpha > amax ? alpha : amax;
+    bool paired = false;
+    if constexpr (WK <= WENO_FD5) paired = FINESS_PAIR1D && 2 * (a.mx + 1) <= SMALL1D_NT;
+    if (paired) {
+        // a grid of at most SMALL1D_NT / 2 faces: two lanes per face, one
+        // split part each, so that a face's two reconstructions run side by
+        // side; the even lane adds them in face1d's order
+        const int f = threadIdx.x >> 1, part = threadIdx.x & 1;
+        if (f <= a.mx) {
+            double alpha;
+            const double w = face1d_part<WK <= WENO_FD5 ? WK : WENO_JS5_P2>(f, a, part, alpha);
+            const double wo = __shfl_xor(w, 1, 64);
+            if (!part) {
+                a.fhat[f] = w + wo;
+                if (a.smax_faces) a.smax_faces[f] = alpha;
+                if (alpha != alpha) alpha = __builtin_inf();
+                // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
+                if (f < a.mx) amax = alpha > amax ? alpha : amax;
+            }
+        }
+    } else {
+        for (int f = threadIdx.x; f <= a.mx; f += SMALL1D_NT) {
+            double alpha = face1d_any<WK>(f, a);
+            if (alpha != alpha) alpha = __builtin_inf();
+            // GetCFL (lib/1d/GetCFL.cpp:30-33) reads faces 1..mx (1-based), i.e. f < mx
+            if (f < a.mx) amax = alpha > amax ? alpha : amax;
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < a.mx; i += SMALL1D_NT) update1d(i, a);
@@ -136,44 +161,99 @@ __global__ __launch_bounds__(SMALL1D_NT) void k_step1d_small(Stage1DStep s)
 // from the block after the barrier that follows it: a step that raised the
 // accepted count rotates the plane sets (an nv stop happens after its step
 // is accepted, as on the host); a rejection is cleared here and retried.
+//
+// The run works in LDS (run1d_lds_bytes): the three plane sets, the face
+// fluxes, the advection speeds, the CFL words and the controller block.  The stage code reaches
+// them through the same pointers it takes for global memory (generic
+// addresses), the state at entry and the block are read once, and the state
+// at exit, the CFL words and the block are written back once: each stage
+// then waits on LDS instead of on a round trip through L2 after every
+// barrier.
+#ifndef FINESS_RUN1D_LDS
+#define FINESS_RUN1D_LDS 1
+#endif
 template <int WK>
 __global__ __launch_bounds__(SMALL1D_NT) void k_run1d_small(Stage1DRun r)
 {
     __shared__ double red[SMALL1D_NT / 64];
     int cur = r.cur;
+#if FINESS_RUN1D_LDS
+    extern __shared__ double lds1d[];
+    __shared__ StepCtl lctl;
+    __shared__ unsigned long long lsmax[2 * CFL_SPREAD];
+    const int mx = r.base.mx;
+    double* const buf[3] = {lds1d, lds1d + mx, lds1d + 2 * mx};
+    double* const laux = lds1d + 4 * mx + 1;
+    for (int i = threadIdx.x; i < mx; i += SMALL1D_NT) {
+        buf[cur][i] = r.buf[cur][i];
+        laux[i] = r.base.aux[i];
+    }
+    for (int k = threadIdx.x; k < 2 * CFL_SPREAD; k += SMALL1D_NT) lsmax[k] = r.smax_slot[k];
+    if (threadIdx.x == 0) lctl = *r.ctl;
+    __syncthreads();
+    StepCtl* const ctl = &lctl;
+    unsigned long long* const smax = lsmax;
+    Stage1DArgs base = r.base;
+    base.fhat = lds1d + 3 * mx;
+    base.aux = laux;
+    base.dtp = &lctl.dt;
+    base.halt = &lctl.halt;
+#else
+    double* const* const buf = r.buf;
+    StepCtl* const ctl = r.ctl;
+    unsigned long long* const smax = r.smax_slot;
+    const Stage1DArgs base = r.base;
+#endif
     for (int att = 0; att < r.max_attempts; att++) {
-        if (__hip_atomic_load(&r.ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != HALT_NONE) break;
-        const long long acc0 = __hip_atomic_load(&r.ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_load(&ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != HALT_NONE) break;
+        const long long acc0 = __hip_atomic_load(&ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int k = 0; k < r.n; k++) {
             const SspStage g = ssp_stage(r.n, k);
-            Stage1DArgs a = r.base;
-            a.src = r.buf[(cur + g.src) % 3];
-            a.qa = r.buf[(cur + g.qa) % 3];
-            a.qb = g.qb >= 0 ? r.buf[(cur + g.qb) % 3] : nullptr;
+            Stage1DArgs a = base;
+            a.src = buf[(cur + g.src) % 3];
+            a.qa = buf[(cur + g.qa) % 3];
+            a.qb = g.qb >= 0 ? buf[(cur + g.qb) % 3] : nullptr;
             a.use_b = g.qb >= 0 && g.a2 != 0.0;
-            a.out = r.buf[(cur + g.out) % 3];
+            a.out = buf[(cur + g.out) % 3];
             a.a1 = g.a1;
             a.a2 = g.a2;
             a.beta = g.beta;
-            a.smax_slot = k == r.n - 1 ? r.smax_slot : nullptr;
+            a.smax_slot = k == r.n - 1 ? smax : nullptr;
             stage1d_small_body<WK>(a, red);
             __syncthreads();
         }
-        if (threadIdx.x < 64) step_control_wave(r.ctl, r.smax_slot, r.cp, threadIdx.x);
+        if (threadIdx.x < 64) step_control_wave(ctl, smax, r.cp, threadIdx.x);
         __syncthreads();
-        const unsigned long long h = __hip_atomic_load(&r.ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (__hip_atomic_load(&r.ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != acc0)
+        const unsigned long long h = __hip_atomic_load(&ctl->halt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_load(&ctl->acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != acc0)
             cur = (cur + ssp_result(r.n)) % 3;
         if (h == HALT_REJECT) {
             __syncthreads();  // every thread has read the halt word before it is cleared
-            if (threadIdx.x == 0) __hip_atomic_store(&r.ctl->halt, (unsigned long long)HALT_NONE, __ATOMIC_RELAXED,
+            if (threadIdx.x == 0) __hip_atomic_store(&ctl->halt, (unsigned long long)HALT_NONE, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
             continue;
         }
         if (h != HALT_NONE) break;
     }
+#if FINESS_RUN1D_LDS
+    for (int i = threadIdx.x; i < mx; i += SMALL1D_NT) r.buf[cur][i] = buf[cur][i];
+    for (int k = threadIdx.x; k < 2 * CFL_SPREAD; k += SMALL1D_NT) r.smax_slot[k] = lsmax[k];
+    if (threadIdx.x == 0) {
+        lctl.buf = cur;
+        *r.ctl = lctl;
+    }
+#else
     if (threadIdx.x == 0) r.ctl->buf = cur;
+#endif
+}
+
+// Dynamic LDS of the run kernel: three plane sets of mx cells, mx + 1 face
+// fluxes and mx advection speeds (61 KB at the largest single-workgroup
+// grid, 1536 cells).
+static size_t run1d_lds_bytes(int mx)
+{
+    return FINESS_RUN1D_LDS ? (size_t)(5 * mx + 1) * sizeof(double) : 0;
 }
 
 template <int WK>
@@ -247,14 +327,17 @@ int launch_run1d(int wk, const Stage1DRun& r, hipStream_t st)
     if (r.n < 1 || r.n > 3 || !run1d_fits(wk, r.base.mx)) return -1;
     if (r.base.galpha) hipLaunchKernelGGL(k_global_speed1d, dim3(1), dim3(256), 0, st, r.base);
     return with_kind(wk, [&](auto k) {
-        hipLaunchKernelGGL(k_run1d_small<decltype(k)::value>, dim3(1), dim3(SMALL1D_NT), 0, st, r);
+        hipLaunchKernelGGL(k_run1d_small<decltype(k)::value>, dim3(1), dim3(SMALL1D_NT), run1d_lds_bytes(r.base.mx),
+                           st, r);
         return hipGetLastError() == hipSuccess ? 0 : -4;
     });
 }
 
 bool run1d_fits(int wk, int mx)
 {
-    return with_kind(wk, [&](auto k) { return mx <= small1d_cells<decltype(k)::value>() ? 1 : 0; }) == 1;
+    // (and the run's LDS within the 64 KB a launch gets without an attribute)
+    return run1d_lds_bytes(mx) + 2048 <= 65536 &&
+           with_kind(wk, [&](auto k) { return mx <= small1d_cells<decltype(k)::value>() ? 1 : 0; }) == 1;
 }
 
 int launch_step1d(int wk, const Stage1DStep& s, hipStream_t st)

@@ -85,6 +85,30 @@ __host__ __device__ __forceinline__ long long idx1d(const Stage1DArgs& a, int g)
 }
 
 // Face f sits between cells f-1 and f (0-based); lib/1d/ConstructL.cpp:49-165.
+// The reconstruction of split part `part` alone (0: f + alpha q over cells
+// 0..4, 1: f - alpha q over cells 5..1), for two lanes per face; alpha
+// returned through `alpha`.  The same arithmetic as face1d below.
+template <int WK>
+__host__ __device__ __forceinline__ double face1d_part(int f, const Stage1DArgs& a, int part, double& alpha)
+{
+    const WenoParams<double> wp{a.eps, a.power};
+    double q[6], fl[6];
+    for (int s = 0; s < 6; s++) {
+        const long long k = idx1d(a, f - 3 + s);
+        q[s] = a.src[k];
+        fl[s] = a.lxwF ? a.lxwF[f + s] : q[s] * a.aux[k];  // FluxFunc: Q(i,1)*Aux(i,1)
+    }
+    const double al = a.aux[idx1d(a, f - 1)], ar = a.aux[idx1d(a, f)];
+    const double s1 = rmin(al, 0.5 * (al + ar));
+    const double s2 = rmax(ar, 0.5 * (al + ar));
+    alpha = alpha1d(a, rmax(fabs(s1), fabs(s2)));
+    const double la = a.alpha_scaling * alpha;
+    double g[5];
+    for (int s = 0; s < 5; s++) g[s] = part ? 0.5 * (fl[5 - s] - la * q[5 - s]) : 0.5 * (fl[s] + la * q[s]);
+    return weno5<WK>(g, wp);
+}
+
+// Both parts in one pass (the stencil read and alpha formed once).
 template <int WK>
 __host__ __device__ __forceinline__ double face1d(int f, const Stage1DArgs& a)
 {

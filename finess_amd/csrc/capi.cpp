@@ -84,6 +84,11 @@ struct finess_ctx {
     bool devctl = false;                          // the device controller drives the current run
     const StepCtlParams* ctlp = nullptr;          // its parameters while devctl
     bool ctl_enqueued = false;                    // enqueue_step launched the controller with the step
+    // the last step's controller, not launched yet: it runs in the next
+    // stage's troubled-face pass, or on its own before an indicator launch
+    // or the chunk's read-back (flush_ctl)
+    bool ctl_pending = false;
+    int ctl_pending_slot = 0;
     bool route_fresh = true;                      // next routed stage starts a step
     int rec_age = -1;                             // steps since the record was decided (-1: none)
     int rec_par = 0;                              // counter pair of the current record
@@ -216,12 +221,27 @@ int tmark(finess_ctx* c)
     return 0;
 }
 
+#ifndef FINESS_FOLD_CTL
+#define FINESS_FOLD_CTL 1
+#endif
+
+// Launches the pending controller on its own.
+int flush_ctl(finess_ctx* c)
+{
+    if (!c->ctl_pending) return 0;
+    c->ctl_pending = false;
+    if (launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * c->ctl_pending_slot, *c->ctlp, c->st))
+        return fail(c, FINESS_ERR_HIP, "controller launch failed");
+    return 0;
+}
+
 int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
 {
     const finess_params& p = c->p;
     if (p.time_stepping_method != FINESS_TS_RK) return fail(c, FINESS_ERR_CONFIG, "routing needs the RK stepper");
     if (p.routing == FINESS_ROUTE_LEARNED && !c->frags)
         return fail(c, FINESS_ERR_CONFIG, "routing = learned but no indicator installed (finess_set_indicator)");
+    int rc0 = 0;
     a.frags = c->frags;
     a.route_exact = p.routing == FINESS_ROUTE_ORACLE;
     a.route_tau = p.route_tau;
@@ -243,7 +263,20 @@ int stage_routed(finess_ctx* c, StageArgs& a, bool stepping)
             a.route_n = c->route_n + 2 * c->rec_par;
         }
         c->route_fresh = false;
+        // the previous step's controller: with the troubled-face pass of a
+        // stage that reuses the record, else before the indicator
+        if (c->ctl_pending) {
+            if (a.route_mask_read) {
+                a.ctl_run = c->ctl;
+                a.ctl_smax = c->smax + 2 * CFL_SPREAD * c->ctl_pending_slot;
+                a.ctl_p = *c->ctlp;
+                c->ctl_pending = false;
+            } else if (const int rc = flush_ctl(c)) {
+                return rc;
+            }
+        }
     }
+    if (c->ctl_pending && (rc0 = flush_ctl(c))) return rc0;
     const int rc = launch_stage2d_routed(p.physics, p.precision == FINESS_PREC_FP32, weno_kind(p), a, c->st);
     return rc ? fail(c, rc == -1 ? FINESS_ERR_CONFIG : FINESS_ERR_HIP, "routed stage launch failed") : 0;
 }
@@ -1459,6 +1492,12 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
         run = false;
     }
     std::vector<Snap> snap;
+    // the controller of a routed run with per-step records (and no per-step
+    // conservation records, which follow the controller) goes with the next
+    // step's troubled-face pass where it can (stage_routed)
+    const bool fold_ctl = FINESS_FOLD_CTL && p.ndims == 2 && p.time_stepping_method == FINESS_TS_RK &&
+                          p.time_order <= 3 && p.routing != FINESS_ROUTE_NONE && p.route_per_step > 0 &&
+                          c->route_tile && !c->steplog;
     long long seen = 0;   // attempts processed before the current chunk
     long long logged = 0; // conservation records read back
     long long kmax = 64;  // chunk length: 64, or shorter after a rejection
@@ -1478,9 +1517,14 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
             int res = 0, slot = 0;
             c->ctl_enqueued = false;
             if ((rc = enqueue_step(c, 0.0, &res, &slot))) break;
-            if (!c->ctl_enqueued && launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * slot, cp, c->st)) {
-                rc = fail(c, FINESS_ERR_HIP, "controller launch failed");
-                break;
+            if (!c->ctl_enqueued) {
+                if (fold_ctl) {
+                    c->ctl_pending = true;
+                    c->ctl_pending_slot = slot;
+                } else if (launch_step_control(c->ctl, c->smax + 2 * CFL_SPREAD * slot, cp, c->st)) {
+                    rc = fail(c, FINESS_ERR_HIP, "controller launch failed");
+                    break;
+                }
             }
             // the attempt's conservation record, written only if the controller just accepted it
             if (c->steplog && launch_con_soln(c->buf[res], c->plane, c->M, step_log_weight(p), c->spart, c->ctl,
@@ -1495,6 +1539,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
             snap.push_back(sn);
             c->cur = res;
         }
+        if (!rc) rc = flush_ctl(c);
         if (!rc && hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->st) != hipSuccess) rc = FINESS_ERR_HIP;
         if (!rc && hipStreamSynchronize(c->st) != hipSuccess) rc = FINESS_ERR_HIP;
         if (rc) {
@@ -1544,6 +1589,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
     c->devctl = false;
     c->ctlp = nullptr;
     c->ctl_enqueued = false;
+    c->ctl_pending = false;
     if (!rc) dtv[1] = h.dt_next;
     c->t = h.t;
     if (stats) {

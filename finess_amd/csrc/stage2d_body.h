@@ -26,6 +26,7 @@
 #pragma once
 #include "physics.h"
 #include "source.h"
+#include "step_control.h"
 #include "weno.h"
 
 namespace finess {
@@ -177,6 +178,11 @@ struct StageArgs {
     int src_kind;
     double src_coef;
     const double* aux;
+    // the previous step's device controller, run by the first workgroup of
+    // the troubled-face pass (null: none); its CFL words ctl_smax
+    struct StepCtl* ctl_run;
+    unsigned long long* ctl_smax;
+    StepCtlParams ctl_p;
 };
 
 // beta * dt of the stage update (dt from the device controller when set)

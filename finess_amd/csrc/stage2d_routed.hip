@@ -21,6 +21,7 @@
 #include "mlp.h"
 #include "mlp_infer.h"
 #include "stage2d_body.h"
+#include "step_control_body.h"
 
 namespace finess {
 
@@ -407,12 +408,27 @@ __device__ __forceinline__ void route_dir(const StageArgs& a, long long g0, long
 // direction (a compile-time constant in route_face) and every lane knows its
 // first list slot before the list lengths arrive: the halt word, the lengths
 // and that slot's entry are read together instead of one after another.
+//
+// With a.ctl_run the first workgroup runs the previous step's controller
+// instead (the pass does not read dt; the stage launch after it does): its
+// latency hides behind the pass's, and the step saves a launch.  Workgroups
+// that read the halt word before the controller set it compute fluxes no
+// later launch reads: the stage kernels after it return at once.
 template <class P, typename R, int WK>
 __global__ __launch_bounds__(256) void k_route_full(StageArgs a)
 {
-    const int half = (int)gridDim.x / 2;
-    const int dir = (int)blockIdx.x >= half;
-    const long long g0 = (long long)((int)blockIdx.x - dir * half) * blockDim.x + threadIdx.x;
+    int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    if (a.ctl_run) {
+        if (b == 0) {
+            if (threadIdx.x < 64) step_control_wave(a.ctl_run, a.ctl_smax, a.ctl_p, (int)threadIdx.x);
+            return;
+        }
+        b -= 1;
+        nb -= 1;
+    }
+    const int half = nb / 2;
+    const int dir = b >= half;
+    const long long g0 = (long long)(b - dir * half) * blockDim.x + threadIdx.x;
     const long long step = (long long)half * blockDim.x;
     const long long nfx = (long long)(a.mx + 1) * a.my, nfy = (long long)a.mx * (a.my + 1);
     const long long off = dir ? nfx : 0;
@@ -566,7 +582,8 @@ static int launch_routed(const StageArgs& a0, hipStream_t st)
             else
                 hipLaunchKernelGGL((k_classify<Euler2D, R, TI, TJ, false>), grid, block, 0, st, a);
         }
-        hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(ROUTE_FULL_BLOCKS), block, 0, st, a);
+        hipLaunchKernelGGL((k_route_full<Euler2D, R, WENO_JS5_P2>), dim3(ROUTE_FULL_BLOCKS + (a.ctl_run ? 1 : 0)),
+                           block, 0, st, a);
         hipLaunchKernelGGL((k_stage2d_lookup<Euler2D, R, TI, TJ>), grid, block, 0, st, a);
     } else if (a.route_exact)
         hipLaunchKernelGGL((k_stage2d_routed<Euler2D, R, WENO_JS5_P2, TI, TJ, true>), grid, block, 0, st, a);
@@ -585,6 +602,9 @@ int launch_stage2d_routed(int physics, int fp32, int wk, const StageArgs& a, hip
     if (physics != PHYS_EULER || wk != WENO_JS5_P2) return -1;
     if ((a.route_mask_read || a.route_tile) && !(a.route_tile && a.route_list && a.route_n && a.route_flux && a.out))
         return -1;
+    // the controller rides only on a stage that reuses a record (whose
+    // troubled-face pass has no indicator launch before it)
+    if (a.ctl_run && !(a.route_mask_read && a.route_tile && a.ctl_smax)) return -1;
     return fp32 ? launch_routed<float>(a, st) : launch_routed<double>(a, st);
 }
 

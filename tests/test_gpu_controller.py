@@ -1,5 +1,6 @@
 """The device-side time-step controller (k_step_control, one launch per step
-attempt) against the host controller: the same accepted and rejected
+attempt; in the routed form with records, the first workgroup of the next
+step's troubled-face pass) against the host controller: the same accepted and rejected
 attempts, the same final dt and bit-identical states.  The controller's
 arithmetic is the host's operation for operation and the stage kernels are
 the same, so nothing may differ at all.  Cases: frames with tend clipping,
@@ -75,6 +76,16 @@ def test_routed_record_reuse_with_rejection():
                                 precision=PREC_FP32)
     st, _ = _same(p, q, lambda s, dtv: s.advance(0.02, dtv))
     assert st.steps_rejected >= 1 and st.steps_accepted > 4
+
+
+def test_routed_fixed_steps_across_chunks():
+    """The routed form with a record every third step over 70 steps (a chunk
+    boundary after 64): a step's controller runs in the next step's
+    troubled-face pass when that step reuses the record, on its own before an
+    indicator launch, and on its own before each chunk's read-back."""
+    p, q, _ = problems.riemann4(mx=64, routing=ROUTE_ORACLE, route_per_step=3, precision=PREC_FP32)
+    st, _ = _same(p, q, lambda s, dtv: s.advance_steps(70, dtv))
+    assert st.steps_accepted == 70
 
 
 def test_nothing_to_do():

@@ -498,7 +498,7 @@ def main():
         print(json.dumps(wide_order(args.wide, args.wide_version)), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+        print("bench.py: finess_amd runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU slab decomposition is a design, not code)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     from finess_amd.indicator import Indicator, train_indicator

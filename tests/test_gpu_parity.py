@@ -297,12 +297,14 @@ def test_fin_solve_rk_advection_full_run(time_order):
     assert l1 < {1: 5e-3, 2: 2e-4, 3: 2e-6}[time_order]
 
 
-@pytest.mark.parametrize("mx,order", [(1536, 5), (1537, 5), (2000, 5), (512, 11), (600, 11)])
+@pytest.mark.parametrize("mx,order", [(255, 5), (256, 5), (1536, 5), (1537, 5), (2000, 5), (512, 11), (600, 11)])
 def test_fin_solve_rk_advection_launch_paths(mx, order):
     """The 1D run on either side of the single-workgroup limit (1536 cells
     for WENO5, 512 for WENO11): the whole run in one workgroup up to it, two
-    launches per stage and the controller kernel per step above it; both
-    against the oracle with the controller, to t = 0.05."""
+    launches per stage and the controller kernel per step above it; and on
+    either side of the fifth-order faces' two-lanes-per-face form (up to 255
+    cells, 256 faces on the 512 threads).  All against the oracle with the
+    controller, to t = 0.05."""
     r = (order + 1) // 2
     p, q, aux = problems.advection_1d(mx=mx, space_order=order, mbc=max(3, r), initial_dt=0.4 / mx)
     tend = 0.05

@@ -88,8 +88,13 @@ __global__ __launch_bounds__(256) void k_batch_stage(BatchArgs b)
 #endif
         if (ax != ax) ax = R(__builtin_inf());
         if (ay != ay) ay = R(__builtin_inf());
-        ax = wave_max64(ax);
-        ay = wave_max64(ay);
+        if constexpr (sizeof(R) == sizeof(float)) {
+            ax = wave_max64_xor(ax);
+            ay = wave_max64_xor(ay);
+        } else {
+            ax = wave_max64(ax);
+            ay = wave_max64(ay);
+        }
         if ((t & 63) == 0) { red[0][t >> 6] = double(ax); red[1][t >> 6] = double(ay); }
         __syncthreads();
         if (t == 0) {

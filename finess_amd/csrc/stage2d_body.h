@@ -60,16 +60,58 @@ constexpr int CFL_SPREAD = 64;
 
 // Maximum over the 64 lanes of a wave (the wave-speed maxima: non-negative
 // doubles or +inf).
-__device__ __forceinline__ double wave_max64(double v)
+//
+// A DPP ladder: the maximum over each quad (quad permutations), half row
+// (row_half_mirror) and row (row_mirror), then row 0's into row 1 and row
+// 2's into row 3 (row_bcast15), then rows 0-1's into rows 2-3 (row_bcast31),
+// so that lane 63 holds the wave's maximum; it is returned in every lane.
+// Lanes a broadcast does not write keep their own value (`old` = src).
+// Register moves between lanes instead of LDS-crossbar permutes.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_move(int x)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_max_step(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<CTRL, ROWS>((int)b), hi = dpp_move<CTRL, ROWS>((int)(b >> 32));
+    const double w = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    return w > v ? w : v;
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_max_step(float v)
+{
+    const float w = __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+    return w > v ? w : v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max64_dpp(T v)
+{
+    v = dpp_max_step<0xB1>(v);        // quad_perm [1,0,3,2]
+    v = dpp_max_step<0x4E>(v);        // quad_perm [2,3,0,1]
+    v = dpp_max_step<0x141>(v);       // row_half_mirror
+    v = dpp_max_step<0x140>(v);       // row_mirror
+    v = dpp_max_step<0x142, 0xA>(v);  // row_bcast15 into rows 1 and 3
+    v = dpp_max_step<0x143, 0xC>(v);  // row_bcast31 into rows 2 and 3
     return v;
 }
-// the same for FP32 values (half the shuffles and compares)
+__device__ __forceinline__ double wave_max64(double v)
+{
+    v = wave_max64_dpp(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// the same for FP32 values (one move per step)
 __device__ __forceinline__ float wave_max64(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_max64_dpp(v)), 63));
+}
+// the FP32 maximum by lane permutes (the batched stage measured 1.3% faster
+// with it than with the DPP ladder; the other kernels gain with the ladder)
+__device__ __forceinline__ float wave_max64_xor(float v)
 {
     for (int o = 32; o > 0; o >>= 1) {
         const float w = __shfl_xor(v, o, 64);

@@ -28,19 +28,16 @@ __device__ __forceinline__ void step_control_wave(StepCtl* c, unsigned long long
     unsigned long long by = __hip_atomic_load(smax + CFL_SPREAD + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     StepCtl h;
     if (l == 0) h = *c;
-    const unsigned long long halt = __shfl(l == 0 ? h.halt : 0ull, 0, 64);
+    const unsigned long long h0 = l == 0 ? h.halt : 0ull;
+    const unsigned long long halt = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(h0 >> 32), 0) << 32) |
+                                    (unsigned)__builtin_amdgcn_readlane((int)h0, 0);
     if (halt != HALT_NONE) return;
     smax[l] = 0ull;
     smax[CFL_SPREAD + l] = 0ull;
-    // bit patterns of non-negative doubles (or +inf) order as the values
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long ox = (unsigned long long)__double_as_longlong(
-            __shfl_xor(__longlong_as_double((long long)bx), o, 64));
-        const unsigned long long oy = (unsigned long long)__double_as_longlong(
-            __shfl_xor(__longlong_as_double((long long)by), o, 64));
-        bx = ox > bx ? ox : bx;
-        by = oy > by ? oy : by;
-    }
+    // the words hold non-negative doubles or +inf (never NaN), whose maximum
+    // is the maximum of their bit patterns; wave_max64 returns it in every lane
+    bx = (unsigned long long)__double_as_longlong(wave_max64(__longlong_as_double((long long)bx)));
+    by = (unsigned long long)__double_as_longlong(wave_max64(__longlong_as_double((long long)by)));
     if (l != 0) return;
     const double sx = __longlong_as_double((long long)bx), sy = __longlong_as_double((long long)by);
     const double dt = h.dt, told = h.t;

@@ -102,7 +102,11 @@ __device__ __forceinline__ void stage1d_small_body(const Stage1DArgs& a, double*
         if (f <= a.mx) {
             double alpha;
             const double w = face1d_part<WK <= WENO_FD5 ? WK : WENO_JS5_P2>(f, a, part, alpha);
-            const double wo = __shfl_xor(w, 1, 64);
+            // the partner lane's part (quad_perm [1,0,3,2]: both lanes of a
+            // pair take this branch together)
+            const long long wb = __double_as_longlong(w);
+            const int lo = dpp_move<0xB1>((int)wb), hi = dpp_move<0xB1>((int)(wb >> 32));
+            const double wo = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
             if (!part) {
                 a.fhat[f] = w + wo;
                 if (a.smax_faces) a.smax_faces[f] = alpha;

@@ -94,8 +94,8 @@ __global__ __launch_bounds__(256) void k_mpp_speed(MppArgs a)
 __device__ __forceinline__ void mpp_global_speeds(const MppArgs& a, double& ax, double& ay)
 {
     const int lane = threadIdx.x & 63;
-    ax = wave_max64(__longlong_as_double((long long)a.gmax[lane]));
-    ay = wave_max64(__longlong_as_double((long long)a.gmax[CFL_SPREAD + lane]));
+    ax = wave_max64(spread_lane_max(a.gmax, lane));
+    ay = wave_max64(spread_lane_max(a.gmax + CFL_SPREAD, lane));
 }
 
 __global__ __launch_bounds__(256) void k_mpp_lf(MppArgs a)

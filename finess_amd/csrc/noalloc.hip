@@ -28,12 +28,12 @@ namespace finess {
 
 namespace {
 
-// Scratch layout (bytes):
+// Scratch layout (bytes, with CFL_SPREAD = 64):
 //   [0, 1024)      2 x CFL_SPREAD words: per-direction wave-speed maxima of
 //                  the stage's workgroups (bit patterns of non-negative doubles)
 //   [1024, 2048)   2 x CFL_SPREAD words: GlobalWaveSpd's partial maxima
 //   [2048, 2064)   the two global wave speeds (global_alpha)
-constexpr size_t SCRATCH_BYTES = 4096;
+constexpr size_t SCRATCH_BYTES = 4096 * (CFL_SPREAD / 64);
 constexpr size_t OFF_CFL = 0;
 constexpr size_t OFF_GSPREAD = OFF_CFL + 2 * CFL_SPREAD * sizeof(unsigned long long);
 constexpr size_t OFF_GALPHA = OFF_GSPREAD + 2 * CFL_SPREAD * sizeof(unsigned long long);
@@ -65,10 +65,8 @@ hipError_t scratch_ptr(int dev, unsigned char** out)
 __global__ __launch_bounds__(64) void k_fold_cfl_words(const unsigned long long* w, double* out)
 {
     const int t = threadIdx.x;
-    double vx = __longlong_as_double((long long)w[t]);
-    double vy = __longlong_as_double((long long)w[CFL_SPREAD + t]);
-    vx = wave_max64(vx);
-    vy = wave_max64(vy);
+    const double vx = wave_max64(spread_lane_max(w, t));
+    const double vy = wave_max64(spread_lane_max(w + CFL_SPREAD, t));
     if (t == 0) {
         out[0] = vx;
         out[1] = vy;

@@ -173,10 +173,8 @@ __global__ __launch_bounds__(64) void k_global_speed_final(StageArgs a)
 {
     if (stage_halted(a)) return;
     const int t = threadIdx.x;
-    double vx = __longlong_as_double((long long)a.gspread[t]);
-    double vy = __longlong_as_double((long long)a.gspread[CFL_SPREAD + t]);
-    vx = wave_max64(vx);
-    vy = wave_max64(vy);
+    const double vx = wave_max64(spread_lane_max(a.gspread, t));
+    const double vy = wave_max64(spread_lane_max(a.gspread + CFL_SPREAD, t));
     if (t == 0) {
         a.galpha[0] = vx;
         a.galpha[1] = vy;
@@ -185,7 +183,7 @@ __global__ __launch_bounds__(64) void k_global_speed_final(StageArgs a)
 
 static int launch_global_speed(int physics, const StageArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(2 * CFL_SPREAD), 0, st, a.gspread, 2 * CFL_SPREAD, 0ull);
+    if (launch_fill_u64(a.gspread, 2 * CFL_SPREAD, 0ull, st)) return -4;
     const unsigned nb = (unsigned)std::min<long long>(1024, (a.plane + 255) / 256);
     if (physics == PHYS_EULER) hipLaunchKernelGGL(k_global_speed<Euler2D>, dim3(nb), dim3(256), 0, st, a);
     else if (physics == PHYS_SHALLOW_WATER) hipLaunchKernelGGL(k_global_speed<ShallowWater2D>, dim3(nb), dim3(256), 0, st, a);

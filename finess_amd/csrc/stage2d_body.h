@@ -56,7 +56,21 @@ __host__ __device__ __forceinline__ int bc_map(int bc_lo, int bc_hi, int g, int 
 // per quantity (chosen by block index): a single word hit by every
 // workgroup of a launch serialises thousands of atomics in one L2 channel.
 // The host reduces the CFL_SPREAD words when it reads them.
-constexpr int CFL_SPREAD = 64;
+#ifndef FINESS_CFL_SPREAD
+#define FINESS_CFL_SPREAD 64
+#endif
+constexpr int CFL_SPREAD = FINESS_CFL_SPREAD;
+static_assert(CFL_SPREAD % 64 == 0, "the readers take CFL_SPREAD / 64 words per lane");
+
+// The maximum of one direction's spread words w[0 .. CFL_SPREAD) (bit
+// patterns of non-negative doubles or +inf) over the words lane, lane + 64,
+// ... of this lane; a wave maximum of the results folds them all.
+__device__ __forceinline__ double spread_lane_max(const unsigned long long* w, int lane)
+{
+    unsigned long long m = w[lane];
+    for (int k = lane + 64; k < CFL_SPREAD; k += 64) m = w[k] > m ? w[k] : m;
+    return __longlong_as_double((long long)m);
+}
 
 // Maximum over the 64 lanes of a wave (the wave-speed maxima: non-negative
 // doubles or +inf).

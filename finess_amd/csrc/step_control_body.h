@@ -20,20 +20,26 @@ namespace finess {
 
 __device__ __forceinline__ void step_control_wave(StepCtl* c, unsigned long long* smax, const StepCtlParams& p, int l)
 {
-    static_assert(CFL_SPREAD == 64, "one maxima word per lane and direction");
-    // every load is issued before anything waits on one: the maxima, and
-    // the whole control block in lane 0 (decided on a register copy and
-    // stored back in one piece)
-    unsigned long long bx = __hip_atomic_load(smax + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long by = __hip_atomic_load(smax + CFL_SPREAD + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // every load is issued before anything waits on one: the maxima (words
+    // l, l + 64, ... of each direction), and the whole control block in lane 0
+    // (decided on a register copy and stored back in one piece)
+    unsigned long long bx = 0ull, by = 0ull;
+    for (int k = l; k < CFL_SPREAD; k += 64) {
+        const unsigned long long x = __hip_atomic_load(smax + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long y = __hip_atomic_load(smax + CFL_SPREAD + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bx = x > bx ? x : bx;
+        by = y > by ? y : by;
+    }
     StepCtl h;
     if (l == 0) h = *c;
     const unsigned long long h0 = l == 0 ? h.halt : 0ull;
     const unsigned long long halt = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(h0 >> 32), 0) << 32) |
                                     (unsigned)__builtin_amdgcn_readlane((int)h0, 0);
     if (halt != HALT_NONE) return;
-    smax[l] = 0ull;
-    smax[CFL_SPREAD + l] = 0ull;
+    for (int k = l; k < CFL_SPREAD; k += 64) {
+        smax[k] = 0ull;
+        smax[CFL_SPREAD + k] = 0ull;
+    }
     // the words hold non-negative doubles or +inf (never NaN), whose maximum
     // is the maximum of their bit patterns; wave_max64 returns it in every lane
     bx = (unsigned long long)__double_as_longlong(wave_max64(__longlong_as_double((long long)bx)));

@@ -21,7 +21,9 @@ CFL 0.4, transmissive boundaries, t_final 0.3.  Two forms of the same solver:
 A "step" is one accepted time step: three stages (plain: one fused kernel
 each; learned: the troubled-face pass and the lookup stage kernel, with the
 indicator kernel in front at every fourth step) and the time-step
-controller, which runs on the device after each step.  Both forms are timed over
+controller, which runs on the device after each step (in the learned form
+inside the next step's troubled-face pass, or on its own before an
+indicator step).  Both forms are timed over
 K steps from the state at t = 0.15 (the middle of the run) after W warm-up
 steps; inputs are resident in HBM before the timed region starts.  The L1
 density error of each form at t_final is measured against a 2048 x 2048
@@ -61,14 +63,14 @@ STAGES_PER_STEP = 3
 # counts in profiles/k_stage2d_lookup_euler_fp32_riemann4_1024.md and
 # profiles/k_stage2d_euler_fp64_riemann4_1024.md)
 TRAFFIC_ROUTED_1024 = 134.0e6   # mean over the 12 stages of a 4-step indicator interval
-TRAFFIC_PLAIN_1024 = 113.9e6
+TRAFFIC_PLAIN_1024 = 114.1e6
 # VALU instructions per launch at 1024x1024 (rocprofv3 SQ_INSTS_VALU and the
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters, same profile files): a wave64
 # VALU instruction holds a SIMD for 4 cycles, so at 100% issue a launch takes
 # N x 4 / (1024 SIMDs x 2.4 GHz).  The bench divides that by the measured
 # launch time.
-VALU_PLAIN_1024 = 40.00e6        # k_stage2d, FP64
-VALU_F64_PLAIN_1024 = 31.07e6    # of which FP64 arithmetic
+VALU_PLAIN_1024 = 40.55e6        # k_stage2d, FP64
+VALU_F64_PLAIN_1024 = 31.13e6    # of which FP64 arithmetic
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 T_FINAL = 0.3
 T_TIMED = 0.15

@@ -26,7 +26,7 @@ OK, ERR_CONFIG, ERR_NONFINITE, ERR_NV, ERR_HIP, ERR_DT, ERR_CFL, ERR_ARG, ERR_NO
 WENO_JS, WENO_FD, WENO_Z = 0, 1, 2
 TS_RK, TS_LXW, TS_PIF = 0, 1, 2
 PHYS_ADVECTION, PHYS_EULER, PHYS_SHALLOW_WATER = 0, 1, 2
-BC_EXTRAP, BC_PERIODIC, BC_REFLECT = 0, 1, 2
+BC_EXTRAP, BC_PERIODIC, BC_REFLECT, BC_HOST = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 ROUTE_NONE, ROUTE_LEARNED, ROUTE_ORACLE = 0, 1, 2
 SRC_ZERO, SRC_SW_BATHYMETRY, SRC_DAMPING = 0, 1, 2
@@ -118,6 +118,8 @@ class Stats(ctypes.Structure):
 
 _lib = None
 _PD = ctypes.POINTER(ctypes.c_double)
+# finess_bnd_fn: int (*)(void* user, double t, double* q)
+BND_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double))
 _PF = ctypes.POINTER(ctypes.c_float)
 MLP_NPARAM = 1889
 
@@ -134,6 +136,7 @@ SIGNATURES = {
     "finess_fin_solve_rk": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_fin_solve_lxw": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_set_step_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "finess_set_bnd_callback": (ctypes.c_int, [ctypes.c_void_p, BND_FN, ctypes.c_void_p]),
     "finess_read_step_log": (ctypes.c_int, [ctypes.c_void_p, _PD, ctypes.c_int]),
     "finess_fin_solve_rk_pif": (ctypes.c_int, [ctypes.c_void_p, _PD, _PD, _PD, ctypes.c_double, _PD]),
     "finess_apply_mpp_limiter_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _PD, _PD, _PD, _PD, _PD]),
@@ -293,6 +296,24 @@ class Solver:
         self._check(self.lib.finess_construct_integrated_r(self.h, float(dt), _ptr(q), None, _ptr(F), _ptr(G)),
                     "construct_integrated_r")
         return F, G
+
+    def set_bnd_callback(self, fn):
+        """SetBndValues for the FINESS_BC_HOST sides: ``fn(t, q)`` receives the
+        reference-layout state with ghosts (interior cells within mbc of the
+        boundary filled, the rest NaN) and fills its ghost cells in place."""
+        shape = self.p.shape_q()
+        n = int(np.prod(shape))
+
+        def tramp(_user, t, qp):
+            try:
+                fn(float(t), np.ctypeslib.as_array(qp, shape=(n,)).reshape(shape))
+                return 0
+            except Exception:
+                return 1
+
+        self._bnd = BND_FN(tramp) if fn is not None else None  # kept alive while installed
+        self._check(self.lib.finess_set_bnd_callback(self.h, self._bnd if fn is not None else BND_FN(), None),
+                    "set_bnd_callback")
 
     def set_step_log(self, on=True):
         self._check(self.lib.finess_set_step_log(self.h, 1 if on else 0), "set_step_log")

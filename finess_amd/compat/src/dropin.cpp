@@ -71,6 +71,7 @@ int bc_word(const std::string& w)
     if (w == "periodic") return FINESS_BC_PERIODIC;
     if (w == "reflective" || w == "wall" || w == "reflect") return FINESS_BC_REFLECT;
     if (w == "extrapolation" || w == "transmissive" || w == "outflow") return FINESS_BC_EXTRAP;
+    if (w == "host") return FINESS_BC_HOST;  // the app's SetBndValues on the host, before every stage
     throw std::logic_error("gpu.boundary: unknown boundary kind " + w);
 }
 
@@ -349,6 +350,33 @@ int source_kind_of_app(finess_params p, double* coef)
 
 } // namespace
 
+#if NDIMS == 2
+namespace {
+// aux of the running FinSolve call (reference layout), for the host
+// SetBndValues of FINESS_BC_HOST sides
+std::vector<double> g_bnd_aux;
+
+// finess_bnd_fn: the app's SetBndValues on a StateVars holding q at time t
+int bnd_trampoline(void*, double t, double* q)
+{
+    const IniParams& ini = global_ini_params;
+    try {
+        StateVars Q(t, ini.get_mx(), ini.get_my(), ini.get_meqn(), ini.get_maux(), ini.get_mbc());
+        dTensorBC3& qq = Q.ref_q();
+        for (int k = 0; k < qq.numel(); k++) qq.vset(k, q[k]);
+        dTensorBC3& aux = Q.ref_aux();
+        if ((long long)g_bnd_aux.size() == (long long)aux.numel())
+            for (int k = 0; k < aux.numel(); k++) aux.vset(k, g_bnd_aux[(size_t)k]);
+        SetBndValues(Q);
+        for (int k = 0; k < qq.numel(); k++) q[k] = qq.vget(k);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+} // namespace
+#endif
+
 ProbeReport probe_app()
 {
     const IniParams& ini = global_ini_params;
@@ -401,20 +429,26 @@ ProbeReport probe_app()
         const int mx = ini.get_mx(), mbc = ini.get_mbc(), maux = ini.get_maux();
 #if NDIMS == 2
         const int my = ini.get_my();
-        StateVars Q(0.0, mx, my, meqn, maux, mbc);
-        dTensorBC3& q = Q.ref_q();
-        for (int i = 1 - mbc; i <= mx + mbc; i++)
-            for (int j = 1 - mbc; j <= my + mbc; j++)
-                for (int m = 1; m <= meqn; m++)
-                    q.set(i, j, m, (i >= 1 && i <= mx && j >= 1 && j <= my) ? code(i, j, m, my) : NAN);
-        SetBndValues(Q);
         const int nx = r.physics == FINESS_PHYS_ADVECTION ? -1 : 2, ny = r.physics == FINESS_PHYS_ADVECTION ? -1 : 3;
-        for (int side = 0; side < 4; side++) {
-            int kind = 0;
-            if (!classify_side(Q, side, mx, my, mbc, meqn, side < 2 ? nx : ny, kind))
-                throw std::logic_error("finess-hip: SetBndValues side " + std::to_string(side) +
-                                       " is not an extrapolation, periodic or reflecting rule");
-            r.bc[side] = kind;
+        // at two times: a side whose rule is not one of the device's, or
+        // changes with time (inflow such as apps/2d/euler/mach_reflection/
+        // SetBndValues.cpp:103-139), runs the app's SetBndValues on the host
+        // before every stage (FINESS_BC_HOST)
+        for (int pass = 0; pass < 2; pass++) {
+            StateVars Q(pass ? 1.0 : 0.0, mx, my, meqn, maux, mbc);
+            dTensorBC3& q = Q.ref_q();
+            for (int i = 1 - mbc; i <= mx + mbc; i++)
+                for (int j = 1 - mbc; j <= my + mbc; j++)
+                    for (int m = 1; m <= meqn; m++)
+                        q.set(i, j, m, (i >= 1 && i <= mx && j >= 1 && j <= my) ? code(i, j, m, my) : NAN);
+            SetBndValues(Q);
+            for (int side = 0; side < 4; side++) {
+                int kind = 0;
+                if (!classify_side(Q, side, mx, my, mbc, meqn, side < 2 ? nx : ny, kind) ||
+                    (pass && kind != r.bc[side]))
+                    kind = FINESS_BC_HOST;
+                if (!pass || r.bc[side] != FINESS_BC_HOST) r.bc[side] = kind;
+            }
         }
 #else
         StateVars Q(0.0, mx, meqn, maux, mbc);
@@ -472,6 +506,14 @@ finess_ctx* context()
     g_ctx = nullptr;
     rc = finess_create(&p, &g_ctx);
     if (rc) fail_like_reference(rc, "finess_create failed");
+#if NDIMS == 2
+    if (p.bc[0] == FINESS_BC_HOST || p.bc[1] == FINESS_BC_HOST || p.bc[2] == FINESS_BC_HOST ||
+        p.bc[3] == FINESS_BC_HOST) {
+        if (!&SetBndValues) throw std::logic_error("finess-hip: gpu.boundary = host needs the app's SetBndValues");
+        rc = finess_set_bnd_callback(g_ctx, bnd_trampoline, nullptr);
+        if (rc) fail_like_reference(rc, finess_last_error(g_ctx));
+    }
+#endif
     // ConSoln after every accepted step, reduced on the device (without a
     // capacity function; with one, con_soln_frame writes a line per call)
     if (global_ini_params.get_mcapa() < 1) {
@@ -580,6 +622,9 @@ void fin_solve(StateVars& Qnew, double tend, double dtv[], int ts)
     finess_ctx* c = context();
     std::vector<double> q = gather(Qnew.const_ref_q()), aux = gather(Qnew.const_ref_aux());
     double t = Qnew.get_t();
+#if NDIMS == 2
+    g_bnd_aux = aux;
+#endif
     const int rc = ts == FINESS_TS_LXW   ? finess_fin_solve_lxw(c, q.data(), aux.data(), &t, tend, dtv)
                    : ts == FINESS_TS_PIF ? finess_fin_solve_rk_pif(c, q.data(), aux.data(), &t, tend, dtv)
                                          : finess_fin_solve_rk(c, q.data(), aux.data(), &t, tend, dtv);

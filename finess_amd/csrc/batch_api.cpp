@@ -158,7 +158,8 @@ int finess_batch_create(const finess_params* base, int nprob, const int* mx, con
     if (rc) return rc;
     if (p.ndims != 2 || p.time_stepping_method != FINESS_TS_RK || p.time_order != 3 ||
         p.weno_version != FINESS_WENO_JS || p.power_param != 2.0 || p.routing != FINESS_ROUTE_NONE ||
-        p.space_order != 5 || p.global_alpha || p.source_term ||
+        p.space_order != 5 || p.global_alpha || p.source_term || p.bc[0] == FINESS_BC_HOST ||
+        p.bc[1] == FINESS_BC_HOST || p.bc[2] == FINESS_BC_HOST || p.bc[3] == FINESS_BC_HOST ||
         (p.physics != FINESS_PHYS_SHALLOW_WATER && p.physics != FINESS_PHYS_EULER))
         return FINESS_ERR_CONFIG;
     if (finess_device_count() < 1) return FINESS_ERR_NODEVICE;

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <cmath>
+#include <limits>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -96,6 +98,16 @@ struct finess_ctx {
     std::vector<hipEvent_t> tev;  // per-launch timing marks (run_fixed_steps)
     size_t tev_used = 0;
     bool timing = false;
+    // FINESS_BC_HOST sides: the host SetBndValues, the device ring and ghost
+    // strips (packed side after side), their host copies, and the time
+    // UpdateSoln gives each plane set (set before every stage)
+    bool hostbc = false;
+    finess_bnd_fn bnd_fn = nullptr;
+    void* bnd_user = nullptr;
+    double* edge = nullptr;  // [2][edge_numel]: interior ring, then the ghosts
+    std::vector<double> edge_h, bnd_q;
+    double buf_t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double step_t0 = 0.0;    // time of the state a step starts from
     double t = 0.0;
     int cur = 0;
     bool have_state = false;
@@ -306,6 +318,89 @@ Stage1DArgs stage1d_args(finess_ctx* c, int src, int qa, int qb, int out, double
     return a;
 }
 
+int alloc(finess_ctx* c, void** ptr, size_t bytes);
+int ensure_ghosted(finess_ctx* c);
+
+// Edge values of the ring (or of the ghost strips): 2 meqn g (mx + my).
+size_t edge_numel(const finess_params& p, int M) { return 2 * (size_t)M * p.mbc * ((size_t)p.mx + p.my); }
+
+// SetBndValues on the host for the FINESS_BC_HOST sides (lib/2d/FinSolveRK.cpp:
+// 171-206 calls it before every stage), then the stage's ghosted source in
+// c->ghosted: the callback sees the interior cells within mbc of the boundary
+// (the rest NaN) in the reference layout, at time t; the edge ghosts it writes
+// on those sides are uploaded, and the other sides keep the device rule.
+int host_boundary(finess_ctx* c, int src, double t)
+{
+    const finess_params& p = c->p;
+    if (!c->bnd_fn) return fail(c, FINESS_ERR_CONFIG, "a FINESS_BC_HOST side needs finess_set_bnd_callback");
+    const int g = p.mbc, M = c->M, mx = p.mx, my = p.my, ld = my + 2 * g;
+    const size_t ne = edge_numel(p, M), nx = (size_t)M * g * my, ny = (size_t)M * g * mx;
+    int rc;
+    if (!c->edge && (rc = alloc(c, (void**)&c->edge, 2 * ne * sizeof(double)))) return rc;
+    if ((rc = ensure_ghosted(c))) return rc;
+    if (launch_ring_pack(c->buf[src], c->edge, mx, my, M, g, c->st))
+        return fail(c, FINESS_ERR_HIP, "edge ring launch failed");
+    c->edge_h.resize(2 * ne);
+    HIPCHK(c, hipMemcpyAsync(c->edge_h.data(), c->edge, ne * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->bnd_q.assign(ref_numel(p, M), std::numeric_limits<double>::quiet_NaN());
+    // the reference-layout index of edge slot (side, m, layer, along): for
+    // the ring (ghost = false) the l-th interior cell from the side, for the
+    // ghosts the l-th cell beyond it
+    auto ref_at = [&](int side, int m, int l, int a, bool ghost) -> size_t {
+        const int in = ghost ? -1 - l : l;
+        const int i = side == 0 ? in : side == 1 ? mx - 1 - in : a;
+        const int j = side == 2 ? in : side == 3 ? my - 1 - in : a;
+        return ((size_t)(i + g) * ld + (size_t)(j + g)) * M + m;
+    };
+    for (int side = 0; side < 4; side++) {
+        const int n = side < 2 ? my : mx;
+        const size_t off = side < 2 ? side * nx : 2 * nx + (side - 2) * ny;
+        for (int m = 0; m < M; m++)
+            for (int l = 0; l < g; l++)
+                for (int a = 0; a < n; a++)
+                    c->bnd_q[ref_at(side, m, l, a, false)] = c->edge_h[off + ((size_t)m * g + l) * n + a];
+    }
+    if (c->bnd_fn(c->bnd_user, t, c->bnd_q.data()))
+        return fail(c, FINESS_ERR_CONFIG, "the boundary callback returned an error");
+    double* gh = c->edge_h.data() + ne;
+    for (int side = 0; side < 4; side++) {
+        const int n = side < 2 ? my : mx;
+        const size_t off = side < 2 ? side * nx : 2 * nx + (side - 2) * ny;
+        for (int m = 0; m < M; m++)
+            for (int l = 0; l < g; l++)
+                for (int a = 0; a < n; a++) {
+                    double v = 0.0;
+                    if (p.bc[side] == FINESS_BC_HOST) {
+                        v = c->bnd_q[ref_at(side, m, l, a, true)];
+                        if (!std::isfinite(v))
+                            return fail(c, FINESS_ERR_CONFIG,
+                                        "the boundary callback left a non-finite ghost value (a rule reading cells "
+                                        "deeper than mbc from the boundary is not supported)");
+                    }
+                    gh[off + ((size_t)m * g + l) * n + a] = v;
+                }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->edge + ne, gh, ne * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (launch_ghost_fill(c->buf[src], c->edge + ne, c->ghosted, mx, my, M, g, p.bc, normal_x(p), normal_y(p), c->st))
+        return fail(c, FINESS_ERR_HIP, "ghost fill launch failed");
+    // the upload reads the host copy: it must land before the next stage
+    // rewrites that copy
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+// UpdateSoln's time, alpha1 tstar + alpha2 tnew + beta dt
+// (lib/2d/UpdateSoln.cpp:27), rounded after every operation as the
+// reference's build does (no fused multiply-add)
+double update_time(double a1, double ta, bool use_b, double a2, double tb, double beta, double dt)
+{
+#pragma clang fp contract(off)
+    double tn = a1 * ta;
+    if (use_b) tn = tn + a2 * tb;
+    return tn + beta * dt;
+}
+
 // One stage: out = a1*qa + a2*qb + (beta*dt) L(src).  slot: CFL slot.
 int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2, double beta, double dt, int slot)
 {
@@ -341,8 +436,17 @@ int stage(finess_ctx* c, int src, int qa, int qb, int out, double a1, double a2,
         a.lxw_F = c->lxwF;
         a.lxw_G = c->lxwG;
     }
-    if (tmark(c)) return FINESS_ERR_HIP;
     int rc;
+    if (c->hostbc) {
+        // the stage reads its ghosted source; the source's time is the one
+        // UpdateSoln formed (alpha1 tstar + alpha2 tnew + beta dt,
+        // lib/2d/UpdateSoln.cpp:20-27)
+        if ((rc = host_boundary(c, src, c->buf_t[src]))) return rc;
+        a.src = c->ghosted;
+        a.src_ghosts = p.mbc;
+        c->buf_t[out] = update_time(a1, c->buf_t[qa], qb >= 0, a2, qb >= 0 ? c->buf_t[qb] : 0.0, beta, dt);
+    }
+    if (tmark(c)) return FINESS_ERR_HIP;
     if (p.routing != FINESS_ROUTE_NONE) {
         rc = stage_routed(c, a, true);
         if (!rc && tmark(c)) return FINESS_ERR_HIP;
@@ -528,6 +632,7 @@ int enqueue_step(finess_ctx* c, double dt, int* res, int* slot)
     if (rc) return fail(c, FINESS_ERR_HIP, "slot reset failed");
     if (c->rec_age >= 0 && c->route_fresh == false) c->rec_age++;  // the previous step used the record
     c->route_fresh = true;
+    c->buf_t[A] = c->step_t0;
     if (p.time_stepping_method == FINESS_TS_PIF) {
         if ((rc = pif_step(c, A, B, C, dt))) return rc;
         *res = C;
@@ -897,8 +1002,16 @@ int finess_params_finalize(finess_params* p)
         (p->mx > 65535 || p->my > 65535 ||
          (long long)(p->mx + 1) * p->my + (long long)p->mx * (p->my + 1) >= (1ll << 31)))
         return FINESS_ERR_CONFIG;
-    for (int k = 0; k < 4; k++)
-        if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_REFLECT) return FINESS_ERR_CONFIG;
+    bool host = false;
+    for (int k = 0; k < 4; k++) {
+        if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_HOST) return FINESS_ERR_CONFIG;
+        host = host || p->bc[k] == FINESS_BC_HOST;
+    }
+    // a host SetBndValues runs before every stage of the unrouted 2D
+    // Runge-Kutta schemes with the SSP schedule (orders 1-3)
+    if (host && (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->time_order > 3 ||
+                 p->routing != FINESS_ROUTE_NONE))
+        return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }
 
@@ -1017,6 +1130,7 @@ int finess_create(const finess_params* pin, finess_ctx** out)
             if (!rc) rc = alloc(c, (void**)&c->mpp_gmax, 2 * CFL_SPREAD * sizeof(unsigned long long));
         }
     }
+    for (int k = 0; k < 4; k++) c->hostbc = c->hostbc || p.bc[k] == FINESS_BC_HOST;
     if (!rc && p.global_alpha) {
         rc = alloc(c, (void**)&c->galpha, 2 * sizeof(double));
         if (!rc) rc = alloc(c, (void**)&c->gspread, 2 * CFL_SPREAD * sizeof(unsigned long long));
@@ -1072,6 +1186,7 @@ void finess_destroy(finess_ctx* c)
     hipFree(c->wide_fy);
     hipFree(c->galpha);
     hipFree(c->gspread);
+    hipFree(c->edge);
     hipFree(c->mpp_fl);
     hipFree(c->mpp_gl);
     hipFree(c->mpp_amin);
@@ -1090,6 +1205,14 @@ void finess_destroy(finess_ctx* c)
 }
 
 const char* finess_last_error(const finess_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int finess_set_bnd_callback(finess_ctx* c, finess_bnd_fn fn, void* user)
+{
+    if (!c) return FINESS_ERR_ARG;
+    c->bnd_fn = fn;
+    c->bnd_user = user;
+    return FINESS_OK;
+}
 
 int finess_set_step_log(finess_ctx* c, int on)
 {
@@ -1606,7 +1729,7 @@ static int advance_dev(finess_ctx* c, double tend, double* dtv, long long fixed_
 static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed_steps, finess_stats* stats)
 {
     if (c && dtv && c->have_state && c->ctl && c->p.time_stepping_method == FINESS_TS_RK &&
-        c->p.time_order <= 3 && !c->p.verbosity && !c->p.host_controller)
+        c->p.time_order <= 3 && !c->p.verbosity && !c->p.host_controller && !c->hostbc)
         return advance_dev(c, tend, dtv, fixed_steps, stats);
     if (!c || !dtv) return FINESS_ERR_ARG;
     if (!c->have_state) return fail(c, FINESS_ERR_ARG, "no state");
@@ -1626,6 +1749,7 @@ static int advance_impl(finess_ctx* c, double tend, double* dtv, long long fixed
             if (!by_steps && told + dt > tend) dt = tend - told;
             t = told + dt;
             int res = 0, slot = 0;
+            c->step_t0 = told;
             if ((rc = enqueue_step(c, dt, &res, &slot))) break;
             if ((rc = read_cfl(c, slot, dt, dtv[2], &cfl))) break;
             if (cfl > 0.0) {
@@ -1717,6 +1841,7 @@ int finess_run_fixed_steps(finess_ctx* c, double dt, int nsteps, double* last_cf
     c->timing = true;
     c->tev_used = 0;
     for (int n = 0; n < nsteps; n++) {
+        c->step_t0 = c->t;
         if ((rc = enqueue_step(c, dt, &res, &slot))) { c->timing = false; return rc; }
         c->cur = res;
         c->t += dt;

@@ -99,5 +99,10 @@ int launch_step_control(StepCtl* c, unsigned long long* smax, const StepCtlParam
 int launch_ref_to_planes(const double* src, double* dst, int mx, int my, int meqn, int g, int dg, hipStream_t st);
 int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meqn, int g, const int bc[4],
                          int nx_mom, int ny_mom, hipStream_t st);
+// BC_HOST exchange: the interior edge ring of the planes, packed; the
+// ghosted planes with BC_HOST edge ghosts from packed strips
+int launch_ring_pack(const double* src, double* ring, int mx, int my, int meqn, int g, hipStream_t st);
+int launch_ghost_fill(const double* src, const double* ghost, double* dst, int mx, int my, int meqn, int g,
+                      const int bc[4], int nx_mom, int ny_mom, hipStream_t st);
 
 } // namespace finess

@@ -287,4 +287,98 @@ int launch_planes_to_ref(const double* src, double* dst, int mx, int my, int meq
     return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// Edge strips exchanged with a host SetBndValues (BC_HOST sides), packed side
+// after side: left and right hold meqn x g x my values (field, layer, j),
+// bottom and top meqn x g x mx (field, layer, i).  Layer l is the l-th cell
+// from the boundary: inward for the interior ring the host reads, outward for
+// the ghosts it returns.
+__host__ __device__ __forceinline__ void edge_slot(long long k, int mx, int my, int meqn, int g, int& side, int& m,
+                                                   int& l, int& along)
+{
+    const long long nx = (long long)meqn * g * my, ny = (long long)meqn * g * mx;
+    long long r;
+    if (k < 2 * nx) {
+        side = (int)(k / nx);
+        r = k - side * nx;
+    } else {
+        r = k - 2 * nx;
+        side = 2 + (int)(r / ny);
+        r -= (side - 2) * ny;
+    }
+    const int n = side < 2 ? my : mx;
+    m = (int)(r / ((long long)g * n));
+    r -= (long long)m * g * n;
+    l = (int)(r / n);
+    along = (int)(r - (long long)l * n);
+}
+
+__global__ void k_ring_pack(const double* __restrict__ src, double* __restrict__ ring, int mx, int my, int meqn, int g)
+{
+    const long long n = 2ll * meqn * g * (mx + my);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int side, m, l, a;
+    edge_slot(k, mx, my, meqn, g, side, m, l, a);
+    const int i = side == 0 ? l : side == 1 ? mx - 1 - l : a;
+    const int j = side == 2 ? l : side == 3 ? my - 1 - l : a;
+    ring[k] = src[(long long)m * mx * my + (long long)i * my + j];
+}
+
+int launch_ring_pack(const double* src, double* ring, int mx, int my, int meqn, int g, hipStream_t st)
+{
+    const long long n = 2ll * meqn * g * (mx + my);
+    hipLaunchKernelGGL(k_ring_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, ring, mx, my, meqn, g);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
+// Field planes with g ghosts (plane (mx+2g) x (my+2g), the stage kernels'
+// ghosted source): the interior from src, the edge ghosts of BC_HOST sides
+// from the packed strips, every other ghost through the boundary map.
+__global__ void k_ghost_fill(const double* __restrict__ src, const double* __restrict__ ghost, double* __restrict__ dst,
+                             int mx, int my, int meqn, int g, int bc0, int bc1, int bc2, int bc3, int nx_mom, int ny_mom)
+{
+    const int ld = my + 2 * g;
+    const long long pl = (long long)(mx + 2 * g) * ld;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= pl) return;
+    const int i = (int)(k / ld) - g, j = (int)(k - (long long)(k / ld) * ld) - g;
+    const bool oi = i < 0 || i >= mx, oj = j < 0 || j >= my;
+    int side = -1, l = 0, a = 0;
+    if (oi && !oj) {
+        side = i < 0 ? 0 : 1;
+        l = i < 0 ? -1 - i : i - mx;
+        a = j;
+    } else if (oj && !oi) {
+        side = j < 0 ? 2 : 3;
+        l = j < 0 ? -1 - j : j - my;
+        a = i;
+    }
+    const int bcs = side == 0 ? bc0 : side == 1 ? bc1 : side == 2 ? bc2 : bc3;
+    if (side >= 0 && bcs == BC_HOST) {
+        const long long nx = (long long)meqn * g * my, ny = (long long)meqn * g * mx;
+        const long long off = side < 2 ? side * nx : 2 * nx + (side - 2) * ny;
+        const int n = side < 2 ? my : mx;
+        for (int m = 0; m < meqn; m++) dst[m * pl + k] = ghost[off + ((long long)m * g + l) * n + a];
+        return;
+    }
+    int fi, fj;
+    const int si = bc_map(bc0, bc1, i, mx, fi);
+    const int sj = bc_map(bc2, bc3, j, my, fj);
+    const long long plane = (long long)mx * my;
+    for (int m = 0; m < meqn; m++) {
+        double v = src[m * plane + (long long)si * my + sj];
+        if ((fi && m == nx_mom) || (fj && m == ny_mom)) v = -v;
+        dst[m * pl + k] = v;
+    }
+}
+
+int launch_ghost_fill(const double* src, const double* ghost, double* dst, int mx, int my, int meqn, int g,
+                      const int bc[4], int nx_mom, int ny_mom, hipStream_t st)
+{
+    const long long n = (long long)(mx + 2 * g) * (my + 2 * g);
+    hipLaunchKernelGGL(k_ghost_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, ghost, dst, mx, my,
+                       meqn, g, bc[0], bc[1], bc[2], bc[3], nx_mom, ny_mom);
+    return hipGetLastError() == hipSuccess ? 0 : -4;
+}
+
 } // namespace finess

@@ -32,7 +32,10 @@
 namespace finess {
 
 // Boundary condition kinds (values equal the FINESS_BC_* macros).
-enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1, BC_REFLECT = 2 };
+// BC_HOST sides take their ghosts from a host callback: the stage then reads
+// a ghosted copy of its source, and the boundary map treats them as
+// extrapolation (it only forms that copy's corners, which no stencil reads).
+enum : int { BC_EXTRAP = 0, BC_PERIODIC = 1, BC_REFLECT = 2, BC_HOST = 3 };
 
 // Source index along an axis of n cells for a (possibly ghost) index g,
 // 0-based; *flip is set for a reflecting wall.  Valid for g in [-n, 2n-1].

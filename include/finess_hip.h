@@ -59,6 +59,12 @@ extern "C" {
 #define FINESS_BC_EXTRAP   0
 #define FINESS_BC_PERIODIC 1
 #define FINESS_BC_REFLECT  2
+/*   HOST     any other SetBndValues rule (time-dependent inflow such as
+ *            apps/2d/euler/mach_reflection/SetBndValues.cpp:14-146): the
+ *            ghost cells of the side come from the callback installed with
+ *            finess_set_bnd_callback, called on the host before every stage
+ *            (2D Runge-Kutta of orders 1-3, unrouted, host controller)      */
+#define FINESS_BC_HOST     3
 
 /* device forms of the app's SourceTermFunc (used when source_term = 1):
  *   ZERO           psi = 0: the SourceTermFunc of every reference app except
@@ -162,8 +168,23 @@ const char* finess_last_error(const finess_ctx* ctx);
 /* Uploads q (and aux, may be NULL when maux==0 or unused) with ghosts; t. */
 int finess_set_state(finess_ctx* ctx, const double* q, const double* aux, double t);
 /* Downloads the state into q (reference layout; ghost cells are filled by the
- * configured boundary condition, corners included). */
+ * configured boundary condition, corners included; FINESS_BC_HOST sides by
+ * extrapolation, the callback running only before stages). */
 int finess_get_state(finess_ctx* ctx, double* q, double* t);
+
+/* SetBndValues (lib/2d/FinSolveRK.cpp:171-206 calls it before every stage)
+ * for the sides whose bc is FINESS_BC_HOST.  Before each stage the library
+ * calls fn(user, t, q) on the host: q is a reference-layout array with ghosts
+ * ((mx+2mbc) x (my+2mbc) x meqn) whose interior cells within mbc of the
+ * boundary hold the stage's source state and whose every other value is NaN;
+ * t is the time UpdateSoln gives that state (lib/2d/UpdateSoln.cpp:20-27).  fn
+ * fills the ghost cells as the app's SetBndValues does and returns 0.  The
+ * library then takes the edge ghosts of the FINESS_BC_HOST sides (not the
+ * corners, which the Runge-Kutta stencils never read); a non-finite one, such as
+ * a value computed from a cell deeper than mbc, fails the step with
+ * FINESS_ERR_CONFIG, as does a nonzero return.  fn = NULL removes it. */
+typedef int (*finess_bnd_fn)(void* user, double t, double* q);
+int finess_set_bnd_callback(finess_ctx* ctx, finess_bnd_fn fn, void* user);
 
 /* ---- drop-in operators -------------------------------------------------- */
 

@@ -90,6 +90,8 @@ void SetBndValues(StateVars& Q)
     for (int k = 0; k < 4; k++) p.bc[k] = bc[k];
     orc_set_bnd(&p, q.data(), nullptr);
     if (g_bc == "timedep") q.set(0, 1, 1, q.get(0, 1, 1) + 1.0);  // not a copy of any cell
+    // extrapolation at t = 0, an inflow value later (a clock-driven rule)
+    if (g_bc == "clock" && Q.get_t() != 0.0) q.set(2, p.my + 1, 1, 1.0 + Q.get_t());
 }
 
 int main(int argc, char* argv[])

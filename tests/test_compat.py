@@ -46,9 +46,13 @@ def test_probe_identifies_boundaries(harness, bc, expect):
     assert "probe physics=1 checked=1 bc=%s bc_checked=1" % expect in out
 
 
-def test_probe_rejects_time_dependent_boundary(harness):
-    rc, out = run(harness, INI, "timedep")
-    assert rc == 3 and "SetBndValues side 0" in out
+@pytest.mark.parametrize("bc,expect", [("timedep", "3,0,0,0"), ("clock", "0,0,0,3")])
+def test_probe_sends_other_boundaries_to_the_host(harness, bc, expect):
+    """A side whose ghosts are no copy of a cell, or change between t = 0 and
+    t = 1, runs the app's SetBndValues on the host (FINESS_BC_HOST)."""
+    rc, out = run(harness, INI, bc)
+    assert rc == 0, out
+    assert "probe physics=1 checked=1 bc=%s bc_checked=1" % expect in out
 
 
 def test_probe_rejects_foreign_physics(harness):

@@ -69,5 +69,30 @@ def test_lxw_needs_five_ghosts():
     assert lib.finess_params_finalize(ctypes.byref(p)) == OK
 
 
+def test_host_boundary_scope():
+    """FINESS_BC_HOST (a host SetBndValues before every stage) is accepted for
+    the unrouted 2D Runge-Kutta schemes of orders 1-3 and refused elsewhere."""
+    lib = load_library()
+
+    def fin(**kw):
+        p = Params()
+        lib.finess_params_default(ctypes.byref(p))
+        p.bc[3] = finess_amd.BC_HOST
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return lib.finess_params_finalize(ctypes.byref(p))
+
+    assert fin() == OK
+    assert fin(time_order=1) == OK and fin(precision=finess_amd.PREC_FP32) == OK
+    assert fin(time_order=4) == ERR_CONFIG
+    assert fin(routing=finess_amd.ROUTE_ORACLE) == ERR_CONFIG
+    assert fin(time_stepping_method=finess_amd.TS_LXW, mbc=5) == ERR_CONFIG
+    assert fin(ndims=1, physics=finess_amd.PHYS_ADVECTION, meqn=1) == ERR_CONFIG
+    p = Params()
+    lib.finess_params_default(ctypes.byref(p))
+    p.bc[0] = 4
+    assert lib.finess_params_finalize(ctypes.byref(p)) == ERR_CONFIG
+
+
 def test_version_string():
     assert b"gfx950" in load_library().finess_version()

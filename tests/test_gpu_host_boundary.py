@@ -4,8 +4,9 @@ boundary rule runs on the host before every stage, as lib/2d/FinSolveRK.cpp:
 
 * a host callback that restates the device's reflecting and extrapolation
   rules gives bit for bit the run of those device rules (controlled SSP-RK3
-  and fixed steps at orders 1-3): the callback's ghosts reach the stencils
-  unchanged, on every side, layer and field;
+  and fixed steps at orders 1-3; the FP32 stencil form, global alpha and
+  seventh-order WENO with mbc = 4 at order 3): the callback's ghosts reach the
+  stencils unchanged, on every side, layer and field;
 * a time-dependent inflow rule of the kind of apps/2d/euler/mach_reflection/
   SetBndValues.cpp:14-146 (fixed inflow on the left, a reflecting wall right
   of x0 on the bottom, a shock-tracking top), stepped against the oracle's
@@ -17,7 +18,7 @@ import numpy as np
 import pytest
 
 import orc
-from finess_amd import BC_EXTRAP, BC_HOST, BC_REFLECT, ERR_CONFIG, FinessError, Solver, problems
+from finess_amd import BC_EXTRAP, BC_HOST, BC_REFLECT, ERR_CONFIG, PREC_FP32, FinessError, Solver, problems
 
 pytestmark = pytest.mark.gpu
 
@@ -58,11 +59,12 @@ def _device_rules(q, g, rules):
 RULES = [BC_REFLECT, BC_EXTRAP, BC_EXTRAP, BC_REFLECT]
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-def test_host_rule_equals_device_rule_fixed_steps(order):
-    p, _, _ = problems.riemann4(mx=40, my=36, bc=RULES, time_order=order)
+@pytest.mark.parametrize("order,kw", [(1, {}), (2, {}), (3, {}), (3, dict(precision=PREC_FP32)),
+                                      (3, dict(global_alpha=1)), (3, dict(space_order=7, mbc=4))])
+def test_host_rule_equals_device_rule_fixed_steps(order, kw):
+    p, _, _ = problems.riemann4(mx=40, my=36, bc=RULES, time_order=order, **kw)
     q = _state(p, order)
-    ph, _, _ = problems.riemann4(mx=40, my=36, bc=[BC_HOST] * 4, time_order=order)
+    ph, _, _ = problems.riemann4(mx=40, my=36, bc=[BC_HOST] * 4, time_order=order, **kw)
     calls = []
 
     def rule(t, qq):

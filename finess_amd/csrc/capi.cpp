@@ -1007,10 +1007,9 @@ int finess_params_finalize(finess_params* p)
         if (p->bc[k] < FINESS_BC_EXTRAP || p->bc[k] > FINESS_BC_HOST) return FINESS_ERR_CONFIG;
         host = host || p->bc[k] == FINESS_BC_HOST;
     }
-    // a host SetBndValues runs before every stage of the unrouted 2D
-    // Runge-Kutta schemes with the SSP schedule (orders 1-3)
-    if (host && (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->time_order > 3 ||
-                 p->routing != FINESS_ROUTE_NONE))
+    // a host SetBndValues runs before every stage of the 2D Runge-Kutta
+    // schemes with the SSP schedule (orders 1-3), plain or routed
+    if (host && (p->ndims != 2 || p->time_stepping_method != FINESS_TS_RK || p->time_order > 3))
         return FINESS_ERR_CONFIG;
     return FINESS_OK;
 }

@@ -63,7 +63,7 @@ extern "C" {
  *            apps/2d/euler/mach_reflection/SetBndValues.cpp:14-146): the
  *            ghost cells of the side come from the callback installed with
  *            finess_set_bnd_callback, called on the host before every stage
- *            (2D Runge-Kutta of orders 1-3, unrouted, host controller)      */
+ *            (2D Runge-Kutta of orders 1-3, plain or routed, host controller) */
 #define FINESS_BC_HOST     3
 
 /* device forms of the app's SourceTermFunc (used when source_term = 1):

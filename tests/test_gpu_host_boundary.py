@@ -5,8 +5,10 @@ boundary rule runs on the host before every stage, as lib/2d/FinSolveRK.cpp:
 * a host callback that restates the device's reflecting and extrapolation
   rules gives bit for bit the run of those device rules (controlled SSP-RK3
   and fixed steps at orders 1-3; the FP32 stencil form, global alpha and
-  seventh-order WENO with mbc = 4 at order 3): the callback's ghosts reach the
-  stencils unchanged, on every side, layer and field;
+  seventh-order WENO with mbc = 4 at order 3, and the routed stages with the
+  exact-departure indicator; the FP32 learned form agrees to FP32 rounding):
+  the callback's ghosts reach the stencils unchanged, on every side, layer and
+  field;
 * a time-dependent inflow rule of the kind of apps/2d/euler/mach_reflection/
   SetBndValues.cpp:14-146 (fixed inflow on the left, a reflecting wall right
   of x0 on the bottom, a shock-tracking top), stepped against the oracle's
@@ -18,7 +20,8 @@ import numpy as np
 import pytest
 
 import orc
-from finess_amd import BC_EXTRAP, BC_HOST, BC_REFLECT, ERR_CONFIG, PREC_FP32, FinessError, Solver, problems
+from finess_amd import (BC_EXTRAP, BC_HOST, BC_REFLECT, ERR_CONFIG, PREC_FP32, ROUTE_LEARNED, ROUTE_ORACLE, FinessError,
+                        Mlp, Solver, problems)
 
 pytestmark = pytest.mark.gpu
 
@@ -84,6 +87,53 @@ def test_host_rule_equals_device_rule_fixed_steps(order, kw):
     assert len(calls) == 5 * order
     assert np.array_equal(orc.interior(p, q1), orc.interior(p, q0))
     assert t1 == t0
+
+
+@pytest.mark.parametrize("kw", [dict(routing=ROUTE_ORACLE, route_tau=0.05, route_per_step=2),
+                                dict(routing=ROUTE_LEARNED, precision=PREC_FP32, route_per_step=4)])
+def test_host_rule_equals_device_rule_routed(kw):
+    """The routed stages (indicator record, troubled-face pass, lookup) read
+    the ghosted source like the plain stage: the same bits as the device rules
+    in FP64; the FP32 learned form to FP32 rounding."""
+    p, q, _ = problems.riemann4(mx=64, my=72, bc=RULES, **kw)
+    ph, _, _ = problems.riemann4(mx=64, my=72, bc=[BC_HOST] * 4, **kw)
+    net = None
+    if kw["routing"] == ROUTE_LEARNED:
+        # an untrained network flags faces at random: on smooth data both
+        # fluxes it chooses between stay stable
+        m = Mlp(seed=5, capacity=1024)
+        net = m.params
+        m.close()
+        q = _state(p, 7)
+    dt = 0.3 / 72 / 2.5
+    out = []
+    runs = [(p, None), (ph, lambda t, qq: _device_rules(qq, p.mbc, RULES))]
+    if kw["routing"] == ROUTE_LEARNED:
+        runs.append((p, None))
+    for pp, cb in runs:
+        with Solver(pp) as s:
+            if net is not None:
+                s.set_indicator(net, 0.0)
+            if cb:
+                s.set_bnd_callback(cb)
+            s.set_state(q, t=0.0)
+            print("run with", "host" if cb else "device", "rules", flush=True)
+            s.run_fixed_steps(dt, 6)
+            out.append((s.get_state()[0], s.route_stats()))
+    print("routed", kw["routing"], "troubled / faces", out[0][1], out[1][1])
+    if kw["routing"] == ROUTE_ORACLE:
+        assert 0 < out[0][1][0] < out[0][1][1]
+    assert out[0][1] == out[1][1]
+    if kw["routing"] == ROUTE_LEARNED:
+        # the FP32 routed form: the device rules' runs repeat bit for bit, and
+        # the host rule's run agrees to FP32 rounding (2.7e-7 relative measured;
+        # the FP32 form's 1e-5 bound), not bit for bit
+        assert np.array_equal(orc.interior(p, out[2][0]), orc.interior(p, out[0][0]))
+        err = orc.rel_linf(orc.interior(p, out[1][0]), orc.interior(p, out[0][0]))
+        print("learned FP32: host vs device rules rel err", err)
+        assert np.all(err <= 1e-5), err
+        return
+    assert np.array_equal(orc.interior(p, out[1][0]), orc.interior(p, out[0][0]))
 
 
 def test_host_rule_equals_device_rule_controlled():

@@ -71,7 +71,8 @@ def test_lxw_needs_five_ghosts():
 
 def test_host_boundary_scope():
     """FINESS_BC_HOST (a host SetBndValues before every stage) is accepted for
-    the unrouted 2D Runge-Kutta schemes of orders 1-3 and refused elsewhere."""
+    the 2D Runge-Kutta schemes of orders 1-3, routed or not, and refused
+    elsewhere."""
     lib = load_library()
 
     def fin(**kw):
@@ -85,7 +86,7 @@ def test_host_boundary_scope():
     assert fin() == OK
     assert fin(time_order=1) == OK and fin(precision=finess_amd.PREC_FP32) == OK
     assert fin(time_order=4) == ERR_CONFIG
-    assert fin(routing=finess_amd.ROUTE_ORACLE) == ERR_CONFIG
+    assert fin(routing=finess_amd.ROUTE_ORACLE) == OK
     assert fin(time_stepping_method=finess_amd.TS_LXW, mbc=5) == ERR_CONFIG
     assert fin(ndims=1, physics=finess_amd.PHYS_ADVECTION, meqn=1) == ERR_CONFIG
     p = Params()

@@ -352,25 +352,23 @@ __device__ __forceinline__ void route_face(const StageArgs& a, unsigned id, long
     R cv[NV][6];
     bool nz3 = false;  // a stencil cell with rho w != 0 (face_part's skip3)
     const bool inner = !a.src_ghosts && (DIR == 0 ? (gi >= 3 && gi + 2 < a.mx) : (gj >= 3 && gj + 2 < a.my));
-    if (inner) {
-        const int stride = DIR == 0 ? a.my : 1;
-        const int c0 = DIR == 0 ? (gi - 3) * a.my + gj : gi * a.my + (gj - 3);
-        for (int s = 0; s < 6; s++) {
-            double q[M], rec[NV];
+    const int stride = DIR == 0 ? a.my : 1;
+    const int c0 = DIR == 0 ? (gi - 3) * a.my + gj : gi * a.my + (gj - 3);
+    // one record formation for both kinds of stencil (only the loads
+    // differ), so that a face's flux has the same bits whichever way its
+    // cells were read
+    for (int s = 0; s < 6; s++) {
+        double q[M], rec[NV];
+        if (inner) {
             for (int m = 0; m < M; m++) q[m] = (a.src + m * a.plane)[c0 + s * stride];
-            if constexpr (M == 5) nz3 = nz3 || q[3] != 0.0;
-            P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
-            for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
+        } else if (DIR == 0) {
+            load_state<P>(a, gi - 3 + s, gj, q);
+        } else {
+            load_state<P>(a, gi, gj - 3 + s, q);
         }
-    } else {
-        for (int s = 0; s < 6; s++) {
-            double q[M], rec[NV];
-            if (DIR == 0) load_state<P>(a, gi - 3 + s, gj, q);
-            else load_state<P>(a, gi, gj - 3 + s, q);
-            if constexpr (M == 5) nz3 = nz3 || q[3] != 0.0;
-            P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
-            for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
-        }
+        if constexpr (M == 5) nz3 = nz3 || q[3] != 0.0;
+        P::template cell_record<sizeof(R) == 4>(q, a.pp, rec);
+        for (int v = 0; v < NV; v++) cv[v][s] = R(rec[v]);
     }
     const WenoParams<R> wp{R(a.eps), R(a.power)};
     R fh[M];

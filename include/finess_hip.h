@@ -180,7 +180,8 @@ int finess_get_state(finess_ctx* ctx, double* q, double* t);
  * t is the time UpdateSoln gives that state (lib/2d/UpdateSoln.cpp:20-27).  fn
  * fills the ghost cells as the app's SetBndValues does and returns 0.  The
  * library then takes the edge ghosts of the FINESS_BC_HOST sides (not the
- * corners, which the Runge-Kutta stencils never read); a non-finite one, such as
+ * corners, which the Runge-Kutta stencils never read, and nothing the callback
+ * writes into interior cells); a non-finite one, such as
  * a value computed from a cell deeper than mbc, fails the step with
  * FINESS_ERR_CONFIG, as does a nonzero return.  fn = NULL removes it. */
 typedef int (*finess_bnd_fn)(void* user, double t, double* q);

@@ -90,6 +90,9 @@ extern "C" int finess_construct_l_planes_noalloc(const finess_params* pin, const
     if (p.ndims != 2 || p.space_order != 5 || p.routing != FINESS_ROUTE_NONE) return FINESS_ERR_CONFIG;
     // the bathymetry source term reads aux planes, which this entry point does not take
     if (p.source_term && p.source_kind == FINESS_SRC_SW_BATHYMETRY) return FINESS_ERR_CONFIG;
+    // the planes carry no ghosts: a host-side boundary rule has nowhere to go
+    for (int k = 0; k < 4; k++)
+        if (p.bc[k] == FINESS_BC_HOST) return FINESS_ERR_CONFIG;
     // face and cell indices of the stage kernel are 32-bit
     if ((long long)(p.mx + 1) * (p.my + 1) >= (1ll << 31)) return FINESS_ERR_CONFIG;
     int dev = 0;

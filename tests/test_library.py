@@ -93,6 +93,11 @@ def test_host_boundary_scope():
     lib.finess_params_default(ctypes.byref(p))
     p.bc[0] = 4
     assert lib.finess_params_finalize(ctypes.byref(p)) == ERR_CONFIG
+    # the no-alloc entry has no ghosts to take a host rule's values: refused
+    # before any device call
+    p.bc[0] = finess_amd.BC_HOST
+    dummy = ctypes.c_void_p(ctypes.addressof(ctypes.c_double()))  # never dereferenced: the check comes first
+    assert lib.finess_construct_l_planes_noalloc(ctypes.byref(p), dummy, dummy, dummy, None) == ERR_CONFIG
 
 
 def test_version_string():

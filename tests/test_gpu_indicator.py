@@ -168,7 +168,7 @@ def test_routed_stepping_matches_oracle(per_step, rho_w):
     assert np.all(err <= 1e-10), err
 
 
-@pytest.mark.xfail(strict=False, reason="gave a NaN state once in six GPU runs (the other five and every "
+@pytest.mark.xfail(strict=False, reason="gave a NaN state once in seven GPU runs (the other six and every "
                    "smaller routed test matched the oracle to 1e-10); cause not found yet, so it reports "
                    "XFAIL/XPASS instead of stopping the suite")
 def test_routed_stepping_all_troubled_large():
